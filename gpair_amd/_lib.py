@@ -1,0 +1,105 @@
+"""ctypes binding of libgpair_hip.so (include/gpair_hip.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgpair_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "gpair_hip.h")
+
+_vp = C.c_void_p
+_i = C.c_int
+_d = C.c_double
+_pd = C.POINTER(C.c_double)
+_pi = C.POINTER(C.c_int)
+
+
+class CgParams(C.Structure):
+    _fields_ = [("alpha", _d), ("method", _i), ("max_iter", _i), ("chi2r_target", _d), ("ls_max_evals", _i),
+                ("minvalue", _d), ("wolfe1", _d), ("wolfe2", _d), ("step_max", _d), ("verbose", _i)]
+
+
+class CgResult(C.Structure):
+    _fields_ = [("iterations", _i), ("criterion_evals", _i), ("grad_evals", _i), ("J", _d), ("chi2", _d),
+                ("entropy", _d), ("chi2r", _d), ("converged", _i), ("status", _i)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "gpair_strerror": (C.c_char_p, [_i]),
+    "gpair_last_error": (C.c_char_p, [_vp]),
+    "gpu_init": (_i, [C.POINTER(_vp), _i]),
+    "gpu_copy_data": (_i, [_vp, _vp, _vp, _i, _i, _vp, _i, _i, _vp, _vp, _i, _vp, _i, _i]),
+    "gpu_copy_image": (_i, [_vp, _vp, _i, _i]),
+    "gpu_copy_dft": (_i, [_vp, _vp, _vp, _i]),
+    "gpair_set_uv": (_i, [_vp, _vp, _vp, _i, C.c_float]),
+    "gpu_build_kernels": (_i, [_vp, _i, _i, _i]),
+    "gpu_cleanup": (_i, [_vp]),
+    "gpu_shutdown": (_i, [_vp]),
+    "gpair_set_precision": (_i, [_vp, _i]),
+    "gpu_getp_ci": (_vp, [_vp]),
+    "gpu_getp_ti": (_vp, [_vp]),
+    "gpu_getp_fgn": (_vp, [_vp]),
+    "gpu_getp_fg": (_vp, [_vp]),
+    "gpu_getp_dd": (_vp, [_vp]),
+    "gpu_getp_tg": (_vp, [_vp]),
+    "gpu_getp_eg": (_vp, [_vp]),
+    "gpair_getp_dg": (_vp, [_vp]),
+    "gpu_get_chi2_curr": (_i, [_vp, _i, _i, _i, _i, _i, _pd]),
+    "gpu_get_chi2_temp": (_i, [_vp, _i, _i, _i, _i, _i, _pd]),
+    "gpu_get_entropy_curr": (_i, [_vp, _i, _pd]),
+    "gpu_get_entropy_temp": (_i, [_vp, _i, _pd]),
+    "gpu_compute_data_gradient_curr": (_i, [_vp, _i, _i, _i, _i]),
+    "gpu_compute_data_gradient_temp": (_i, [_vp, _i, _i, _i, _i]),
+    "gpu_compute_entropy_gradient_curr": (_i, [_vp, _i]),
+    "gpu_compute_entropy_gradient_temp": (_i, [_vp, _i]),
+    "gpu_compute_criterion_gradient_curr": (_i, [_vp, _i, _d]),
+    "gpu_compute_criterion_gradient_temp": (_i, [_vp, _i, _d]),
+    "gpu_get_scalprod": (_i, [_vp, _i, _i, _vp, _vp, _pd]),
+    "gpu_compute_descent_dir": (_i, [_vp, _i, _d]),
+    "gpu_update_tempimage": (_i, [_vp, _i, _d, _d, _vp]),
+    "gpu_update_image": (_i, [_vp, _i, _d, _d, _vp]),
+    "gpu_backup_gradient": (_i, [_vp, _i, _vp, _vp]),
+    "gpu_linesearch_zoom": (_i, [_vp, _i, _i, _i, _i, _i, _i, _d, _d, _d, _d, _d, _pi, _pi, _vp, _vp, _d, _i, _pd]),
+    "gpu_get_image": (_i, [_vp, _i, _vp, _vp]),
+    "gpair_get_vis": (_i, [_vp, _vp, _i]),
+    "gpair_get_mock": (_i, [_vp, _vp, _i]),
+    "gpair_put_image": (_i, [_vp, _vp, _vp, _i]),
+    "gpair_cg_default_params": (None, [C.POINTER(CgParams)]),
+    "gpair_cg_run": (_i, [_vp, C.POINTER(CgParams), C.POINTER(CgResult), _vp, _i]),
+    "gpair_sizes": (_i, [_vp, _pi, _pi, _pi, _pi]),
+    "gpair_sync": (_i, [_vp]),
+}
+
+_LIB = None
+
+
+def load(path: str = LIB_PATH):
+    """Load and type the library; raises when it is missing."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run `python -m gpair_amd.build` (there is no CPU fallback)")
+    L = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def header_symbols(path: str = HEADER):
+    """function names declared in include/gpair_hip.h"""
+    import re
+    txt = open(path).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt)
+    skip = {"if", "defined"}
+    return sorted({n for n in names if n.startswith(("gpu_", "gpair_")) and n not in skip})

@@ -1,0 +1,224 @@
+"""Python host side over the C ABI.
+
+``GpairContext`` keeps the reference's call surface (gpair1/trunk/gpu.h): the
+methods carry the reference names (``gpu_get_chi2_curr``, ``gpu_linesearch_zoom``,
+...) with the same argument meaning; errors raise ``GpairError`` instead of the
+reference's exit(0) (gpair1/trunk/gpu.c:186-202).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+PREC_FP32 = 0
+PREC_BF16X3 = 1
+
+
+class GpairError(RuntimeError):
+    pass
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class GpairContext:
+    def __init__(self, device: int = 0):
+        self.L = _lib.load()
+        h = C.c_void_p()
+        rc = self.L.gpu_init(C.byref(h), device)
+        if rc != 0:
+            raise GpairError(f"gpu_init: {self.L.gpair_strerror(rc).decode()}")
+        self.h = h
+        self.W = self.nuv = self.npow = self.nbis = 0
+
+    # -- plumbing -----------------------------------------------------------
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.L.gpair_last_error(self.h)
+            raise GpairError(f"{what}: {self.L.gpair_strerror(rc).decode()}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.h:
+            self.L.gpu_cleanup(self.h)
+            self.h = None
+
+    gpu_cleanup = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ndof(self):
+        return self.npow + 2 * self.nbis
+
+    # -- setup (gpair1/trunk/gpu.c:1175-1451) ----------------------------------
+    def gpu_copy_data(self, data, data_err, nuv, phasor, npow, uvpnt4, sign4, default_model, W):
+        data = np.ascontiguousarray(data, np.float32)
+        err = np.ascontiguousarray(data_err, np.float32)
+        ph = np.ascontiguousarray(np.asarray(phasor, np.complex64)).view(np.float32)
+        uv4 = np.ascontiguousarray(uvpnt4, np.int64)
+        s4 = np.ascontiguousarray(sign4, np.int16)
+        model = np.ascontiguousarray(default_model, np.float32)
+        nbis = uv4.shape[0]
+        self._check(self.L.gpu_copy_data(self.h, _ptr(data), _ptr(err), data.size, int(nuv), _ptr(ph), nbis,
+                                         int(npow), _ptr(uv4), _ptr(s4), nbis, _ptr(model), model.size, int(W)),
+                    "gpu_copy_data")
+        self.W, self.nuv, self.npow, self.nbis = int(W), int(nuv), int(npow), int(nbis)
+
+    def gpu_copy_image(self, image):
+        img = np.ascontiguousarray(image, np.float32)
+        self._check(self.L.gpu_copy_image(self.h, _ptr(img), self.W, self.W), "gpu_copy_image")
+
+    def gpair_set_uv(self, u, v, pixellation):
+        u = np.ascontiguousarray(u, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        self._check(self.L.gpair_set_uv(self.h, _ptr(u), _ptr(v), u.size, float(pixellation)), "gpair_set_uv")
+
+    def gpu_copy_dft(self, dft_x, dft_y):
+        """tables in the reference GPU layout, complex [W][nuv]."""
+        x = np.ascontiguousarray(np.asarray(dft_x, np.complex64))
+        y = np.ascontiguousarray(np.asarray(dft_y, np.complex64))
+        self._check(self.L.gpu_copy_dft(self.h, _ptr(x), _ptr(y), x.size), "gpu_copy_dft")
+
+    def gpu_build_kernels(self):
+        self._check(self.L.gpu_build_kernels(self.h, self.ndof, self.W, self.W * self.W), "gpu_build_kernels")
+
+    def set_precision(self, prec: int):
+        self._check(self.L.gpair_set_precision(self.h, int(prec)), "gpair_set_precision")
+
+    @classmethod
+    def from_problem(cls, prob, device: int = 0, image=None):
+        """full set-up from a gpair_amd.synth.Problem (tables built on the device)."""
+        ctx = cls(device)
+        ctx.gpu_copy_data(prob.data, prob.err, prob.nuv, prob.phasor, prob.npow, prob.uvpnt4(), prob.sign4(),
+                          prob.model, prob.W)
+        ctx.gpair_set_uv(prob.u, prob.v, prob.pixellation)
+        ctx.gpu_build_kernels()
+        ctx.gpu_copy_image(prob.start if image is None else image)
+        return ctx
+
+    # -- buffers --------------------------------------------------------------
+    def buf(self, name):
+        f = {"ci": self.L.gpu_getp_ci, "ti": self.L.gpu_getp_ti, "fgn": self.L.gpu_getp_fgn,
+             "fg": self.L.gpu_getp_fg, "dd": self.L.gpu_getp_dd, "tg": self.L.gpu_getp_tg,
+             "eg": self.L.gpu_getp_eg, "dg": self.L.gpair_getp_dg}[name]
+        return C.c_void_p(f(self.h))
+
+    def get(self, name):
+        out = np.empty(self.W * self.W, np.float32)
+        self._check(self.L.gpu_get_image(self.h, out.size, _ptr(out), self.buf(name)), "gpu_get_image")
+        return out
+
+    def put(self, name, arr):
+        a = np.ascontiguousarray(arr, np.float32)
+        self._check(self.L.gpair_put_image(self.h, self.buf(name), _ptr(a), a.size), "gpair_put_image")
+
+    def vis(self):
+        out = np.empty(self.nuv, np.complex64)
+        self._check(self.L.gpair_get_vis(self.h, _ptr(out), self.nuv), "gpair_get_vis")
+        return out
+
+    def mock(self):
+        out = np.empty(self.ndof, np.float32)
+        self._check(self.L.gpair_get_mock(self.h, _ptr(out), self.ndof), "gpair_get_mock")
+        return out
+
+    # -- objective / gradients (gpair1/trunk/gpu.c:596-857, 1598-1802) ---------
+    def gpu_get_chi2_curr(self):
+        v = C.c_double()
+        self._check(self.L.gpu_get_chi2_curr(self.h, self.nuv, self.npow, self.nbis, self.ndof, self.nuv,
+                                             C.byref(v)), "gpu_get_chi2_curr")
+        return v.value
+
+    def gpu_get_chi2_temp(self):
+        v = C.c_double()
+        self._check(self.L.gpu_get_chi2_temp(self.h, self.nuv, self.npow, self.nbis, self.ndof, self.nuv,
+                                             C.byref(v)), "gpu_get_chi2_temp")
+        return v.value
+
+    def gpu_get_entropy_curr(self):
+        v = C.c_double()
+        self._check(self.L.gpu_get_entropy_curr(self.h, self.W, C.byref(v)), "gpu_get_entropy_curr")
+        return v.value
+
+    def gpu_get_entropy_temp(self):
+        v = C.c_double()
+        self._check(self.L.gpu_get_entropy_temp(self.h, self.W, C.byref(v)), "gpu_get_entropy_temp")
+        return v.value
+
+    def gpu_compute_data_gradient_curr(self):
+        self._check(self.L.gpu_compute_data_gradient_curr(self.h, self.nuv, self.npow, self.nbis, self.W),
+                    "gpu_compute_data_gradient_curr")
+
+    def gpu_compute_data_gradient_temp(self):
+        self._check(self.L.gpu_compute_data_gradient_temp(self.h, self.nuv, self.npow, self.nbis, self.W),
+                    "gpu_compute_data_gradient_temp")
+
+    def gpu_compute_entropy_gradient_curr(self):
+        self._check(self.L.gpu_compute_entropy_gradient_curr(self.h, self.W), "gpu_compute_entropy_gradient_curr")
+
+    def gpu_compute_entropy_gradient_temp(self):
+        self._check(self.L.gpu_compute_entropy_gradient_temp(self.h, self.W), "gpu_compute_entropy_gradient_temp")
+
+    def gpu_compute_criterion_gradient_curr(self, alpha):
+        self._check(self.L.gpu_compute_criterion_gradient_curr(self.h, self.W, float(alpha)),
+                    "gpu_compute_criterion_gradient_curr")
+
+    def gpu_compute_criterion_gradient_temp(self, alpha):
+        self._check(self.L.gpu_compute_criterion_gradient_temp(self.h, self.W, float(alpha)),
+                    "gpu_compute_criterion_gradient_temp")
+
+    # -- CG ---------------------------------------------------------------------
+    def gpu_get_scalprod(self, a, b):
+        v = C.c_double()
+        self._check(self.L.gpu_get_scalprod(self.h, self.W, self.W, self.buf(a), self.buf(b), C.byref(v)),
+                    "gpu_get_scalprod")
+        return v.value
+
+    def gpu_compute_descent_dir(self, beta):
+        self._check(self.L.gpu_compute_descent_dir(self.h, self.W, float(beta)), "gpu_compute_descent_dir")
+
+    def gpu_update_tempimage(self, step, minval=1e-8, d="dd"):
+        self._check(self.L.gpu_update_tempimage(self.h, self.W, float(step), float(minval), self.buf(d)),
+                    "gpu_update_tempimage")
+
+    def gpu_update_image(self, step, minval=1e-8, d="dd"):
+        self._check(self.L.gpu_update_image(self.h, self.W, float(step), float(minval), self.buf(d)),
+                    "gpu_update_image")
+
+    def gpu_backup_gradient(self, src="fgn", dst="fg"):
+        self._check(self.L.gpu_backup_gradient(self.h, self.W * self.W, self.buf(src), self.buf(dst)),
+                    "gpu_backup_gradient")
+
+    def gpu_linesearch_zoom(self, lo, hi, J_lo, w1, J_init, alpha, max_evals=-1):
+        ce, ge, sel = C.c_int(0), C.c_int(0), C.c_double()
+        self._check(self.L.gpu_linesearch_zoom(self.h, self.nuv, self.npow, self.nbis, self.ndof, self.nuv, self.W,
+                                               float(lo), float(hi), float(J_lo), float(w1), float(J_init),
+                                               C.byref(ce), C.byref(ge), self.buf("dd"), self.buf("tg"),
+                                               float(alpha), int(max_evals), C.byref(sel)), "gpu_linesearch_zoom")
+        return sel.value, ce.value, ge.value
+
+    # -- the loop -----------------------------------------------------------------
+    def cg_run(self, alpha, method=2, max_iter=10, chi2r_target=1.0, ls_max_evals=64, verbose=False):
+        p = _lib.CgParams()
+        self.L.gpair_cg_default_params(C.byref(p))
+        p.alpha, p.method, p.max_iter = float(alpha), int(method), int(max_iter)
+        p.chi2r_target, p.ls_max_evals, p.verbose = float(chi2r_target), int(ls_max_evals), int(verbose)
+        res = _lib.CgResult()
+        trace = np.zeros((max(max_iter, 1), 8))
+        rc = self.L.gpair_cg_run(self.h, C.byref(p), C.byref(res), _ptr(trace), trace.shape[0])
+        self._check(rc, "gpair_cg_run")
+        return res, trace[:res.iterations]

@@ -1,0 +1,791 @@
+// C ABI of the MEM-CG path (include/gpair_hip.h): context, buffers, launches.
+// Each entry point keeps the meaning of the reference's gpu_* function of the
+// same name (gpair1/trunk/gpu.c); the citation sits in the header.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/gpair_hip.h"
+#include "gpair_kernels.hip"
+
+// gpair1/trunk/getoifits.h:87-88
+#define REF_PI 3.14159265358979323
+#define REF_RPMAS ((REF_PI / 180.0) / 3600000.0)
+
+struct gpair_buf_s {
+    float *p;
+    int n;
+};
+
+struct gpair_ctx {
+    int device;
+    hipStream_t stream;
+    int W, n, nuv, npow, nbis, ndof;
+    int have_data, have_tables, built;
+    int precision;
+    // data
+    float *data, *err;
+    float2 *phasor;
+    int4 *tri;
+    int *csr_ptr, *csr_ent;
+    float2 *X, *Y;
+    // images
+    gpair_buf_s img, tmp, model, gdata, gent, gnew, gold, gtmp, dir;
+    float2 *vis, *coef;
+    float *mock;
+    // reductions
+    double *scal;
+    float *invflux;
+    double *part;
+    unsigned *ticket;
+    double *host_scal;   // pinned
+    // adjoint split-K partials
+    float *adj_part;
+    int adj_nsplit, adj_pper;
+    char msg[512];
+};
+
+static int fail(gpair_ctx *c, int code, const char *fmt, ...)
+{
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->msg, sizeof(c->msg), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail((ctx), GPAIR_E_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                                 \
+    } while (0)
+
+#define LAUNCHCHK(ctx) HIPCHK(ctx, hipGetLastError())
+
+extern "C" const char *gpair_strerror(int code)
+{
+    switch (code) {
+    case GPAIR_OK: return "ok";
+    case GPAIR_E_ARG: return "invalid argument";
+    case GPAIR_E_HIP: return "HIP runtime error";
+    case GPAIR_E_STATE: return "call out of order";
+    case GPAIR_E_NOMEM: return "device allocation failed";
+    case GPAIR_E_NUMERIC: return "non-finite value";
+    default: return "unknown error";
+    }
+}
+
+extern "C" const char *gpair_last_error(const gpair_ctx *ctx) { return ctx ? ctx->msg : "no context"; }
+
+template <typename T>
+static int dalloc(gpair_ctx *c, T **p, size_t count)
+{
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (count == 0)
+        count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(c, GPAIR_E_NOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    }
+    return GPAIR_OK;
+}
+
+template <typename T>
+static void dfree(T **p)
+{
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+}
+
+static int red_blocks(int n)
+{
+    int b = (n + RED_THREADS * 4 - 1) / (RED_THREADS * 4);
+    if (b < 1) b = 1;
+    if (b > 1024) b = 1024;
+    return b;
+}
+
+static inline int ew_blocks(int n) { return (n + 255) / 256; }
+
+// ---------------------------------------------------------------------------
+// setup / teardown
+// ---------------------------------------------------------------------------
+extern "C" int gpu_init(gpair_ctx **out, int device)
+{
+    if (!out)
+        return GPAIR_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return GPAIR_E_HIP;
+    if (device < 0 || device >= ndev)
+        return GPAIR_E_ARG;
+    gpair_ctx *c = (gpair_ctx *)calloc(1, sizeof(gpair_ctx));
+    if (!c)
+        return GPAIR_E_NOMEM;
+    c->device = device;
+    c->precision = GPAIR_PREC_FP32;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        free(c);
+        return GPAIR_E_HIP;
+    }
+    int rc;
+    if ((rc = dalloc(c, &c->scal, GPAIR_NSCAL)) || (rc = dalloc(c, &c->invflux, 4)) ||
+        (rc = dalloc(c, &c->part, (size_t)RED_MAX_BLOCKS * 4)) || (rc = dalloc(c, &c->ticket, 4))) {
+        gpu_cleanup(c);
+        return rc;
+    }
+    if (hipHostMalloc((void **)&c->host_scal, sizeof(double) * GPAIR_NSCAL, hipHostMallocDefault) != hipSuccess) {
+        gpu_cleanup(c);
+        return GPAIR_E_NOMEM;
+    }
+    if (hipMemsetAsync(c->ticket, 0, 4 * sizeof(unsigned), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->scal, 0, GPAIR_NSCAL * sizeof(double), c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        gpu_cleanup(c);
+        return GPAIR_E_HIP;
+    }
+    *out = c;
+    return GPAIR_OK;
+}
+
+static void free_images(gpair_ctx *c)
+{
+    gpair_buf_s *bufs[] = {&c->img, &c->tmp, &c->model, &c->gdata, &c->gent, &c->gnew, &c->gold, &c->gtmp, &c->dir};
+    for (auto *b : bufs) {
+        dfree(&b->p);
+        b->n = 0;
+    }
+}
+
+extern "C" int gpu_cleanup(gpair_ctx *c)
+{
+    if (!c)
+        return GPAIR_E_ARG;
+    (void)hipSetDevice(c->device);
+    if (c->stream)
+        (void)hipStreamSynchronize(c->stream);
+    free_images(c);
+    dfree(&c->data); dfree(&c->err); dfree(&c->phasor); dfree(&c->tri); dfree(&c->csr_ptr); dfree(&c->csr_ent);
+    dfree(&c->X); dfree(&c->Y); dfree(&c->vis); dfree(&c->coef); dfree(&c->mock);
+    dfree(&c->scal); dfree(&c->invflux); dfree(&c->part); dfree(&c->ticket); dfree(&c->adj_part);
+    if (c->host_scal)
+        (void)hipHostFree(c->host_scal);
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+    free(c);
+    return GPAIR_OK;
+}
+
+extern "C" int gpu_shutdown(gpair_ctx *c) { return gpu_cleanup(c); }
+
+extern "C" int gpair_sizes(const gpair_ctx *c, int *W, int *nuv, int *npow, int *nbis)
+{
+    if (!c)
+        return GPAIR_E_ARG;
+    if (W) *W = c->W;
+    if (nuv) *nuv = c->nuv;
+    if (npow) *npow = c->npow;
+    if (nbis) *nbis = c->nbis;
+    return GPAIR_OK;
+}
+
+extern "C" int gpair_sync(gpair_ctx *c)
+{
+    if (!c)
+        return GPAIR_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GPAIR_OK;
+}
+
+static int check_width(gpair_ctx *c, int W)
+{
+    if (W < 64 || W > 1024 || W % 64 != 0)
+        return fail(c, GPAIR_E_ARG, "image width %d: must be a multiple of 64 in [64, 1024]", W);
+    int nb = W / 64;
+    if (nb != 1 && nb != 2 && nb != 3 && nb != 4 && nb != 8)
+        return fail(c, GPAIR_E_ARG, "image width %d: supported widths are 64, 128, 192, 256, 512", W);
+    return GPAIR_OK;
+}
+
+static int alloc_images(gpair_ctx *c, int W)
+{
+    int n = W * W, rc;
+    if (c->W == W && c->img.p)
+        return GPAIR_OK;
+    free_images(c);
+    gpair_buf_s *bufs[] = {&c->img, &c->tmp, &c->model, &c->gdata, &c->gent, &c->gnew, &c->gold, &c->gtmp, &c->dir};
+    for (auto *b : bufs) {
+        if ((rc = dalloc(c, &b->p, n)))
+            return rc;
+        b->n = n;
+        HIPCHK(c, hipMemsetAsync(b->p, 0, sizeof(float) * n, c->stream));
+    }
+    c->W = W;
+    c->n = n;
+    return GPAIR_OK;
+}
+
+extern "C" int gpu_copy_data(gpair_ctx *c, const float *data, const float *data_err, int data_size,
+                             int data_size_uv, const float *phasor, int phasor_size, int pow_size,
+                             const int64_t *uvpnt, const int16_t *sign, int bsref_size, const float *default_model,
+                             int image_size, int image_width)
+{
+    if (!c || !data || !data_err || !default_model || (bsref_size > 0 && (!phasor || !uvpnt || !sign)))
+        return fail(c, GPAIR_E_ARG, "gpu_copy_data: NULL pointer");
+    int rc;
+    if ((rc = check_width(c, image_width)))
+        return rc;
+    const int nuv = data_size_uv, npow = pow_size, nbis = bsref_size;
+    if (nuv <= 0 || npow < 0 || npow > nuv || nbis < 0 || phasor_size != nbis || data_size != npow + 2 * nbis ||
+        image_size != image_width * image_width)
+        return fail(c, GPAIR_E_ARG,
+                    "gpu_copy_data: inconsistent sizes (data %d, nuv %d, npow %d, nbis %d, phasor %d, image %d)",
+                    data_size, nuv, npow, nbis, phasor_size, image_size);
+    // pack T3 references: int4 (ab, bc, ca, conj bits) and the uv -> leg CSR
+    std::vector<int4> tri(nbis > 0 ? nbis : 1);
+    std::vector<int> cnt(nuv + 1, 0);
+    for (int t = 0; t < nbis; t++) {
+        int idx[3], bits = 0;
+        for (int j = 0; j < 3; j++) {
+            int64_t p = uvpnt[4 * t + j];
+            int s = sign[4 * t + j];
+            if (p < 0 || p >= nuv || (s != 1 && s != -1))
+                return fail(c, GPAIR_E_ARG, "gpu_copy_data: T3 %d leg %d: uv index %lld sign %d invalid", t, j,
+                            (long long)p, s);
+            idx[j] = (int)p;
+            if (s < 0)
+                bits |= 1 << j;
+            cnt[p + 1]++;
+        }
+        tri[t] = make_int4(idx[0], idx[1], idx[2], bits);
+    }
+    for (int p = 0; p < nuv; p++)
+        cnt[p + 1] += cnt[p];
+    std::vector<int> ent(3 * (size_t)nbis + 1), fill(cnt.begin(), cnt.end() - 1);
+    for (int t = 0; t < nbis; t++)
+        for (int j = 0; j < 3; j++)
+            ent[fill[uvpnt[4 * t + j]]++] = 3 * t + j;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int ndof = npow + 2 * nbis;
+    if ((rc = dalloc(c, &c->data, ndof)) || (rc = dalloc(c, &c->err, ndof)) || (rc = dalloc(c, &c->mock, ndof)) ||
+        (rc = dalloc(c, &c->phasor, nbis)) || (rc = dalloc(c, &c->tri, nbis)) ||
+        (rc = dalloc(c, &c->csr_ptr, nuv + 1)) || (rc = dalloc(c, &c->csr_ent, 3 * (size_t)nbis)) ||
+        (rc = dalloc(c, &c->vis, nuv)) || (rc = dalloc(c, &c->coef, nuv)))
+        return rc;
+    if ((rc = alloc_images(c, image_width)))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->data, data, sizeof(float) * ndof, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->err, data_err, sizeof(float) * ndof, hipMemcpyHostToDevice, c->stream));
+    if (nbis > 0) {
+        HIPCHK(c, hipMemcpyAsync(c->phasor, phasor, sizeof(float2) * nbis, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->tri, tri.data(), sizeof(int4) * nbis, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->csr_ent, ent.data(), sizeof(int) * 3 * (size_t)nbis, hipMemcpyHostToDevice,
+                                 c->stream));
+    }
+    HIPCHK(c, hipMemcpyAsync(c->csr_ptr, cnt.data(), sizeof(int) * (nuv + 1), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->model.p, default_model, sizeof(float) * image_size, hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemsetAsync(c->mock, 0, sizeof(float) * ndof, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->vis, 0, sizeof(float2) * nuv, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));   // host vectors go out of scope
+    if (c->nuv != nuv)
+        c->have_tables = 0;
+    c->nuv = nuv;
+    c->npow = npow;
+    c->nbis = nbis;
+    c->ndof = ndof;
+    c->have_data = 1;
+    c->built = 0;
+    return GPAIR_OK;
+}
+
+extern "C" int gpu_copy_image(gpair_ctx *c, const float *image, int x_size, int y_size)
+{
+    if (!c || !image)
+        return fail(c, GPAIR_E_ARG, "gpu_copy_image: NULL pointer");
+    if (!c->have_data)
+        return fail(c, GPAIR_E_STATE, "gpu_copy_image before gpu_copy_data");
+    if (x_size != c->W || y_size != c->W)
+        return fail(c, GPAIR_E_ARG, "gpu_copy_image: %dx%d, expected %dx%d", x_size, y_size, c->W, c->W);
+    HIPCHK(c, hipMemcpyAsync(c->img.p, image, sizeof(float) * c->n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GPAIR_OK;
+}
+
+extern "C" int gpair_set_uv(gpair_ctx *c, const float *u, const float *v, int nuv, float pixellation)
+{
+    if (!c || !u || !v)
+        return fail(c, GPAIR_E_ARG, "gpair_set_uv: NULL pointer");
+    if (!c->have_data)
+        return fail(c, GPAIR_E_STATE, "gpair_set_uv before gpu_copy_data");
+    if (nuv != c->nuv)
+        return fail(c, GPAIR_E_ARG, "gpair_set_uv: nuv %d, data has %d", nuv, c->nuv);
+    if (!(pixellation > 0.f) || !isfinite(pixellation))
+        return fail(c, GPAIR_E_ARG, "gpair_set_uv: pixellation %g", (double)pixellation);
+    int rc;
+    const size_t tsz = (size_t)nuv * c->W;
+    if ((rc = dalloc(c, &c->X, tsz)) || (rc = dalloc(c, &c->Y, tsz)))
+        return rc;
+    float *du = nullptr, *dv = nullptr;
+    if ((rc = dalloc(c, &du, nuv)) || (rc = dalloc(c, &dv, nuv))) {
+        dfree(&du);
+        return rc;
+    }
+    int err = GPAIR_OK;
+    if (hipMemcpyAsync(du, u, sizeof(float) * nuv, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(dv, v, sizeof(float) * nuv, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        err = fail(c, GPAIR_E_HIP, "gpair_set_uv: upload failed");
+    if (!err) {
+        const double a = REF_RPMAS * (double)pixellation;
+        k_tables<<<(unsigned)((tsz + 255) / 256), 256, 0, c->stream>>>(du, dv, a, nuv, c->W, c->X, c->Y);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+            err = fail(c, GPAIR_E_HIP, "gpair_set_uv: table kernel failed");
+    }
+    dfree(&du);
+    dfree(&dv);
+    if (!err)
+        c->have_tables = 1;
+    return err;
+}
+
+extern "C" int gpu_copy_dft(gpair_ctx *c, const float *dft_x, const float *dft_y, int dft_size)
+{
+    if (!c || !dft_x || !dft_y)
+        return fail(c, GPAIR_E_ARG, "gpu_copy_dft: NULL pointer");
+    if (!c->have_data)
+        return fail(c, GPAIR_E_STATE, "gpu_copy_dft before gpu_copy_data");
+    const size_t tsz = (size_t)c->nuv * c->W;
+    if ((size_t)dft_size < tsz)
+        return fail(c, GPAIR_E_ARG, "gpu_copy_dft: %d entries, need nuv*W = %zu", dft_size, tsz);
+    int rc;
+    if ((rc = dalloc(c, &c->X, tsz)) || (rc = dalloc(c, &c->Y, tsz)))
+        return rc;
+    float2 *tmp = nullptr;
+    if ((rc = dalloc(c, &tmp, tsz)))
+        return rc;
+    const float *src[2] = {dft_x, dft_y};
+    float2 *dst[2] = {c->X, c->Y};
+    int err = GPAIR_OK;
+    for (int k = 0; k < 2 && !err; k++) {
+        if (hipMemcpyAsync(tmp, src[k], sizeof(float2) * tsz, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+            err = fail(c, GPAIR_E_HIP, "gpu_copy_dft: upload failed");
+            break;
+        }
+        k_transpose_tables<<<(unsigned)((tsz + 255) / 256), 256, 0, c->stream>>>(tmp, c->nuv, c->W, dst[k]);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+            err = fail(c, GPAIR_E_HIP, "gpu_copy_dft: transpose failed");
+    }
+    dfree(&tmp);
+    if (!err)
+        c->have_tables = 1;
+    return err;
+}
+
+extern "C" int gpu_build_kernels(gpair_ctx *c, int data_size, int image_width, int image_size)
+{
+    if (!c)
+        return GPAIR_E_ARG;
+    if (!c->have_data || !c->have_tables)
+        return fail(c, GPAIR_E_STATE, "gpu_build_kernels needs data and DFT tables first");
+    if (data_size != c->ndof || image_width != c->W || image_size != c->n)
+        return fail(c, GPAIR_E_ARG, "gpu_build_kernels: sizes (%d, %d, %d) differ from the data (%d, %d, %d)",
+                    data_size, image_width, image_size, c->ndof, c->W, c->n);
+    // adjoint split over uv: about 1024 workgroups in total
+    const int tiles = (c->W / 64) * (c->W / 64);
+    int nsplit = 1024 / tiles;
+    if (nsplit < 1) nsplit = 1;
+    int maxsplit = (c->nuv + 7) / 8;
+    if (nsplit > maxsplit) nsplit = maxsplit;
+    int pper = (c->nuv + nsplit - 1) / nsplit;
+    pper = (pper + 7) / 8 * 8;
+    nsplit = (c->nuv + pper - 1) / pper;
+    int rc;
+    if ((rc = dalloc(c, &c->adj_part, (size_t)nsplit * c->n)))
+        return rc;
+    c->adj_nsplit = nsplit;
+    c->adj_pper = pper;
+    c->built = 1;
+    return GPAIR_OK;
+}
+
+extern "C" int gpair_set_precision(gpair_ctx *c, int precision)
+{
+    if (!c)
+        return GPAIR_E_ARG;
+    if (precision != GPAIR_PREC_FP32)
+        return fail(c, GPAIR_E_ARG, "precision %d not available", precision);
+    c->precision = precision;
+    return GPAIR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// buffer handles
+// ---------------------------------------------------------------------------
+extern "C" gpair_buf gpu_getp_ci(gpair_ctx *c) { return c ? &c->img : nullptr; }
+extern "C" gpair_buf gpu_getp_ti(gpair_ctx *c) { return c ? &c->tmp : nullptr; }
+extern "C" gpair_buf gpu_getp_fgn(gpair_ctx *c) { return c ? &c->gnew : nullptr; }
+extern "C" gpair_buf gpu_getp_fg(gpair_ctx *c) { return c ? &c->gold : nullptr; }
+extern "C" gpair_buf gpu_getp_dd(gpair_ctx *c) { return c ? &c->dir : nullptr; }
+extern "C" gpair_buf gpu_getp_tg(gpair_ctx *c) { return c ? &c->gtmp : nullptr; }
+extern "C" gpair_buf gpu_getp_eg(gpair_ctx *c) { return c ? &c->gent : nullptr; }
+extern "C" gpair_buf gpair_getp_dg(gpair_ctx *c) { return c ? &c->gdata : nullptr; }
+
+static int ready(gpair_ctx *c)
+{
+    if (!c)
+        return GPAIR_E_ARG;
+    if (!c->built)
+        return fail(c, GPAIR_E_STATE, "call gpu_copy_data, gpair_set_uv/gpu_copy_dft and gpu_build_kernels first");
+    return GPAIR_OK;
+}
+
+static int own_buf(gpair_ctx *c, gpair_buf b)
+{
+    gpair_buf_s *bufs[] = {&c->img, &c->tmp, &c->model, &c->gdata, &c->gent, &c->gnew, &c->gold, &c->gtmp, &c->dir};
+    for (auto *x : bufs)
+        if (x == b && b->p)
+            return 1;
+    return 0;
+}
+
+static int read_scalar(gpair_ctx *c, int slot, double *out)
+{
+    HIPCHK(c, hipMemcpyAsync(c->host_scal + slot, c->scal + slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = c->host_scal[slot];
+    return GPAIR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launches
+// ---------------------------------------------------------------------------
+static int launch_flux(gpair_ctx *c, const float *img)
+{
+    k_flux<<<red_blocks(c->n), RED_THREADS, 0, c->stream>>>(img, c->n, c->part, c->ticket, c->scal, c->invflux);
+    LAUNCHCHK(c);
+    return GPAIR_OK;
+}
+
+static int launch_forward(gpair_ctx *c, const float *img)
+{
+    const unsigned grid = (unsigned)((c->nuv + FWD_PT - 1) / FWD_PT);
+    switch (c->W / 64) {
+    case 1: k_vis_fwd_f32<1><<<grid, 256, 0, c->stream>>>(img, c->X, c->Y, c->nuv, c->invflux, c->vis); break;
+    case 2: k_vis_fwd_f32<2><<<grid, 256, 0, c->stream>>>(img, c->X, c->Y, c->nuv, c->invflux, c->vis); break;
+    case 3: k_vis_fwd_f32<3><<<grid, 256, 0, c->stream>>>(img, c->X, c->Y, c->nuv, c->invflux, c->vis); break;
+    case 4: k_vis_fwd_f32<4><<<grid, 256, 0, c->stream>>>(img, c->X, c->Y, c->nuv, c->invflux, c->vis); break;
+    case 8: k_vis_fwd_f32<8><<<grid, 256, 0, c->stream>>>(img, c->X, c->Y, c->nuv, c->invflux, c->vis); break;
+    default: return fail(c, GPAIR_E_ARG, "unsupported width %d", c->W);
+    }
+    LAUNCHCHK(c);
+    return GPAIR_OK;
+}
+
+static int launch_chi2(gpair_ctx *c, const float *img)
+{
+    int rc;
+    if ((rc = launch_flux(c, img)) || (rc = launch_forward(c, img)))
+        return rc;
+    const int items = c->npow + c->nbis;
+    k_model_chi2<<<red_blocks(items), RED_THREADS, 0, c->stream>>>(c->vis, c->data, c->err, c->phasor, c->tri,
+                                                                    c->npow, c->nbis, c->mock, c->part,
+                                                                    c->ticket, c->scal);
+    LAUNCHCHK(c);
+    return GPAIR_OK;
+}
+
+static int launch_data_gradient(gpair_ctx *c, const float *img)
+{
+    int rc;
+    if ((rc = launch_flux(c, img)))
+        return rc;
+    k_coef<<<red_blocks(c->nuv), RED_THREADS, 0, c->stream>>>(c->vis, c->mock, c->data, c->err, c->phasor, c->tri,
+                                                               c->csr_ptr, c->csr_ent, c->npow, c->nuv, c->coef,
+                                                               c->part, c->ticket, c->scal);
+    LAUNCHCHK(c);
+    dim3 grid((c->W / 64) * (c->W / 64), c->adj_nsplit);
+    k_adj_f32<<<grid, 256, 0, c->stream>>>(c->coef, c->X, c->Y, c->nuv, c->W, c->adj_pper, c->adj_part);
+    LAUNCHCHK(c);
+    k_grad_finish<<<ew_blocks(c->n), 256, 0, c->stream>>>(c->adj_part, c->adj_nsplit, c->n, c->scal, c->gdata.p);
+    LAUNCHCHK(c);
+    return GPAIR_OK;
+}
+
+static int check_counts(gpair_ctx *c, int nuv, int npow, int nbis)
+{
+    if (nuv != c->nuv || npow != c->npow || nbis != c->nbis)
+        return fail(c, GPAIR_E_ARG, "sizes (nuv %d, npow %d, nbis %d) differ from the data (%d, %d, %d)", nuv, npow,
+                    nbis, c->nuv, c->npow, c->nbis);
+    return GPAIR_OK;
+}
+
+static int get_chi2(gpair_ctx *c, const float *img, int nuv, int npow, int nbis, double *chi2)
+{
+    int rc;
+    if ((rc = ready(c)) || (rc = check_counts(c, nuv, npow, nbis)))
+        return rc;
+    if (!chi2)
+        return fail(c, GPAIR_E_ARG, "chi2 output is NULL");
+    if ((rc = launch_chi2(c, img)))
+        return rc;
+    return read_scalar(c, SC_CHI2, chi2);
+}
+
+extern "C" int gpu_get_chi2_curr(gpair_ctx *c, int nuv, int npow, int nbis, int data_alloc, int data_alloc_uv,
+                                 double *chi2)
+{
+    (void)data_alloc; (void)data_alloc_uv;
+    return get_chi2(c, c ? c->img.p : nullptr, nuv, npow, nbis, chi2);
+}
+
+extern "C" int gpu_get_chi2_temp(gpair_ctx *c, int nuv, int npow, int nbis, int data_alloc, int data_alloc_uv,
+                                 double *chi2)
+{
+    (void)data_alloc; (void)data_alloc_uv;
+    return get_chi2(c, c ? c->tmp.p : nullptr, nuv, npow, nbis, chi2);
+}
+
+static int get_entropy(gpair_ctx *c, const float *img, int W, double *S)
+{
+    int rc;
+    if ((rc = ready(c)))
+        return rc;
+    if (W != c->W || !S)
+        return fail(c, GPAIR_E_ARG, "entropy: width %d (expected %d) or NULL output", W, c->W);
+    k_entropy<<<red_blocks(c->n), RED_THREADS, 0, c->stream>>>(img, c->model.p, c->n, c->part, c->ticket, c->scal);
+    LAUNCHCHK(c);
+    return read_scalar(c, SC_ENT, S);
+}
+
+extern "C" int gpu_get_entropy_curr(gpair_ctx *c, int W, double *S) { return get_entropy(c, c ? c->img.p : nullptr, W, S); }
+extern "C" int gpu_get_entropy_temp(gpair_ctx *c, int W, double *S) { return get_entropy(c, c ? c->tmp.p : nullptr, W, S); }
+
+static int data_gradient(gpair_ctx *c, const float *img, int nuv, int npow, int nbis, int W)
+{
+    int rc;
+    if ((rc = ready(c)) || (rc = check_counts(c, nuv, npow, nbis)))
+        return rc;
+    if (W != c->W)
+        return fail(c, GPAIR_E_ARG, "data gradient: width %d, expected %d", W, c->W);
+    return launch_data_gradient(c, img);
+}
+
+extern "C" int gpu_compute_data_gradient_curr(gpair_ctx *c, int nuv, int npow, int nbis, int W)
+{
+    return data_gradient(c, c ? c->img.p : nullptr, nuv, npow, nbis, W);
+}
+
+extern "C" int gpu_compute_data_gradient_temp(gpair_ctx *c, int nuv, int npow, int nbis, int W)
+{
+    return data_gradient(c, c ? c->tmp.p : nullptr, nuv, npow, nbis, W);
+}
+
+static int entropy_gradient(gpair_ctx *c, const float *img, int W)
+{
+    int rc;
+    if ((rc = ready(c)))
+        return rc;
+    if (W != c->W)
+        return fail(c, GPAIR_E_ARG, "entropy gradient: width %d, expected %d", W, c->W);
+    k_entropy_grad<<<ew_blocks(c->n), 256, 0, c->stream>>>(img, c->model.p, c->n, c->gent.p);
+    LAUNCHCHK(c);
+    return GPAIR_OK;
+}
+
+extern "C" int gpu_compute_entropy_gradient_curr(gpair_ctx *c, int W) { return entropy_gradient(c, c ? c->img.p : nullptr, W); }
+extern "C" int gpu_compute_entropy_gradient_temp(gpair_ctx *c, int W) { return entropy_gradient(c, c ? c->tmp.p : nullptr, W); }
+
+static int criterion_gradient(gpair_ctx *c, float *out, int W, double alpha)
+{
+    int rc;
+    if ((rc = ready(c)))
+        return rc;
+    if (W != c->W)
+        return fail(c, GPAIR_E_ARG, "criterion gradient: width %d, expected %d", W, c->W);
+    k_crit_grad<<<ew_blocks(c->n), 256, 0, c->stream>>>(c->gdata.p, c->gent.p, (float)alpha, c->n, out);
+    LAUNCHCHK(c);
+    return GPAIR_OK;
+}
+
+extern "C" int gpu_compute_criterion_gradient_curr(gpair_ctx *c, int W, double alpha)
+{
+    return criterion_gradient(c, c ? c->gnew.p : nullptr, W, alpha);
+}
+
+extern "C" int gpu_compute_criterion_gradient_temp(gpair_ctx *c, int W, double alpha)
+{
+    return criterion_gradient(c, c ? c->gtmp.p : nullptr, W, alpha);
+}
+
+extern "C" int gpu_get_scalprod(gpair_ctx *c, int w, int h, gpair_buf a, gpair_buf b, double *out)
+{
+    int rc;
+    if ((rc = ready(c)))
+        return rc;
+    if (!own_buf(c, a) || !own_buf(c, b) || !out || w * h != c->n)
+        return fail(c, GPAIR_E_ARG, "gpu_get_scalprod: bad buffer, size %dx%d or NULL output", w, h);
+    k_dot<<<red_blocks(c->n), RED_THREADS, 0, c->stream>>>(a->p, b->p, c->n, c->part, c->ticket, c->scal + SC_DOT);
+    LAUNCHCHK(c);
+    return read_scalar(c, SC_DOT, out);
+}
+
+extern "C" int gpu_compute_descent_dir(gpair_ctx *c, int W, double beta)
+{
+    int rc;
+    if ((rc = ready(c)))
+        return rc;
+    if (W != c->W)
+        return fail(c, GPAIR_E_ARG, "descent dir: width %d, expected %d", W, c->W);
+    k_descent<<<ew_blocks(c->n), 256, 0, c->stream>>>(c->dir.p, c->gnew.p, (float)beta, c->n);
+    LAUNCHCHK(c);
+    return GPAIR_OK;
+}
+
+static int step(gpair_ctx *c, float *out, int W, double s, double minval, gpair_buf d)
+{
+    int rc;
+    if ((rc = ready(c)))
+        return rc;
+    if (W != c->W || !own_buf(c, d))
+        return fail(c, GPAIR_E_ARG, "image update: width %d or direction buffer invalid", W);
+    if (!isfinite(s))
+        return fail(c, GPAIR_E_NUMERIC, "image update: non-finite step %g", s);
+    k_step<<<ew_blocks(c->n), 256, 0, c->stream>>>(c->img.p, d->p, (float)s, (float)minval, c->n, out);
+    LAUNCHCHK(c);
+    return GPAIR_OK;
+}
+
+extern "C" int gpu_update_tempimage(gpair_ctx *c, int W, double s, double minval, gpair_buf d)
+{
+    return step(c, c ? c->tmp.p : nullptr, W, s, minval, d);
+}
+
+extern "C" int gpu_update_image(gpair_ctx *c, int W, double s, double minval, gpair_buf d)
+{
+    return step(c, c ? c->img.p : nullptr, W, s, minval, d);
+}
+
+extern "C" int gpu_backup_gradient(gpair_ctx *c, int n, gpair_buf in, gpair_buf out)
+{
+    if (!c)
+        return GPAIR_E_ARG;
+    if (!own_buf(c, in) || !own_buf(c, out) || n < 0 || n > in->n || n > out->n)
+        return fail(c, GPAIR_E_ARG, "gpu_backup_gradient: bad buffers or size %d", n);
+    HIPCHK(c, hipMemcpyAsync(out->p, in->p, sizeof(float) * n, hipMemcpyDeviceToDevice, c->stream));
+    return GPAIR_OK;
+}
+
+extern "C" int gpu_get_image(gpair_ctx *c, int size, float *host, gpair_buf which)
+{
+    if (!c || !host)
+        return fail(c, GPAIR_E_ARG, "gpu_get_image: NULL pointer");
+    if (!own_buf(c, which) || size < 0 || size > which->n)
+        return fail(c, GPAIR_E_ARG, "gpu_get_image: bad buffer or size %d", size);
+    HIPCHK(c, hipMemcpyAsync(host, which->p, sizeof(float) * size, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GPAIR_OK;
+}
+
+extern "C" int gpair_put_image(gpair_ctx *c, gpair_buf which, const float *host, int size)
+{
+    if (!c || !host)
+        return fail(c, GPAIR_E_ARG, "gpair_put_image: NULL pointer");
+    if (!own_buf(c, which) || size < 0 || size > which->n)
+        return fail(c, GPAIR_E_ARG, "gpair_put_image: bad buffer or size %d", size);
+    HIPCHK(c, hipMemcpyAsync(which->p, host, sizeof(float) * size, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GPAIR_OK;
+}
+
+extern "C" int gpair_get_vis(gpair_ctx *c, float *vis, int nuv)
+{
+    if (!c || !vis || nuv != c->nuv)
+        return fail(c, GPAIR_E_ARG, "gpair_get_vis: NULL pointer or nuv mismatch");
+    HIPCHK(c, hipMemcpyAsync(vis, c->vis, sizeof(float2) * nuv, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GPAIR_OK;
+}
+
+extern "C" int gpair_get_mock(gpair_ctx *c, float *mock, int ndata)
+{
+    if (!c || !mock || ndata != c->ndof)
+        return fail(c, GPAIR_E_ARG, "gpair_get_mock: NULL pointer or size mismatch");
+    HIPCHK(c, hipMemcpyAsync(mock, c->mock, sizeof(float) * ndata, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GPAIR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// gpu_linesearch_zoom, gpair1/trunk/gpu.c:1947-2065, with FP64 scalars
+// ---------------------------------------------------------------------------
+extern "C" int gpu_linesearch_zoom(gpair_ctx *c, int nuv, int npow, int nbis, int data_alloc, int data_alloc_uv,
+                                   int W, double lo, double hi, double J_lo, double w1, double J_init,
+                                   int *criterion_evals, int *grad_evals, gpair_buf d, gpair_buf tg, double alpha,
+                                   int max_evals, double *selected)
+{
+    const double minv = 1e-8, c1 = 1e-4, c2 = 0.1;
+    int rc;
+    if ((rc = ready(c)))
+        return rc;
+    if (!selected || !own_buf(c, d) || !own_buf(c, tg))
+        return fail(c, GPAIR_E_ARG, "gpu_linesearch_zoom: bad buffers or NULL output");
+    double s = 0.0, s_old = 0.0, J_old = 0.0;
+    int counter = 0, evals = 0;
+    for (;;) {
+        if (counter > 0 && (J_old - J_init - w1 * s_old) != 0.0)
+            s = fabs(w1 * s_old * s_old / (2.0 * (J_old - J_init - w1 * s_old)));
+        if (counter == 0 || s < lo || s > hi)
+            s = (hi - lo) / 2.0 + lo;
+        if (fabs(hi - lo) < 1e-14 || (max_evals >= 0 && evals >= max_evals)) {
+            *selected = lo;
+            return GPAIR_OK;
+        }
+        double chi2, S;
+        if ((rc = gpu_update_tempimage(c, W, s, minv, d)) ||
+            (rc = gpu_get_chi2_temp(c, nuv, npow, nbis, data_alloc, data_alloc_uv, &chi2)) ||
+            (rc = gpu_get_entropy_temp(c, W, &S)))
+            return rc;
+        const double J = chi2 - alpha * S;
+        evals++;
+        if (criterion_evals)
+            (*criterion_evals)++;
+        if (J > J_init + c1 * s * w1 || J >= J_lo) {
+            hi = s;
+        } else {
+            double w2;
+            if ((rc = gpu_compute_data_gradient_temp(c, nuv, npow, nbis, W)) ||
+                (rc = gpu_compute_entropy_gradient_temp(c, W)) ||
+                (rc = gpu_compute_criterion_gradient_temp(c, W, alpha)) ||
+                (rc = gpu_get_scalprod(c, W, W, d, tg, &w2)))
+                return rc;
+            if (grad_evals)
+                (*grad_evals)++;
+            if (w2 >= c2 * w1 || counter > 10) {
+                *selected = s;
+                return GPAIR_OK;
+            }
+            if (w2 * (hi - lo) >= 0.0)
+                hi = lo;
+            lo = s;
+        }
+        s_old = s;
+        J_old = J;
+        counter++;
+    }
+}

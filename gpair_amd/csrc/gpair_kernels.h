@@ -1,0 +1,123 @@
+// Device kernels of the MEM-CG path (gfx950 / CDNA4, wave64).
+//
+// Layouts in HBM (one reconstruction):
+//   image-sized float buffers   [W*W], pixel (x, y) at x + W*y
+//   X, Y tables                 float2 [nuv][W]  (X = exp(+2 pi i a u x), Y = exp(-2 pi i a v y))
+//   vis, coef                   float2 [nuv]
+//   data, err, mock             float  [npow + 2*nbis]  ([V2 | (Re, Im) per T3])
+//   tri                         int4   [nbis]  (ab, bc, ca uv index, 3 sign bits: bit j set = conjugate leg j)
+//   csr_ptr/csr_ent             uv -> T3 legs touching it (entry = 3*t + leg), ascending t
+//   scal                        double [GPAIR_NSCAL] reduction results, invflux float alongside
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+enum {
+    SC_FLUX = 0,
+    SC_CHI2 = 1,
+    SC_ENT = 2,
+    SC_DOT = 3,
+    SC_C0 = 4,
+    SC_DOT2 = 5,
+    SC_DOT3 = 6,
+    GPAIR_NSCAL = 8
+};
+
+#define RED_THREADS 256
+#define RED_MAX_BLOCKS 2048
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// deterministic block reduction + last-block finish (fixed summation order)
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double (*sh)[RED_THREADS / 64])
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+        double s = wave_sum(v[j]);
+        if (lane == 0)
+            sh[j][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; j++) {
+            double s = 0.0;
+            for (int w = 0; w < RED_THREADS / 64; w++)
+                s += sh[j][w];
+            v[j] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// Every block calls this once with its thread-partial values.  Block sums go to
+// part[blockIdx.x*NV + j]; the last block to arrive sums them in block order and
+// writes out[j] (and, for the flux slot, the float inverse).  The ticket is
+// reset by the last block so no memset is needed between launches.
+template <int NV>
+__device__ inline void reduce_finish(double (&v)[NV], double *part, unsigned *ticket, double *out,
+                                     float *invflux_out)
+{
+    __shared__ double sh[NV][RED_THREADS / 64];
+    __shared__ int is_last;
+    block_sum<NV>(v, sh);
+    const unsigned nblk = gridDim.x;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+            __hip_atomic_store(&part[blockIdx.x * NV + j], v[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = (t == nblk - 1);
+    }
+    __syncthreads();
+    if (!is_last)
+        return;
+    double acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++)
+        acc[j] = 0.0;
+    for (unsigned b = threadIdx.x; b < nblk; b += RED_THREADS) {
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+            acc[j] += __hip_atomic_load(&part[b * NV + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    block_sum<NV>(acc, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+            out[j] = acc[j];
+        if (invflux_out)
+            *invflux_out = (float)(1.0 / acc[0]);
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// phase helpers: exp(2 pi i t) with t range-reduced in FP64
+// ---------------------------------------------------------------------------
+__device__ inline float2 turn_exp(double t)
+{
+    double f = t - rint(t);
+    float s, c;
+    sincospif((float)(2.0 * f), &s, &c);
+    return make_float2(c, s);
+}
+
+__device__ inline float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }

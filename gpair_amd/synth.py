@@ -1,0 +1,290 @@
+"""Seeded synthetic interferometry problems in the reference's packed layout.
+
+The reference reads OIFITS files (gpair1/trunk/getoifits.c:65-357) and packs
+them with init_data (gpair1/trunk/gpair.c:997-1066).  This module builds the
+same packed arrays directly from a seeded array layout and sky model:
+
+* uv list: every V2 baseline first (``npow == nuv``), flipped into the +u half
+  plane as getoifits.c:147-151 does; every T3 leg (ab, bc, ca) points at the V2
+  baseline of the same snapshot/channel with ``sign = -1`` when the leg is the
+  conjugate of the stored point (getoifits.c:218-337 semantics, no dedup
+  search needed because legs are known by construction).
+* data / err / phasor packed exactly like init_data:
+  data = [V2 | (|T3amp|, 0) per T3], err = [1/|sigV2| | (1/sigamp, 1/|amp*sigphi*pi/180|)],
+  phasor = exp(-i * t3phi * pi/180).
+
+Noise conventions (recorded in ``Problem.meta``):
+  sigma_V2  = max(0.02 * V2_true, 1e-4)      (2 % relative, absolute floor)
+  sigma_amp = max(0.05 * |T3_true|, 1e-5)    (5 % relative, absolute floor)
+  sigma_phi = 1 degree
+
+Sky visibilities are analytic (uniform / limb-darkened disks, Gaussians,
+rings, points), so the data carry no pixelisation of their own.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+from scipy import special
+
+# gpair1/trunk/getoifits.h:87-88
+PI_REF = 3.14159265358979323
+RPMAS = (PI_REF / 180.0) / 3600000.0
+
+
+@dataclasses.dataclass
+class Problem:
+    W: int
+    pixellation: float            # mas per pixel
+    u: np.ndarray                 # float32 [nuv] (wavelengths)
+    v: np.ndarray                 # float32 [nuv]
+    npow: int
+    nbis: int
+    data: np.ndarray              # float32 [npow + 2 nbis]
+    err: np.ndarray               # float32 [npow + 2 nbis] (inverse errors)
+    phasor: np.ndarray            # complex64 [nbis]
+    uvpnt: np.ndarray             # int64 [nbis, 3]
+    sign: np.ndarray              # int16 [nbis, 3]
+    model: np.ndarray             # float32 [W*W] prior (default model)
+    start: np.ndarray             # float32 [W*W] starting image
+    truth_vis: np.ndarray         # complex128 [nuv] noiseless visibilities
+    meta: dict
+
+    @property
+    def nuv(self) -> int:
+        return int(self.u.shape[0])
+
+    @property
+    def ndof(self) -> int:
+        return self.npow + 2 * self.nbis
+
+    @property
+    def alpha_px(self) -> float:
+        """radians per pixel, RPMAS * pixellation with the float32 pixellation."""
+        return RPMAS * float(np.float32(self.pixellation))
+
+    def au(self) -> np.ndarray:
+        return self.alpha_px * self.u.astype(np.float64)
+
+    def av(self) -> np.ndarray:
+        return self.alpha_px * self.v.astype(np.float64)
+
+    def uvpnt4(self) -> np.ndarray:
+        """reference cl_long4 layout (gpair1/trunk/gpair.c:585-592)."""
+        out = np.zeros((self.nbis, 4), np.int64)
+        out[:, :3] = self.uvpnt
+        return out
+
+    def sign4(self) -> np.ndarray:
+        """reference cl_short4 layout (gpair1/trunk/gpair.c:594-597)."""
+        out = np.zeros((self.nbis, 4), np.int16)
+        out[:, :3] = self.sign
+        return out
+
+
+# ---------------------------------------------------------------------------
+# sky components: visibility of a unit-flux component centred at the origin
+# ---------------------------------------------------------------------------
+def _rho_rad(u, v):
+    return np.hypot(u, v)  # wavelengths
+
+
+def vis_uniform_disk(u, v, radius_mas):
+    x = math.pi * 2.0 * radius_mas * RPMAS * _rho_rad(u, v)
+    out = np.ones_like(x)
+    nz = x > 1e-12
+    out[nz] = 2.0 * special.j1(x[nz]) / x[nz]
+    return out
+
+
+def vis_ld_disk(u, v, radius_mas, mu):
+    """linear limb darkening I(r) = 1 - mu (1 - sqrt(1 - r^2/R^2))."""
+    x = math.pi * 2.0 * radius_mas * RPMAS * _rho_rad(u, v)
+    out = np.ones_like(x)
+    nz = x > 1e-6
+    xs = x[nz]
+    a = special.j1(xs) / xs
+    b = (np.sin(xs) - xs * np.cos(xs)) / xs ** 3
+    out[nz] = ((1.0 - mu) * a + mu * b) / (0.5 - mu / 6.0)
+    return out
+
+
+def vis_gauss(u, v, fwhm_mas):
+    t = math.pi * fwhm_mas * RPMAS * _rho_rad(u, v)
+    return np.exp(-t * t / (4.0 * math.log(2.0)))
+
+
+def vis_ring(u, v, radius_mas):
+    return special.j0(2.0 * math.pi * radius_mas * RPMAS * _rho_rad(u, v))
+
+
+def shift(u, v, x_mas, y_mas):
+    """phase of a component at (x, y) mas in the image frame of the DFT:
+    X = exp(+2 pi i a u x), Y = exp(-2 pi i a v y) (gpair1/trunk/gpair.c:242-243)."""
+    return np.exp(2j * math.pi * RPMAS * (u * x_mas - v * y_mas))
+
+
+# ---------------------------------------------------------------------------
+# array geometry
+# ---------------------------------------------------------------------------
+def _array_uv(rng, ntel, nch, nsnap, uvmax):
+    """station uv coordinates s[snap, ch, tel, 2] (wavelengths) with every
+    baseline inside +-uvmax on each axis."""
+    pos = rng.uniform(-0.5, 0.5, size=(ntel, 2))
+    pos -= pos.mean(axis=0)
+    lam = np.linspace(1.0, 1.6, nch)              # relative wavelengths
+    ang = np.linspace(0.0, math.pi / 2.0, nsnap, endpoint=False) + rng.uniform(0, 0.3)
+    s = np.empty((nsnap, nch, ntel, 2))
+    for k, a in enumerate(ang):
+        rot = np.array([[math.cos(a), -math.sin(a)], [math.sin(a), math.cos(a)]])
+        p = pos @ rot.T
+        for c, l in enumerate(lam):
+            s[k, c] = p / l
+    # scale so that the longest baseline component reaches uvmax
+    i, j = np.triu_indices(ntel, 1)
+    b = s[:, :, j, :] - s[:, :, i, :]
+    s *= uvmax / np.abs(b).max()
+    return s
+
+
+def build_layout(rng, ntel, nch, nsnap, uvmax):
+    """V2 points and T3 references for all baselines and all triangles."""
+    s = _array_uv(rng, ntel, nch, nsnap, uvmax)
+    bi, bj = np.triu_indices(ntel, 1)
+    nbase = bi.size
+    base_index = {}
+    for b in range(nbase):
+        base_index[(bi[b], bj[b])] = b
+    # V2 uv points ordered [snap][ch][baseline]
+    uvl = s[:, :, bj, :] - s[:, :, bi, :]          # [snap, ch, nbase, 2]
+    uvl = uvl.reshape(-1, 2)
+    flip = uvl[:, 0] < 0.0                         # getoifits.c:147-151
+    stored = np.where(flip[:, None], -uvl, uvl).astype(np.float32)
+    tri = [(a, b, c) for a in range(ntel) for b in range(a + 1, ntel) for c in range(b + 1, ntel)]
+    ntri = len(tri)
+    uvpnt = np.empty((nsnap, nch, ntri, 3), np.int64)
+    sign = np.empty((nsnap, nch, ntri, 3), np.int16)
+    for k in range(nsnap):
+        for c in range(nch):
+            off = (k * nch + c) * nbase
+            for t, (a, b, cc) in enumerate(tri):
+                # legs ab = s_b - s_a, bc = s_c - s_b, ca = s_a - s_c
+                for leg, (p, q) in enumerate(((a, b), (b, cc), (cc, a))):
+                    lo, hi = min(p, q), max(p, q)
+                    idx = off + base_index[(lo, hi)]
+                    same_dir = (p, q) == (lo, hi)  # leg == stored baseline before the flip
+                    sgn = 1 if same_dir else -1
+                    if flip[idx]:
+                        sgn = -sgn
+                    uvpnt[k, c, t, leg] = idx
+                    sign[k, c, t, leg] = sgn
+    return stored, uvpnt.reshape(-1, 3), sign.reshape(-1, 3), (nbase, ntri)
+
+
+# ---------------------------------------------------------------------------
+# sky models for the configurations
+# ---------------------------------------------------------------------------
+def sky_components(kind, rng, fov_mas):
+    """list of (flux, vis_fn(u, v) -> complex, x_mas, y_mas) with fluxes summing to 1."""
+    comps = []
+    if kind == "disk_spots":  # uniform disk r = 1.5 mas + 2 Gaussian spots
+        fs = rng.uniform(0.10, 0.30, size=2)
+        comps.append((1.0 - fs.sum(), lambda u, v: vis_uniform_disk(u, v, 1.5), 0.0, 0.0))
+        for f in fs:
+            r, th = rng.uniform(0.2, 1.2), rng.uniform(0, 2 * math.pi)
+            comps.append((f, lambda u, v: vis_gauss(u, v, 0.3), r * math.cos(th), r * math.sin(th)))
+    elif kind == "ld_disk_spots_companion":
+        nsp = int(rng.integers(1, 6))
+        fs = rng.uniform(0.03, 0.10, size=nsp)
+        fc = rng.uniform(0.01, 0.05)
+        R = 0.2 * fov_mas
+        comps.append((1.0 - fs.sum() - fc, lambda u, v: vis_ld_disk(u, v, R, 0.4), 0.0, 0.0))
+        for f in fs:
+            r, th = rng.uniform(0.1, 0.8) * R, rng.uniform(0, 2 * math.pi)
+            fw = rng.uniform(0.15, 0.4) * R
+            comps.append((f, (lambda fw_: (lambda u, v: vis_gauss(u, v, fw_)))(fw), r * math.cos(th), r * math.sin(th)))
+        r, th = rng.uniform(0.3, 0.42) * fov_mas, rng.uniform(0, 2 * math.pi)
+        comps.append((fc, lambda u, v: np.ones_like(u), r * math.cos(th), r * math.sin(th)))
+    elif kind == "ring_disk":
+        R = 0.2 * fov_mas
+        fr = rng.uniform(0.3, 0.6)
+        comps.append((fr, lambda u, v: vis_ring(u, v, R), 0.0, 0.0))
+        comps.append((1.0 - fr, lambda u, v: vis_uniform_disk(u, v, 0.4 * R), 0.1 * R, -0.05 * R))
+    else:
+        raise ValueError(kind)
+    return comps
+
+
+CONFIGS = {
+    # reference CPU-runnable case: 64x64 @ 0.1 mas, 6 telescopes, 8 ch x 4 snapshots
+    "cfg1": dict(W=64, pix=0.1, ntel=6, nch=8, nsnap=4, uvmax=3e8, sky="disk_spots"),
+    # headline: 256x256 @ 0.05 mas, 9 telescopes, 64 ch x 16 snapshots
+    "cfg2": dict(W=256, pix=0.05, ntel=9, nch=64, nsnap=16, uvmax=3e8, sky="ld_disk_spots_companion"),
+    # precision variant: 512x512, 256 channels
+    "cfg3": dict(W=512, pix=0.05, ntel=9, nch=256, nsnap=16, uvmax=3e8, sky="ring_disk"),
+}
+
+
+def make_problem(cfg="cfg1", seed=1, W=None, pix=None, ntel=None, nch=None, nsnap=None, uvmax=None,
+                 sky=None, prior_flux=1.0, noise=True) -> Problem:
+    """Seeded synthetic problem.  Named configs fix the sizes; keyword
+    overrides allow small test shapes with the same construction."""
+    c = dict(CONFIGS[cfg])
+    for k, val in (("W", W), ("pix", pix), ("ntel", ntel), ("nch", nch), ("nsnap", nsnap),
+                   ("uvmax", uvmax), ("sky", sky)):
+        if val is not None:
+            c[k] = val
+    rng = np.random.default_rng(seed)
+    uv, uvpnt, sign, (nbase, ntri) = build_layout(rng, c["ntel"], c["nch"], c["nsnap"], c["uvmax"])
+    u = uv[:, 0].astype(np.float64)
+    v = uv[:, 1].astype(np.float64)
+    W_ = c["W"]
+    fov = W_ * c["pix"]
+    comps = sky_components(c["sky"], rng, fov)
+    # the sky is centred on pixel (W/2, W/2) of the DFT frame
+    xc = (W_ // 2) * float(np.float32(c["pix"]))
+    V = np.zeros(u.shape, np.complex128)
+    for f, fn, x, y in comps:
+        V += f * fn(u, v) * shift(u, v, xc + x, xc + y)
+    npow = u.size
+    nbis = uvpnt.shape[0]
+    v2 = np.abs(V) ** 2
+    legs = np.where(sign < 0, np.conj(V[uvpnt]), V[uvpnt])
+    T3 = legs.prod(axis=1)
+    sig_v2 = np.maximum(0.02 * v2, 1e-4)
+    sig_amp = np.maximum(0.05 * np.abs(T3), 1e-5)
+    sig_phi = np.full(nbis, 1.0)
+    if noise:
+        v2_obs = v2 + rng.normal(size=npow) * sig_v2
+        amp_obs = np.abs(T3) + rng.normal(size=nbis) * sig_amp
+        phi_obs = np.degrees(np.angle(T3)) + rng.normal(size=nbis) * sig_phi
+    else:
+        v2_obs, amp_obs, phi_obs = v2, np.abs(T3), np.degrees(np.angle(T3))
+    v2_obs = v2_obs.astype(np.float32)
+    amp_obs = amp_obs.astype(np.float32)
+    phi_obs = phi_obs.astype(np.float32)
+    sig_v2 = sig_v2.astype(np.float32)
+    sig_amp = sig_amp.astype(np.float32)
+    sig_phi = sig_phi.astype(np.float32)
+    # init_data packing, gpair1/trunk/gpair.c:1000-1063 (float arithmetic)
+    ndof = npow + 2 * nbis
+    data = np.zeros(ndof, np.float32)
+    err = np.zeros(ndof, np.float32)
+    data[:npow] = v2_obs
+    err[:npow] = 1.0 / np.abs(sig_v2)
+    data[npow::2] = np.abs(amp_obs)
+    err[npow::2] = 1.0 / sig_amp
+    err[npow + 1::2] = 1.0 / np.abs(amp_obs.astype(np.float64) * sig_phi * PI_REF / 180.0)
+    phasor = np.exp(-1j * phi_obs.astype(np.float64) * PI_REF / 180.0).astype(np.complex64)
+    # flat prior of the given total flux (set_model type 0, gpair1/trunk/cpu.c:653-662),
+    # start image = max(prior, 1e-8) (gpair1/trunk/gpair.c:217-225)
+    model = np.full(W_ * W_, prior_flux / float(W_ * W_), np.float32)
+    start = np.maximum(model, np.float32(1e-8))
+    meta = dict(cfg=cfg, seed=seed, nbase=nbase, ntri=ntri, sky=c["sky"], W=W_, pix=c["pix"],
+                noise="sigV2=max(0.02*V2,1e-4); sigamp=max(0.05*|T3|,1e-5); sigphi=1deg",
+                uvmax=c["uvmax"], ntel=c["ntel"], nch=c["nch"], nsnap=c["nsnap"])
+    return Problem(W=W_, pixellation=c["pix"], u=uv[:, 0].copy(), v=uv[:, 1].copy(), npow=npow,
+                   nbis=nbis, data=data, err=err, phasor=phasor, uvpnt=uvpnt, sign=sign,
+                   model=model, start=start, truth_vis=V, meta=meta)

@@ -1,0 +1,186 @@
+/*
+ * gpair_hip.h -- C ABI of the MI355X (gfx950) MEM-CG reconstruction path.
+ *
+ * The entry points keep the names and meaning of GPAIR's OpenCL host layer,
+ * gpair1/trunk/gpu.h:1-120 / gpair1/trunk/gpu.c, so that the CG loop of
+ * gpair1/trunk/gpair.c:567-922 drops in.  Differences, all deliberate:
+ *   - every call takes an explicit gpair_ctx* (the reference keeps file-static
+ *     globals, gpair1/trunk/gpu.c:22-115); contexts are independent;
+ *   - every call returns an int status (GPAIR_OK == 0) instead of exit(0)
+ *     on error (gpair1/trunk/gpu.c:186-202); scalars come back through double*;
+ *   - scalars are FP64 (the reference returns float);
+ *   - device buffers are opaque gpair_buf handles instead of cl_mem*;
+ *   - sizes are exact counts (npow + 2*nbis, nbis, nuv), not the pow2-padded
+ *     allocation sizes the OpenCL kernels needed (gpair1/trunk/gpair.c:166-172);
+ *   - the image width must be a multiple of 64 (64..1024).
+ * Threading: one HIP stream per context, calls on one context are serialised
+ * by the caller (as in the reference); different contexts may be used from
+ * different host threads.
+ */
+#ifndef GPAIR_HIP_H
+#define GPAIR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gpair_ctx gpair_ctx;
+typedef struct gpair_buf_s *gpair_buf;
+
+enum {
+    GPAIR_OK = 0,
+    GPAIR_E_ARG = 1,      /* invalid argument (sizes, indices, NULL) */
+    GPAIR_E_HIP = 2,      /* HIP runtime error */
+    GPAIR_E_STATE = 3,    /* call out of order (e.g. chi2 before data) */
+    GPAIR_E_NOMEM = 4,    /* device allocation failed */
+    GPAIR_E_NUMERIC = 5   /* non-finite objective or step */
+};
+
+/* precision of the forward / adjoint DFT contraction */
+enum {
+    GPAIR_PREC_FP32 = 0,   /* FP32 MFMA, FP32 accumulation */
+    GPAIR_PREC_BF16X3 = 1  /* split BF16 hi/lo, 3 MFMA products, FP32 accumulation */
+};
+
+const char *gpair_strerror(int code);
+const char *gpair_last_error(const gpair_ctx *ctx);
+
+/* ---- setup / teardown ------------------------------------------------- */
+/* gpu_init, gpair1/trunk/gpu.c:1693-1732 (first GPU, in-order queue) */
+int gpu_init(gpair_ctx **ctx, int device);
+/* gpu_copy_data, gpair1/trunk/gpu.c:1175-1415: data/err [npow + 2 nbis] packed as
+ * init_data does (gpair1/trunk/gpair.c:997-1066); phasor = (re, im) float pairs
+ * [nbis]; uvpnt = 4 x int64 per T3 (cl_long4, gpair1/trunk/gpair.c:585-592);
+ * sign = 4 x int16 per T3 (cl_short4); default_model [image_width^2].
+ * data_size = npow + 2*bsref_size, data_size_uv = nuv, phasor_size = bsref_size = nbis. */
+int gpu_copy_data(gpair_ctx *ctx, const float *data, const float *data_err, int data_size, int data_size_uv,
+                  const float *phasor, int phasor_size, int pow_size, const int64_t *uvpnt, const int16_t *sign,
+                  int bsref_size, const float *default_model, int image_size, int image_width);
+/* gpu_copy_image, gpair1/trunk/gpu.c:1441-1451: the current image, [y][x] */
+int gpu_copy_image(gpair_ctx *ctx, const float *image, int x_size, int y_size);
+/* gpu_copy_dft, gpair1/trunk/gpair.c:653: tables in the reference's GPU layout
+ * float2 [W][nuv] (gpair1/trunk/gpair.c:610-636); dft_size >= W*nuv. */
+int gpu_copy_dft(gpair_ctx *ctx, const float *dft_x, const float *dft_y, int dft_size);
+/* No reference counterpart (the reference builds tables on the host,
+ * gpair1/trunk/gpair.c:231-245): builds X[p][x] = exp(+2 pi i a u_p x) and
+ * Y[p][y] = exp(-2 pi i a v_p y), a = RPMAS*pixellation, on the device with the
+ * phase range-reduced in FP64.  u, v in wavelengths, [nuv]. */
+int gpair_set_uv(gpair_ctx *ctx, const float *u, const float *v, int nuv, float pixellation);
+/* gpu_build_kernels, gpair1/trunk/gpu.c:306-411: kernels are compiled ahead of
+ * time here; this validates the sizes and allocates the work buffers. */
+int gpu_build_kernels(gpair_ctx *ctx, int data_size, int image_width, int image_size);
+/* gpu_cleanup / gpu_shutdown, gpair1/trunk/gpu.c:887-960: free everything */
+int gpu_cleanup(gpair_ctx *ctx);
+int gpu_shutdown(gpair_ctx *ctx);
+
+/* select the DFT precision (default GPAIR_PREC_FP32); no reference counterpart */
+int gpair_set_precision(gpair_ctx *ctx, int precision);
+
+/* ---- buffer handles, gpair1/trunk/gpu.c:1829-1867 -------------------- */
+gpair_buf gpu_getp_ci(gpair_ctx *ctx);   /* current image */
+gpair_buf gpu_getp_ti(gpair_ctx *ctx);   /* temporary (trial) image */
+gpair_buf gpu_getp_fgn(gpair_ctx *ctx);  /* full gradient, new */
+gpair_buf gpu_getp_fg(gpair_ctx *ctx);   /* full gradient, previous */
+gpair_buf gpu_getp_dd(gpair_ctx *ctx);   /* descent direction */
+gpair_buf gpu_getp_tg(gpair_ctx *ctx);   /* gradient at the trial image */
+gpair_buf gpu_getp_eg(gpair_ctx *ctx);   /* entropy gradient */
+gpair_buf gpair_getp_dg(gpair_ctx *ctx); /* chi2 (data) gradient; no reference getter */
+
+/* ---- objective -------------------------------------------------------- */
+/* gpu_get_chi2_curr|temp, gpair1/trunk/gpu.c:1734-1763: flux, DFT, V2/T3 model,
+ * chi2.  Leaves visibilities and mock data for the next gradient call. */
+int gpu_get_chi2_curr(gpair_ctx *ctx, int nuv, int npow, int nbis, int data_alloc, int data_alloc_uv,
+                      double *chi2);
+int gpu_get_chi2_temp(gpair_ctx *ctx, int nuv, int npow, int nbis, int data_alloc, int data_alloc_uv,
+                      double *chi2);
+/* gpu_get_entropy_curr|temp, gpair1/trunk/gpu.c:1767-1802 */
+int gpu_get_entropy_curr(gpair_ctx *ctx, int image_width, double *entropy);
+int gpu_get_entropy_temp(gpair_ctx *ctx, int image_width, double *entropy);
+
+/* ---- gradients -------------------------------------------------------- */
+/* gpu_compute_data_gradient_curr|temp, gpair1/trunk/gpu.c:1598-1691 (adjoint DFT);
+ * uses the visibilities / mock data of the last chi2 call, as the reference does. */
+int gpu_compute_data_gradient_curr(gpair_ctx *ctx, int nuv, int npow, int nbis, int image_width);
+int gpu_compute_data_gradient_temp(gpair_ctx *ctx, int nuv, int npow, int nbis, int image_width);
+/* gpu_compute_entropy_gradient_curr|temp, gpair1/trunk/gpu.c:796-857 */
+int gpu_compute_entropy_gradient_curr(gpair_ctx *ctx, int image_width);
+int gpu_compute_entropy_gradient_temp(gpair_ctx *ctx, int image_width);
+/* gpu_compute_criterion_gradient_curr|temp, gpair1/trunk/gpu.c:615-666:
+ * fgn (curr) or tg (temp) = data gradient - alpha * entropy gradient */
+int gpu_compute_criterion_gradient_curr(gpair_ctx *ctx, int image_width, double alpha);
+int gpu_compute_criterion_gradient_temp(gpair_ctx *ctx, int image_width, double alpha);
+
+/* ---- CG --------------------------------------------------------------- */
+/* gpu_get_scalprod, gpair1/trunk/gpu.c:2114-2129 (FP64 accumulation) */
+int gpu_get_scalprod(gpair_ctx *ctx, int data_width, int data_height, gpair_buf a, gpair_buf b, double *out);
+/* gpu_compute_descent_dir, gpair1/trunk/gpu.c:705-739: dd = beta*dd - fgn */
+int gpu_compute_descent_dir(gpair_ctx *ctx, int image_width, double beta);
+/* gpu_update_tempimage, gpair1/trunk/gpu.c:2300-2339: ti = max(ci + s*d, minval) */
+int gpu_update_tempimage(gpair_ctx *ctx, int image_width, double steplength, double minval, gpair_buf d);
+/* gpu_update_image, gpair1/trunk/gpu.c:2260-2298: ci = max(ci + s*d, minval) */
+int gpu_update_image(gpair_ctx *ctx, int image_width, double steplength, double minval, gpair_buf d);
+/* gpu_backup_gradient, gpair1/trunk/gpu.c:257-268: out <- in (data_size floats) */
+int gpu_backup_gradient(gpair_ctx *ctx, int data_size, gpair_buf in, gpair_buf out);
+/* gpu_linesearch_zoom, gpair1/trunk/gpu.c:1947-2065.  The evaluation counters
+ * are incremented (the reference increments the pointers instead,
+ * gpair1/trunk/gpu.c:2004,2033).  max_evals bounds the J evaluations (a
+ * build-side safeguard; < 0 means unbounded, as in the reference); when it is
+ * reached the current low end of the interval is returned. */
+int gpu_linesearch_zoom(gpair_ctx *ctx, int nuv, int npow, int nbis, int data_alloc, int data_alloc_uv,
+                        int image_width, double steplength_low, double steplength_high,
+                        double criterion_steplength_low, double wolfe_product1, double criterion_init,
+                        int *criterion_evals, int *grad_evals, gpair_buf descent_direction,
+                        gpair_buf temp_gradient, double hyperparameter_entropy, int max_evals,
+                        double *selected_steplength);
+
+/* ---- output ----------------------------------------------------------- */
+/* gpu_get_image, gpair1/trunk/gpu.c:1815-1827: blocking device -> host copy */
+int gpu_get_image(gpair_ctx *ctx, int size, float *cpu_buffer, gpair_buf which);
+/* copies of the last visibilities (float2 [nuv]) / mock data ([npow+2nbis]);
+ * no reference getter (the reference compares them in gpu_check_data) */
+int gpair_get_vis(gpair_ctx *ctx, float *vis, int nuv);
+int gpair_get_mock(gpair_ctx *ctx, float *mock, int ndata);
+/* host -> device copy into any image-sized buffer (tests, warm starts) */
+int gpair_put_image(gpair_ctx *ctx, gpair_buf which, const float *host, int size);
+
+/* ---- the MEM-CG loop (gpair1/trunk/gpair.c:682-915), C host code -------- */
+typedef struct {
+    double alpha;          /* entropy hyperparameter (gpair.c -a) */
+    int method;            /* 0 SD, 1 FR, 2 PR, 3 HS (gpair1/trunk/gpair.c:727-758) */
+    int max_iter;          /* outer iterations */
+    double chi2r_target;   /* stop when chi2/ndof <= target (0 disables) */
+    int ls_max_evals;      /* J evaluations per line search (<= 0: unbounded) */
+    double minvalue;       /* pixel floor, 1e-8 in the reference */
+    double wolfe1, wolfe2; /* 1e-4, 0.1 */
+    double step_max;       /* 100 */
+    int verbose;
+} gpair_cg_params;
+
+typedef struct {
+    int iterations;        /* outer iterations whose J was evaluated */
+    int criterion_evals;
+    int grad_evals;
+    double J, chi2, entropy, chi2r;
+    int converged;         /* chi2r <= target reached */
+    int status;
+} gpair_cg_result;
+
+void gpair_cg_default_params(gpair_cg_params *p);
+/* Runs the loop on the current image of ctx.  trace (may be NULL) receives 8
+ * doubles per outer iteration: J, chi2, S, selected step, beta, J evals,
+ * gradient evals, 0. */
+int gpair_cg_run(gpair_ctx *ctx, const gpair_cg_params *params, gpair_cg_result *result, double *trace,
+                 int trace_rows);
+
+/* problem sizes of a context (0 when not set) */
+int gpair_sizes(const gpair_ctx *ctx, int *W, int *nuv, int *npow, int *nbis);
+/* synchronise the context's stream */
+int gpair_sync(gpair_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPAIR_HIP_H */

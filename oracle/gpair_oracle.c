@@ -1,0 +1,502 @@
+/*
+ * gpair_oracle.c -- FP64 CPU restatement of the GPAIR MEM-CG reconstruction
+ * path.  TEST INFRASTRUCTURE ONLY: tests, the smoke check and the CPU leg of
+ * the benchmark load it as the checker.  The product (libgpair_hip.so) never
+ * links, loads or calls it.
+ *
+ * Every function restates one piece of the reference; the citation is in the
+ * comment above it (paths relative to the reference repository root).  All
+ * arithmetic is double precision (the reference mixes float and double).
+ *
+ * Conventions shared with the product:
+ *   image[x + W*y] holds I[y][x]                 (gpair1/trunk/cpu.c:469)
+ *   au[p] = RPMAS*pixellation*u[p]  (turns per pixel along x)
+ *   av[p] = RPMAS*pixellation*v[p]  (turns per pixel along y)
+ *   X[p][x] = exp(+2 pi i au[p] x), Y[p][y] = exp(-2 pi i av[p] y)
+ *                                                 (gpair1/trunk/gpair.c:242-243)
+ *   data/err/mock: [V2 (npow) | (Re, Im) per T3 (2*nbis)]
+ *   phasor: complex per T3, uvpnt/sign: 3 per T3 (ab, bc, ca)
+ *
+ * Parity status of each function is listed in oracle/README.md.
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <stdio.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static int orc_verbose = 0;
+void orc_set_verbose(int v) { orc_verbose = v; }
+
+/* Build-side safeguard, absent from the reference: at most this many J
+ * evaluations per line search (bracket + zoom).  The reference zoom can loop
+ * without bound when quadratic interpolation keeps landing next to an end of
+ * [lo, hi] and the bracket loop never ends on a NaN objective.  When the cap
+ * is hit the line search returns its current low end (a step that satisfied
+ * the sufficient-decrease test, or 0). */
+static int orc_ls_cap = 64;
+void orc_set_ls_cap(int cap) { orc_ls_cap = cap; }
+
+#define ORC_TWO_PI 6.283185307179586476925286766559
+
+/* phase of exp(2 pi i t) reduced to one turn in FP64 before the cosine/sine */
+static inline double complex turn_exp(double t)
+{
+    double f = t - nearbyint(t);
+    return cos(ORC_TWO_PI * f) + I * sin(ORC_TWO_PI * f);
+}
+
+/* DFT tables, gpair1/trunk/gpair.c:231-245 (there: float complex [nuv][W],
+ * phase evaluated with cexp of a double argument). Here complex128 [nuv][W]
+ * stored as interleaved (re, im) doubles. */
+void orc_tables(int W, int nuv, const double *au, const double *av, double *X, double *Y)
+{
+#pragma omp parallel for schedule(static)
+    for (int p = 0; p < nuv; p++) {
+        for (int i = 0; i < W; i++) {
+            double complex x = turn_exp(au[p] * (double)i);
+            double complex y = turn_exp(-av[p] * (double)i);
+            X[2 * ((size_t)p * W + i)] = creal(x);
+            X[2 * ((size_t)p * W + i) + 1] = cimag(x);
+            Y[2 * ((size_t)p * W + i)] = creal(y);
+            Y[2 * ((size_t)p * W + i) + 1] = cimag(y);
+        }
+    }
+}
+
+/* Total flux F = sum I: gpair1/trunk/cpu.c:270-277 (compute_flux), used by
+ * gpu_compute_flux, gpair1/trunk/gpu.c:860-884. */
+double orc_flux(int n, const double *img)
+{
+    double f = 0.0;
+    for (int i = 0; i < n; i++)
+        f += img[i];
+    return f;
+}
+
+/* image -> complex visibilities, gpair1/trunk/cpu.c:449-475 (image2vis) and
+ * the visi kernel gpair1/trunk/kernel_visi.cl:26-66:
+ *   V_p = (1/F) sum_y sum_x I[y][x] X[p][x] Y[p][y].
+ * Returns F. vis: 2*nuv doubles. */
+double orc_image2vis(int W, int nuv, const double *au, const double *av, const double *img,
+                     double *vis)
+{
+    double F = orc_flux(W * W, img);
+    double inv = (F > 0.0) ? 1.0 / F : 1.0;
+#pragma omp parallel
+    {
+        double complex *xr = malloc(sizeof(double complex) * (size_t)W);
+#pragma omp for schedule(static)
+        for (int p = 0; p < nuv; p++) {
+            for (int i = 0; i < W; i++)
+                xr[i] = turn_exp(au[p] * (double)i);
+            double complex acc = 0.0;
+            for (int y = 0; y < W; y++) {
+                double complex t = 0.0;
+                const double *row = img + (size_t)y * W;
+                for (int x = 0; x < W; x++)
+                    t += row[x] * xr[x];
+                acc += t * turn_exp(-av[p] * (double)y);
+            }
+            acc *= inv;
+            vis[2 * p] = creal(acc);
+            vis[2 * p + 1] = cimag(acc);
+        }
+        free(xr);
+    }
+    return F;
+}
+
+static inline double complex leg(const double *vis, int idx, int sign)
+{
+    double complex v = vis[2 * idx] + I * vis[2 * idx + 1];
+    return sign < 0 ? conj(v) : v;
+}
+
+/* visibilities -> mock data, gpair1/trunk/cpu.c:758-802 (vis2data) and the
+ * kernels gpair1/trunk/kernel_pow_spec.cl:1-11, kernel_bispec.cl:52-81:
+ *   mock[k] = |V_k|^2 (k < npow)
+ *   T_t = V'_ab V'_bc V'_ca rho_t, ' = conj when sign < 0
+ *   mock[npow+2t] = Re T_t, mock[npow+2t+1] = Im T_t */
+void orc_vis2data(int npow, int nbis, const double *vis, const int *uvpnt, const int *sign,
+                  const double *phasor, double *mock)
+{
+    for (int k = 0; k < npow; k++)
+        mock[k] = vis[2 * k] * vis[2 * k] + vis[2 * k + 1] * vis[2 * k + 1];
+    for (int t = 0; t < nbis; t++) {
+        double complex T = leg(vis, uvpnt[3 * t], sign[3 * t]) *
+                           leg(vis, uvpnt[3 * t + 1], sign[3 * t + 1]) *
+                           leg(vis, uvpnt[3 * t + 2], sign[3 * t + 2]) *
+                           (phasor[2 * t] + I * phasor[2 * t + 1]);
+        mock[npow + 2 * t] = creal(T);
+        mock[npow + 2 * t + 1] = cimag(T);
+    }
+}
+
+/* chi2 = sum ((mock - data) * err)^2, gpair1/trunk/cpu.c:371-383 (data2chi2),
+ * gpair1/trunk/kernel_chi2.cl:1-11 plus the reduction of gpu.c:1454-1515. */
+double orc_data2chi2(int n, const double *data, const double *err, const double *mock)
+{
+    double c = 0.0;
+    for (int i = 0; i < n; i++) {
+        double r = (mock[i] - data[i]) * err[i];
+        c += r * r;
+    }
+    return c;
+}
+
+/* Gull-Skilling entropy, gpair1/trunk/cpu.c:385-400 and
+ * gpair1/trunk/kernel_entropy_gs.cl:3-26. */
+double orc_entropy(int n, const double *img, const double *model)
+{
+    double S = 0.0;
+    for (int i = 0; i < n; i++) {
+        if (img[i] > 0.0 && model[i] > 0.0)
+            S += img[i] - model[i] - img[i] * log(img[i] / model[i]);
+        else
+            S += -model[i];
+    }
+    return S;
+}
+
+/* entropy gradient, gpair1/trunk/cpu.c:402-412 and
+ * gpair1/trunk/kernel_entropy_gs_grad.cl:3-24: grad = -ln(I/M), and the old
+ * value is KEPT (grad is in/out) where I <= 0 or M <= 0. */
+void orc_entropy_grad(int n, const double *img, const double *model, double *grad)
+{
+    for (int i = 0; i < n; i++)
+        if (img[i] > 0.0 && model[i] > 0.0)
+            grad[i] = -log(img[i] / model[i]);
+}
+
+/* chi2 gradient by the reference's per-pixel formula, gpair1/trunk/cpu.c:279-369
+ * (compute_data_gradient) == kernel_grad_pow.cl:58-103 + kernel_grad_bis.cl:53-137.
+ * O(W^2 (npow + nbis)): small problems only. vis/mock must belong to img. */
+void orc_grad_direct(int W, int nuv, int npow, int nbis, const double *au, const double *av,
+                     const double *img, const double *vis, const double *mock,
+                     const double *data, const double *err, const int *uvpnt, const int *sign,
+                     const double *phasor, double *grad)
+{
+    (void)nuv;
+    double F = orc_flux(W * W, img);
+    double invflux = 1.0 / F;
+#pragma omp parallel for schedule(static)
+    for (int jj = 0; jj < W; jj++) {
+        for (int ii = 0; ii < W; ii++) {
+            double g = 0.0;
+            for (int k = 0; k < npow; k++) {
+                double complex V = vis[2 * k] + I * vis[2 * k + 1];
+                double complex E = turn_exp(au[k] * ii) * turn_exp(-av[k] * jj);
+                g += 4.0 * err[k] * err[k] * invflux * (mock[k] - data[k]) * creal(conj(V) * (E - V));
+            }
+            for (int t = 0; t < nbis; t++) {
+                int a = uvpnt[3 * t], b = uvpnt[3 * t + 1], c = uvpnt[3 * t + 2];
+                double complex vab = leg(vis, a, sign[3 * t]);
+                double complex vbc = leg(vis, b, sign[3 * t + 1]);
+                double complex vca = leg(vis, c, sign[3 * t + 2]);
+                double complex eab = turn_exp(au[a] * ii) * turn_exp(-av[a] * jj);
+                double complex ebc = turn_exp(au[b] * ii) * turn_exp(-av[b] * jj);
+                double complex eca = turn_exp(au[c] * ii) * turn_exp(-av[c] * jj);
+                if (sign[3 * t] < 0) eab = conj(eab);
+                if (sign[3 * t + 1] < 0) ebc = conj(ebc);
+                if (sign[3 * t + 2] < 0) eca = conj(eca);
+                double complex d = ((eab - vab) * vbc * vca + vab * (ebc - vbc) * vca + vab * vbc * (eca - vca)) *
+                                   (phasor[2 * t] + I * phasor[2 * t + 1]) * invflux;
+                int r = npow + 2 * t;
+                g += 2.0 * err[r] * err[r] * (mock[r] - data[r]) * creal(d);
+                g += 2.0 * err[r + 1] * err[r + 1] * mock[r + 1] * cimag(d);
+            }
+            grad[ii + jj * W] = g;
+        }
+    }
+}
+
+/* Per-uv adjoint coefficients of the separable gradient form (derived from
+ * gpair1/trunk/cpu.c:310-365):
+ *   V2:  a_k += 4 e_k^2 (m_k - d_k) conj(V_k)
+ *   T3:  c_t = 2 e_r^2 (m_r - d_r) - i 2 e_i^2 m_i,
+ *        b = c_t rho_t (product of the other two legs'), a_p += b (sign>0) or conj(b)
+ * coef: 2*nuv doubles (zeroed here). Returns c0 = Re sum_p a_p V_p. */
+double orc_grad_coeffs(int nuv, int npow, int nbis, const double *vis, const double *mock,
+                       const double *data, const double *err, const int *uvpnt, const int *sign,
+                       const double *phasor, double *coef)
+{
+    double complex *a = calloc((size_t)nuv, sizeof(double complex));
+    for (int k = 0; k < npow; k++) {
+        double complex V = vis[2 * k] + I * vis[2 * k + 1];
+        a[k] += 4.0 * err[k] * err[k] * (mock[k] - data[k]) * conj(V);
+    }
+    for (int t = 0; t < nbis; t++) {
+        int r = npow + 2 * t;
+        double complex ct = 2.0 * err[r] * err[r] * (mock[r] - data[r]) - I * 2.0 * err[r + 1] * err[r + 1] * mock[r + 1];
+        double complex rho = phasor[2 * t] + I * phasor[2 * t + 1];
+        double complex l[3];
+        for (int j = 0; j < 3; j++)
+            l[j] = leg(vis, uvpnt[3 * t + j], sign[3 * t + j]);
+        for (int j = 0; j < 3; j++) {
+            double complex b = ct * rho * l[(j + 1) % 3] * l[(j + 2) % 3];
+            a[uvpnt[3 * t + j]] += (sign[3 * t + j] < 0) ? conj(b) : b;
+        }
+    }
+    double c0 = 0.0;
+    for (int p = 0; p < nuv; p++) {
+        coef[2 * p] = creal(a[p]);
+        coef[2 * p + 1] = cimag(a[p]);
+        c0 += creal(a[p] * (vis[2 * p] + I * vis[2 * p + 1]));
+    }
+    free(a);
+    return c0;
+}
+
+/* Separable adjoint: grad[y][x] = (Re sum_p a_p X[p][x] Y[p][y] - c0) / F. */
+void orc_grad_separable(int W, int nuv, const double *au, const double *av, const double *coef,
+                        double c0, double F, double *grad)
+{
+    int nthr = 1;
+#ifdef _OPENMP
+    nthr = omp_get_max_threads();
+#endif
+    size_t n = (size_t)W * W;
+    double *part = calloc(n * (size_t)nthr, sizeof(double));
+#pragma omp parallel
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double *g = part + n * (size_t)tid;
+        double complex *xr = malloc(sizeof(double complex) * (size_t)W);
+        double complex *by = malloc(sizeof(double complex) * (size_t)W);
+#pragma omp for schedule(static)
+        for (int p = 0; p < nuv; p++) {
+            double complex a = coef[2 * p] + I * coef[2 * p + 1];
+            for (int i = 0; i < W; i++) {
+                xr[i] = turn_exp(au[p] * (double)i);
+                by[i] = a * turn_exp(-av[p] * (double)i);
+            }
+            for (int y = 0; y < W; y++) {
+                double br = creal(by[y]), bi = cimag(by[y]);
+                double *row = g + (size_t)y * W;
+                for (int x = 0; x < W; x++)
+                    row[x] += br * creal(xr[x]) - bi * cimag(xr[x]);
+            }
+        }
+        free(xr);
+        free(by);
+    }
+    double inv = 1.0 / F;
+    for (size_t i = 0; i < n; i++) {
+        double s = 0.0;
+        for (int t = 0; t < nthr; t++)
+            s += part[(size_t)t * n + i];
+        grad[i] = (s - c0) * inv;
+    }
+    free(part);
+}
+
+/* ------------------------------------------------------------------------
+ * The MEM-CG loop with the GPU-path control flow of the reference:
+ * gpair1/trunk/gpair.c:682-915 (outer loop, beta, bracket phase) and
+ * gpair1/trunk/gpu.c:1947-2065 (gpu_linesearch_zoom).  Scalars are FP64 here.
+ * Build-side addition: stop when chi2/ndof <= chi2r_target after evaluating J
+ * on the current image (the reference runs a fixed iteration count).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int W, nuv, npow, nbis;
+    const double *au, *av, *data, *err, *phasor, *model;
+    const int *uvpnt, *sign;
+    double alpha;
+    /* work */
+    double *vis, *mock, *coef, *gdata, *gent;
+    int nJ, nG;
+} orc_prob;
+
+static double eval_J(orc_prob *P, const double *img, double *chi2, double *S)
+{
+    int n = P->W * P->W;
+    orc_image2vis(P->W, P->nuv, P->au, P->av, img, P->vis);
+    orc_vis2data(P->npow, P->nbis, P->vis, P->uvpnt, P->sign, P->phasor, P->mock);
+    *chi2 = orc_data2chi2(P->npow + 2 * P->nbis, P->data, P->err, P->mock);
+    *S = orc_entropy(n, img, P->model);
+    P->nJ++;
+    return *chi2 - P->alpha * *S;
+}
+
+/* gradient of J at img; requires vis/mock from the last eval_J on img
+ * (the reference's implicit contract, gpair1/trunk/gpu.c:1598-1675). */
+static void eval_grad(orc_prob *P, const double *img, double *g)
+{
+    int n = P->W * P->W;
+    double F = orc_flux(n, img);
+    double c0 = orc_grad_coeffs(P->nuv, P->npow, P->nbis, P->vis, P->mock, P->data, P->err, P->uvpnt,
+                                P->sign, P->phasor, P->coef);
+    orc_grad_separable(P->W, P->nuv, P->au, P->av, P->coef, c0, F, P->gdata);
+    orc_entropy_grad(n, img, P->model, P->gent);
+    for (int i = 0; i < n; i++)
+        g[i] = P->gdata[i] - P->alpha * P->gent[i];
+    P->nG++;
+}
+
+static double dot(int n, const double *a, const double *b)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; i++)
+        s += a[i] * b[i];
+    return s;
+}
+
+static void step_image(int n, const double *cur, const double *d, double s, double minv, double *out)
+{
+    for (int i = 0; i < n; i++) {
+        double t = cur[i] + s * d[i];
+        out[i] = t < minv ? minv : t;
+    }
+}
+
+/* gpair1/trunk/gpu.c:1947-2065 */
+static double zoom(orc_prob *P, const double *cur, const double *d, double *tmp, double *gt, double lo,
+                   double hi, double J_lo, double w1, double J_init, double c1, double c2, double minv,
+                   int *evals)
+{
+    int n = P->W * P->W;
+    double s = 0.0, s_old = 0.0, J_old = 0.0, chi2, S;
+    int counter = 0;
+    for (;;) {
+        if (counter > 0 && (J_old - J_init - w1 * s_old) != 0.0)
+            s = fabs(w1 * s_old * s_old / (2.0 * (J_old - J_init - w1 * s_old)));
+        if (counter == 0 || s < lo || s > hi)
+            s = (hi - lo) / 2.0 + lo;
+        if (fabs(hi - lo) < 1e-14)
+            return lo;
+        if (*evals >= orc_ls_cap)
+            return lo;
+        step_image(n, cur, d, s, minv, tmp);
+        double J = eval_J(P, tmp, &chi2, &S);
+        (*evals)++;
+        if (orc_verbose)
+            fprintf(stderr, "zoom c=%d s=%.6e lo=%.6e hi=%.6e J=%.10e Jlo=%.10e Jinit=%.10e w1=%.4e\n", counter, s,
+                    lo, hi, J, J_lo, J_init, w1);
+        if (J > J_init + c1 * s * w1 || J >= J_lo) {
+            hi = s;
+        } else {
+            eval_grad(P, tmp, gt);
+            double w2 = dot(n, d, gt);
+            if (w2 >= c2 * w1 || counter > 10)
+                return s;
+            if (w2 * (hi - lo) >= 0.0)
+                hi = lo;
+            lo = s;
+        }
+        s_old = s;
+        J_old = J;
+        counter++;
+    }
+}
+
+/* Runs at most max_iter outer iterations of MEM-CG on img (in/out).
+ * method: 0 steepest descent, 1 Fletcher-Reeves, 2 Polak-Ribiere, 3 Hestenes-Stiefel.
+ * trace (may be NULL): 8 doubles per outer iteration:
+ *   J, chi2, S, selected step, beta, nJ (cumulative), nG (cumulative), flux.
+ * Returns the number of outer iterations whose J was evaluated (trace rows). */
+int orc_cg(int W, int nuv, int npow, int nbis, const double *au, const double *av, const double *data,
+           const double *err, const int *uvpnt, const int *sign, const double *phasor, const double *model,
+           double alpha, int method, int max_iter, double chi2r_target, double *img, double *trace)
+{
+    const double c1 = 1e-4, c2 = 0.1, minv = 1e-8, smax = 100.0;
+    int n = W * W, ndof = npow + 2 * nbis;
+    orc_prob P = {W, nuv, npow, nbis, au, av, data, err, phasor, model, uvpnt, sign, alpha,
+                  NULL, NULL, NULL, NULL, NULL, 0, 0};
+    P.vis = malloc(sizeof(double) * 2 * (size_t)nuv);
+    P.coef = malloc(sizeof(double) * 2 * (size_t)nuv);
+    P.mock = malloc(sizeof(double) * (size_t)ndof);
+    P.gdata = malloc(sizeof(double) * (size_t)n);
+    P.gent = calloc((size_t)n, sizeof(double));
+    double *g = calloc((size_t)n, sizeof(double)), *g_old = calloc((size_t)n, sizeof(double));
+    double *d = calloc((size_t)n, sizeof(double)), *tmp = malloc(sizeof(double) * (size_t)n);
+    double *gt = calloc((size_t)n, sizeof(double));
+    double beta = 0.0, sel = 0.0, chi2, S;
+    int rows = 0;
+    for (int it = 0; it < max_iter; it++) {
+        double J = eval_J(&P, img, &chi2, &S);
+        if (trace) {
+            double *r = trace + 8 * rows;
+            r[0] = J; r[1] = chi2; r[2] = S; r[3] = sel; r[4] = beta;
+            r[5] = P.nJ; r[6] = P.nG; r[7] = orc_flux(n, img);
+        }
+        rows++;
+        if (chi2 / ndof <= chi2r_target)
+            break;
+        eval_grad(&P, img, g);
+        /* beta, gpair1/trunk/gpair.c:727-758 */
+        if (it == 0 || method == 0) {
+            beta = 0.0;
+        } else {
+            double gg = dot(n, g, g), go = dot(n, g, g_old), oo = dot(n, g_old, g_old);
+            if (method == 1)
+                beta = gg / oo;
+            if (method == 2) {
+                beta = (gg - go) / oo;
+                if (beta < 0.0) beta = 0.0;
+            }
+            if (method == 3) {
+                beta = (gg - go) / (dot(n, d, g) - dot(n, d, g_old));
+                if (beta < 0.0) beta = 0.0;
+            }
+            if (fabs(go / gg) > 0.5)
+                beta = 0.0;
+        }
+        for (int i = 0; i < n; i++)
+            d[i] = beta * d[i] - g[i];
+        double w1 = dot(n, d, g);
+        /* bracket phase, gpair1/trunk/gpair.c:791-906 */
+        double s = 1.0, s_old = 0.0, J_init = J, J_old = J;
+        int lsit = 1, evals = 0;
+        for (;;) {
+            if (evals >= orc_ls_cap) {
+                sel = s_old;
+                break;
+            }
+            step_image(n, img, d, s, minv, tmp);
+            double Jt = eval_J(&P, tmp, &chi2, &S);
+            evals++;
+            if (orc_verbose)
+                fprintf(stderr, "it=%d bracket s=%.6e J=%.10e Jold=%.10e w1=%.4e\n", it, s, Jt, J_old, w1);
+            if (Jt > J_init + c1 * s * w1 || (Jt >= J_old && lsit > 1)) {
+                sel = zoom(&P, img, d, tmp, gt, s_old, s, J_old, w1, J_init, c1, c2, minv, &evals);
+                break;
+            }
+            eval_grad(&P, tmp, gt);
+            double w2 = dot(n, d, gt);
+            if (fabs(w2) <= -c2 * w1) {
+                sel = s;
+                break;
+            }
+            if (w2 >= 0.0) {
+                sel = zoom(&P, img, d, tmp, gt, s, s_old, Jt, w1, J_init, c1, c2, minv, &evals);
+                break;
+            }
+            s_old = s;
+            J_old = Jt;
+            s = 10.0 * s;
+            if (s > smax) s = smax;
+            lsit++;
+        }
+        step_image(n, img, d, sel, minv, img);
+        memcpy(g_old, g, sizeof(double) * (size_t)n);
+    }
+    free(P.vis); free(P.coef); free(P.mock); free(P.gdata); free(P.gent);
+    free(g); free(g_old); free(d); free(tmp); free(gt);
+    return rows;
+}
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

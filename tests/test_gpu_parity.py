@@ -1,0 +1,164 @@
+"""HIP path vs the FP64 oracle through the C ABI (MI355X only).
+
+Tolerances (FP32 MFMA contraction, FP32 per-datum model, FP64 reductions):
+  visibilities   max|dV| <= 2e-6 * max|V|
+  mock data      max|dm| <= 1e-5 * max|m|
+  chi2, entropy  relative 2e-5
+  data gradient  max|dg| <= 2e-4 * max|g|   (adjoint sums 2*nuv FP32 products)
+"""
+import numpy as np
+import pytest
+
+from gpair_amd import synth
+from gpair_amd.api import GpairContext, GpairError
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+ALPHA = 100.0
+
+
+def _cases():
+    return [
+        ("cfg1", dict(seed=1)),
+        ("cfg1_w128", dict(seed=2, W=128, ntel=7, nch=6, nsnap=5)),
+        ("cfg2", dict(seed=3)),
+    ]
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle()
+
+
+def _problem(name, kw):
+    cfg = "cfg2" if name.startswith("cfg2") else "cfg1"
+    return synth.make_problem(cfg, **kw)
+
+
+def _image(prob, seed):
+    rng = np.random.default_rng(seed)
+    img = prob.start.astype(np.float64) * (1.0 + rng.random(prob.W * prob.W))
+    return img.astype(np.float32)
+
+
+@pytest.mark.parametrize("name,kw", _cases(), ids=[c[0] for c in _cases()])
+def test_objective_and_gradient(name, kw, orc):
+    prob = _problem(name, kw)
+    print(f"case {name}: W={prob.W} nuv={prob.nuv} nbis={prob.nbis}", flush=True)
+    P = orc.unpack(prob)
+    img = _image(prob, 7)
+    with GpairContext.from_problem(prob, image=img) as ctx:
+        chi2 = ctx.gpu_get_chi2_curr()
+        S = ctx.gpu_get_entropy_curr()
+        vis = ctx.vis().astype(np.complex128)
+        mock = ctx.mock().astype(np.float64)
+        ctx.gpu_compute_data_gradient_curr()
+        ctx.gpu_compute_entropy_gradient_curr()
+        ctx.gpu_compute_criterion_gradient_curr(ALPHA)
+        gd = ctx.get("dg").astype(np.float64)
+        ge = ctx.get("eg").astype(np.float64)
+        g = ctx.get("fgn").astype(np.float64)
+    J, chi2_o, S_o, vis_o, mock_o, F = orc.objective(P, img.astype(np.float64), ALPHA)
+    g_o, gd_o, ge_o = orc.gradient(P, img.astype(np.float64), ALPHA, vis_o, mock_o)
+    ev = np.abs(vis - vis_o).max() / np.abs(vis_o).max()
+    em = np.abs(mock - mock_o).max() / np.abs(mock_o).max()
+    ec = abs(chi2 - chi2_o) / chi2_o
+    es = abs(S - S_o) / abs(S_o)
+    eg = np.abs(gd - gd_o).max() / np.abs(gd_o).max()
+    ee = np.abs(ge - ge_o).max() / np.abs(ge_o).max()
+    ef = np.abs(g - g_o).max() / np.abs(g_o).max()
+    print(f"  vis {ev:.2e} mock {em:.2e} chi2 {ec:.2e} S {es:.2e} grad {eg:.2e} egrad {ee:.2e} full {ef:.2e}",
+          flush=True)
+    assert ev <= 2e-6
+    assert em <= 1e-5
+    assert ec <= 2e-5
+    assert es <= 2e-5
+    assert eg <= 2e-4
+    assert ee <= 1e-5
+    assert ef <= 2e-4
+
+
+def test_temp_path_and_vector_ops(orc):
+    prob = synth.make_problem("cfg1", seed=4)
+    img = _image(prob, 3)
+    rng = np.random.default_rng(5)
+    d = (rng.standard_normal(prob.W ** 2) * 1e-5).astype(np.float32)
+    with GpairContext.from_problem(prob, image=img) as ctx:
+        ctx.put("dd", d)
+        ctx.gpu_update_tempimage(0.5, 1e-8, "dd")
+        ti = ctx.get("ti")
+        ref = np.maximum(img + np.float32(0.5) * d, np.float32(1e-8))
+        assert np.array_equal(ti, ref)
+        c_t = ctx.gpu_get_chi2_temp()
+        J, c_o, *_ = orc.objective(orc.unpack(prob), ti.astype(np.float64), ALPHA)
+        assert abs(c_t - c_o) / c_o <= 2e-5
+        # descent direction d = beta d - g
+        g = rng.standard_normal(prob.W ** 2).astype(np.float32)
+        ctx.put("fgn", g)
+        ctx.gpu_compute_descent_dir(0.25)
+        assert np.allclose(ctx.get("dd"), np.float32(0.25) * d - g, rtol=0, atol=1e-6)
+        # scalar product in FP64
+        dot = ctx.gpu_get_scalprod("fgn", "dd")
+        ref = float(np.dot(g.astype(np.float64), ctx.get("dd").astype(np.float64)))
+        assert abs(dot - ref) <= 1e-12 * abs(ref) * 10 + 1e-12
+        ctx.gpu_backup_gradient("fgn", "fg")
+        assert np.array_equal(ctx.get("fg"), g)
+        ctx.gpu_update_image(1.0, 1e-8, "dd")
+        assert np.array_equal(ctx.get("ci"), np.maximum(img + ctx.get("dd"), np.float32(1e-8)))
+
+
+def test_dft_upload_matches_device_tables(orc):
+    """gpu_copy_dft (reference [W][nuv] layout) == gpair_set_uv (device tables)."""
+    prob = synth.make_problem("cfg1", seed=6)
+    img = _image(prob, 1)
+    X, Y = orc.tables(prob.W, prob.au(), prob.av())
+    with GpairContext.from_problem(prob, image=img) as a:
+        c1 = a.gpu_get_chi2_curr()
+        v1 = a.vis()
+    with GpairContext(0) as b:
+        b.gpu_copy_data(prob.data, prob.err, prob.nuv, prob.phasor, prob.npow, prob.uvpnt4(), prob.sign4(),
+                        prob.model, prob.W)
+        b.gpu_copy_dft(X.T.astype(np.complex64), Y.T.astype(np.complex64))
+        b.gpu_build_kernels()
+        b.gpu_copy_image(img)
+        c2 = b.gpu_get_chi2_curr()
+        v2 = b.vis()
+    assert np.abs(v1 - v2).max() <= 1e-6 * np.abs(v1).max()
+    assert abs(c1 - c2) <= 1e-6 * c1
+
+
+def test_errors_are_reported():
+    prob = synth.make_problem("cfg1", seed=1)
+    with GpairContext(0) as ctx:
+        with pytest.raises(GpairError):
+            ctx.gpu_get_chi2_curr()          # before any data
+        bad = prob.uvpnt4().copy()
+        bad[0, 0] = prob.nuv + 5
+        with pytest.raises(GpairError):
+            ctx.gpu_copy_data(prob.data, prob.err, prob.nuv, prob.phasor, prob.npow, bad, prob.sign4(),
+                              prob.model, prob.W)
+        with pytest.raises(GpairError):
+            ctx.gpu_copy_data(prob.data, prob.err, prob.nuv, prob.phasor, prob.npow, prob.uvpnt4(),
+                              prob.sign4(), np.ones(100 * 100, np.float32), 100)   # width not a multiple of 64
+
+
+def test_cg_loop_matches_oracle(orc):
+    """the C host loop (gpair_cg_run) against the FP64 oracle loop, same rules."""
+    prob = synth.make_problem("cfg1", seed=1)
+    P = orc.unpack(prob)
+    nit = 8
+    with GpairContext.from_problem(prob) as ctx:
+        res, tr = ctx.cg_run(ALPHA, method=2, max_iter=nit, chi2r_target=0.0)
+        img_gpu = ctx.get("ci").astype(np.float64)
+        J_gpu = ctx.gpu_get_chi2_curr() - ALPHA * ctx.gpu_get_entropy_curr()
+    img_o, tr_o = orc.cg(P, prob.start.astype(np.float64), ALPHA, method=2, max_iter=nit, chi2r_target=0.0)
+    print("gpu J", tr[:, 0])
+    print("orc J", tr_o[:, 0])
+    assert res.iterations == nit
+    rel = np.abs(tr[:, 0] - tr_o[:, 0]) / np.abs(tr_o[:, 0])
+    print("trajectory rel diff", rel)
+    assert rel.max() <= 1e-3
+    # the objective of the GPU's final image, re-evaluated in FP64
+    J_o = orc.objective(P, img_gpu, ALPHA)[0]
+    assert abs(J_gpu - J_o) / abs(J_o) <= 2e-5
