@@ -74,6 +74,8 @@ SIGNATURES = {
     "gpair_cg_run": (_i, [_vp, C.POINTER(CgParams), C.POINTER(CgResult), _vp, _i]),
     "gpair_sizes": (_i, [_vp, _pi, _pi, _pi, _pi]),
     "gpair_sync": (_i, [_vp]),
+    "gpair_timing_enable": (_i, [_vp, _i]),
+    "gpair_timing_read": (_i, [_vp, _i, _pi, _pd]),
 }
 
 _LIB = None

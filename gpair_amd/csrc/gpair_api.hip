@@ -8,6 +8,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <new>
 #include <vector>
 
 #include "../../include/gpair_hip.h"
@@ -47,6 +48,10 @@ struct gpair_ctx {
     // adjoint split-K partials
     float *adj_part;
     int adj_nsplit, adj_pper;
+    // optional per-launch timing of the DFT kernels (HIP events on the context stream)
+    int timing;
+    std::vector<hipEvent_t> ev[GPAIR_TK_COUNT];
+    size_t ev_used[GPAIR_TK_COUNT];
     char msg[512];
 };
 
@@ -135,13 +140,13 @@ extern "C" int gpu_init(gpair_ctx **out, int device)
         return GPAIR_E_HIP;
     if (device < 0 || device >= ndev)
         return GPAIR_E_ARG;
-    gpair_ctx *c = (gpair_ctx *)calloc(1, sizeof(gpair_ctx));
+    gpair_ctx *c = new (std::nothrow) gpair_ctx();
     if (!c)
         return GPAIR_E_NOMEM;
     c->device = device;
     c->precision = GPAIR_PREC_FP32;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        free(c);
+        delete c;
         return GPAIR_E_HIP;
     }
     int rc;
@@ -186,9 +191,12 @@ extern "C" int gpu_cleanup(gpair_ctx *c)
     dfree(&c->scal); dfree(&c->invflux); dfree(&c->part); dfree(&c->ticket); dfree(&c->adj_part);
     if (c->host_scal)
         (void)hipHostFree(c->host_scal);
+    for (int k = 0; k < GPAIR_TK_COUNT; k++)
+        for (hipEvent_t e : c->ev[k])
+            (void)hipEventDestroy(e);
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
-    free(c);
+    delete c;
     return GPAIR_OK;
 }
 
@@ -473,6 +481,69 @@ static int read_scalar(gpair_ctx *c, int slot, double *out)
 }
 
 // ---------------------------------------------------------------------------
+// kernel timing: event pairs around the DFT launches while enabled
+// ---------------------------------------------------------------------------
+#define GPAIR_TIMING_MAX_EVENTS 400000
+
+static hipEvent_t timing_event(gpair_ctx *c, int k)
+{
+    if (c->ev_used[k] >= GPAIR_TIMING_MAX_EVENTS)
+        return nullptr;
+    if (c->ev_used[k] == c->ev[k].size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess)
+            return nullptr;
+        c->ev[k].push_back(e);
+    }
+    return c->ev[k][c->ev_used[k]++];
+}
+
+// records a start/stop pair around one launch; pairs are kept even-aligned
+#define TIMED_LAUNCH(c, k, launch)                                                   \
+    do {                                                                             \
+        hipEvent_t a_ = nullptr, b_ = nullptr;                                       \
+        if ((c)->timing && (c)->ev_used[k] + 2 <= GPAIR_TIMING_MAX_EVENTS) {         \
+            a_ = timing_event(c, k);                                                 \
+            b_ = a_ ? timing_event(c, k) : nullptr;                                  \
+            if (a_ && !b_)                                                           \
+                (c)->ev_used[k]--;                                                   \
+        }                                                                            \
+        if (a_ && b_)                                                                \
+            (void)hipEventRecord(a_, (c)->stream);                                   \
+        launch;                                                                      \
+        if (a_ && b_)                                                                \
+            (void)hipEventRecord(b_, (c)->stream);                                   \
+    } while (0)
+
+extern "C" int gpair_timing_enable(gpair_ctx *c, int on)
+{
+    if (!c)
+        return GPAIR_E_ARG;
+    c->timing = on ? 1 : 0;
+    for (int k = 0; k < GPAIR_TK_COUNT; k++)
+        c->ev_used[k] = 0;
+    return GPAIR_OK;
+}
+
+extern "C" int gpair_timing_read(gpair_ctx *c, int kernel, int *count, double *total_ms)
+{
+    if (!c || kernel < 0 || kernel >= GPAIR_TK_COUNT || !count || !total_ms)
+        return fail(c, GPAIR_E_ARG, "gpair_timing_read: bad arguments");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    int n = 0;
+    for (size_t i = 0; i + 1 < c->ev_used[kernel]; i += 2) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[kernel][i], c->ev[kernel][i + 1]));
+        tot += ms;
+        n++;
+    }
+    *count = n;
+    *total_ms = tot;
+    return GPAIR_OK;
+}
+
+// ---------------------------------------------------------------------------
 // launches
 // ---------------------------------------------------------------------------
 static int launch_flux(gpair_ctx *c, const float *img)
@@ -486,11 +557,17 @@ static int launch_forward(gpair_ctx *c, const float *img)
 {
     const unsigned grid = (unsigned)((c->nuv + FWD_PT - 1) / FWD_PT);
     switch (c->W / 64) {
-    case 1: k_vis_fwd_f32<1><<<grid, 256, 0, c->stream>>>(img, c->X, c->Y, c->nuv, c->invflux, c->vis); break;
-    case 2: k_vis_fwd_f32<2><<<grid, 256, 0, c->stream>>>(img, c->X, c->Y, c->nuv, c->invflux, c->vis); break;
-    case 3: k_vis_fwd_f32<3><<<grid, 256, 0, c->stream>>>(img, c->X, c->Y, c->nuv, c->invflux, c->vis); break;
-    case 4: k_vis_fwd_f32<4><<<grid, 256, 0, c->stream>>>(img, c->X, c->Y, c->nuv, c->invflux, c->vis); break;
-    case 8: k_vis_fwd_f32<8><<<grid, 256, 0, c->stream>>>(img, c->X, c->Y, c->nuv, c->invflux, c->vis); break;
+#define FWD_CASE(NB)                                                                                       \
+    case NB:                                                                                               \
+        TIMED_LAUNCH(c, GPAIR_TK_FORWARD,                                                                  \
+                     (k_vis_fwd_f32<NB><<<grid, 256, 0, c->stream>>>(img, c->X, c->Y, c->nuv, c->invflux, c->vis))); \
+        break;
+    FWD_CASE(1)
+    FWD_CASE(2)
+    FWD_CASE(3)
+    FWD_CASE(4)
+    FWD_CASE(8)
+#undef FWD_CASE
     default: return fail(c, GPAIR_E_ARG, "unsupported width %d", c->W);
     }
     LAUNCHCHK(c);
@@ -520,7 +597,8 @@ static int launch_data_gradient(gpair_ctx *c, const float *img)
                                                                c->part, c->ticket, c->scal);
     LAUNCHCHK(c);
     dim3 grid((c->W / 64) * (c->W / 64), c->adj_nsplit);
-    k_adj_f32<<<grid, 256, 0, c->stream>>>(c->coef, c->X, c->Y, c->nuv, c->W, c->adj_pper, c->adj_part);
+    TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,
+                 (k_adj_f32<<<grid, 256, 0, c->stream>>>(c->coef, c->X, c->Y, c->nuv, c->W, c->adj_pper, c->adj_part)));
     LAUNCHCHK(c);
     k_grad_finish<<<ew_blocks(c->n), 256, 0, c->stream>>>(c->adj_part, c->adj_nsplit, c->n, c->scal, c->gdata.p);
     LAUNCHCHK(c);

@@ -15,11 +15,13 @@ same packed arrays directly from a seeded array layout and sky model:
 
 Noise conventions (recorded in ``Problem.meta``):
   sigma_V2  = max(0.02 * V2_true, 1e-4)      (2 % relative, absolute floor)
-  sigma_amp = max(0.05 * |T3_true|, 1e-5)    (5 % relative, absolute floor)
+  sigma_amp = 0.05 * |T3_true|                (5 % relative)
   sigma_phi = 1 degree
 
-Sky visibilities are analytic (uniform / limb-darkened disks, Gaussians,
-rings, points), so the data carry no pixelisation of their own.
+Skies are uniform / limb-darkened disks, Gaussians, rings and points.  By
+default the sky is rendered on the reconstruction grid and the data are the
+FP64 DFT of that image (chi2/ndof of the truth ~ 1); closed-form visibilities of
+the continuous sky are available with ``vis_model="analytic"``.
 """
 from __future__ import annotations
 
@@ -51,6 +53,7 @@ class Problem:
     start: np.ndarray             # float32 [W*W] starting image
     truth_vis: np.ndarray         # complex128 [nuv] noiseless visibilities
     meta: dict
+    truth: np.ndarray = None      # float32 [W*W] sky on the grid, scaled to the prior flux
 
     @property
     def nuv(self) -> int:
@@ -186,51 +189,133 @@ def build_layout(rng, ntel, nch, nsnap, uvmax):
 # ---------------------------------------------------------------------------
 # sky models for the configurations
 # ---------------------------------------------------------------------------
-def sky_components(kind, rng, fov_mas):
-    """list of (flux, vis_fn(u, v) -> complex, x_mas, y_mas) with fluxes summing to 1."""
+def sky_components(kind, rng, fov_mas, pix):
+    """list of component dicts (kind, flux, size parameters, x, y in mas from the
+    image centre); fluxes sum to 1.  Point sources sit on pixel centres."""
     comps = []
-    if kind == "disk_spots":  # uniform disk r = 1.5 mas + 2 Gaussian spots
+
+    def snap(v):
+        return round(v / pix) * pix
+
+    if kind == "disk_spots":  # uniform disk r = 1.5 mas + 2 Gaussian spots (FWHM 0.3 mas)
         fs = rng.uniform(0.10, 0.30, size=2)
-        comps.append((1.0 - fs.sum(), lambda u, v: vis_uniform_disk(u, v, 1.5), 0.0, 0.0))
+        comps.append(dict(kind="disk", flux=1.0 - fs.sum(), R=1.5, x=0.0, y=0.0))
         for f in fs:
             r, th = rng.uniform(0.2, 1.2), rng.uniform(0, 2 * math.pi)
-            comps.append((f, lambda u, v: vis_gauss(u, v, 0.3), r * math.cos(th), r * math.sin(th)))
+            comps.append(dict(kind="gauss", flux=f, fwhm=0.3, x=r * math.cos(th), y=r * math.sin(th)))
     elif kind == "ld_disk_spots_companion":
         nsp = int(rng.integers(1, 6))
         fs = rng.uniform(0.03, 0.10, size=nsp)
         fc = rng.uniform(0.01, 0.05)
         R = 0.2 * fov_mas
-        comps.append((1.0 - fs.sum() - fc, lambda u, v: vis_ld_disk(u, v, R, 0.4), 0.0, 0.0))
+        comps.append(dict(kind="ld_disk", flux=1.0 - fs.sum() - fc, R=R, mu=0.4, x=0.0, y=0.0))
         for f in fs:
             r, th = rng.uniform(0.1, 0.8) * R, rng.uniform(0, 2 * math.pi)
-            fw = rng.uniform(0.15, 0.4) * R
-            comps.append((f, (lambda fw_: (lambda u, v: vis_gauss(u, v, fw_)))(fw), r * math.cos(th), r * math.sin(th)))
+            comps.append(dict(kind="gauss", flux=f, fwhm=rng.uniform(0.15, 0.4) * R, x=r * math.cos(th),
+                              y=r * math.sin(th)))
         r, th = rng.uniform(0.3, 0.42) * fov_mas, rng.uniform(0, 2 * math.pi)
-        comps.append((fc, lambda u, v: np.ones_like(u), r * math.cos(th), r * math.sin(th)))
+        comps.append(dict(kind="point", flux=fc, x=snap(r * math.cos(th)), y=snap(r * math.sin(th))))
     elif kind == "ring_disk":
         R = 0.2 * fov_mas
         fr = rng.uniform(0.3, 0.6)
-        comps.append((fr, lambda u, v: vis_ring(u, v, R), 0.0, 0.0))
-        comps.append((1.0 - fr, lambda u, v: vis_uniform_disk(u, v, 0.4 * R), 0.1 * R, -0.05 * R))
+        comps.append(dict(kind="ring", flux=fr, R=R, width=0.06 * R, x=0.0, y=0.0))
+        comps.append(dict(kind="disk", flux=1.0 - fr, R=0.4 * R, x=0.1 * R, y=-0.05 * R))
     else:
         raise ValueError(kind)
     return comps
 
 
+def component_vis(c, u, v):
+    """analytic visibility of one component (flux-weighted, position phase included)."""
+    k = c["kind"]
+    if k == "disk":
+        base = vis_uniform_disk(u, v, c["R"])
+    elif k == "ld_disk":
+        base = vis_ld_disk(u, v, c["R"], c["mu"])
+    elif k == "gauss":
+        base = vis_gauss(u, v, c["fwhm"])
+    elif k == "point":
+        base = np.ones_like(u)
+    elif k == "ring":
+        base = vis_ring(u, v, c["R"]) * vis_gauss(u, v, c["width"])
+    else:
+        raise ValueError(k)
+    return c["flux"] * base * shift(u, v, c["x"], c["y"])
+
+
+def render(comps, W, pix, oversample=4):
+    """sky image on the W x W grid, image[x + W*y], pixel (x, y) at ((x - W/2) pix,
+    (y - W/2) pix) mas; each extended component is box-averaged over
+    oversample^2 sub-pixels and normalised to its flux."""
+    o = oversample
+    sub = (np.arange(W * o) + 0.5) / o - 0.5 - W // 2      # sub-pixel centres in pixels
+    X, Y = np.meshgrid(sub * pix, sub * pix)               # [y, x] in mas
+    img = np.zeros((W, W))
+    for c in comps:
+        if c["kind"] == "point":
+            ix = int(round(c["x"] / pix)) + W // 2
+            iy = int(round(c["y"] / pix)) + W // 2
+            img[iy, ix] += c["flux"]
+            continue
+        dx, dy = X - c["x"], Y - c["y"]
+        r2 = dx * dx + dy * dy
+        if c["kind"] == "disk":
+            f = (r2 <= c["R"] ** 2).astype(np.float64)
+        elif c["kind"] == "ld_disk":
+            t = np.clip(1.0 - r2 / c["R"] ** 2, 0.0, None)
+            f = np.where(r2 <= c["R"] ** 2, 1.0 - c["mu"] * (1.0 - np.sqrt(t)), 0.0)
+        elif c["kind"] == "gauss":
+            f = np.exp(-4.0 * math.log(2.0) * r2 / c["fwhm"] ** 2)
+        elif c["kind"] == "ring":
+            s = c["width"] / (2.0 * math.sqrt(2.0 * math.log(2.0)))
+            f = np.exp(-(np.sqrt(r2) - c["R"]) ** 2 / (2.0 * s * s))
+        else:
+            raise ValueError(c["kind"])
+        f = f.reshape(W, o, W, o).mean(axis=(1, 3))
+        img += c["flux"] * f / f.sum()
+    return img.ravel()
+
+
+def dft64(img, W, au, av, chunk=4096):
+    """FP64 separable DFT V_p = (1/F) sum_y Y[p,y] sum_x X[p,x] I[y][x] of a grid image."""
+    I2 = img.reshape(W, W)
+    F = img.sum()
+    xs = np.arange(W, dtype=np.float64)
+    out = np.empty(au.size, np.complex128)
+    for a in range(0, au.size, chunk):
+        b = min(a + chunk, au.size)
+        tx = np.outer(au[a:b], xs)
+        ty = np.outer(-av[a:b], xs)
+        Xp = np.exp(2j * math.pi * (tx - np.rint(tx)))
+        Yp = np.exp(2j * math.pi * (ty - np.rint(ty)))
+        T = Xp @ I2.T                     # T[p, y] = sum_x X[p, x] I[y, x]
+        out[a:b] = (T * Yp).sum(axis=1) / F
+    return out
+
+
 CONFIGS = {
     # reference CPU-runnable case: 64x64 @ 0.1 mas, 6 telescopes, 8 ch x 4 snapshots
-    "cfg1": dict(W=64, pix=0.1, ntel=6, nch=8, nsnap=4, uvmax=3e8, sky="disk_spots"),
+    "cfg1": dict(W=64, pix=0.1, ntel=6, nch=8, nsnap=4, uvmax=3e8, sky="disk_spots", alpha=0.025),
     # headline: 256x256 @ 0.05 mas, 9 telescopes, 64 ch x 16 snapshots
-    "cfg2": dict(W=256, pix=0.05, ntel=9, nch=64, nsnap=16, uvmax=3e8, sky="ld_disk_spots_companion"),
+    "cfg2": dict(W=256, pix=0.05, ntel=9, nch=64, nsnap=16, uvmax=3e8, sky="ld_disk_spots_companion", alpha=0.2),
     # precision variant: 512x512, 256 channels
-    "cfg3": dict(W=512, pix=0.05, ntel=9, nch=256, nsnap=16, uvmax=3e8, sky="ring_disk"),
+    "cfg3": dict(W=512, pix=0.05, ntel=9, nch=256, nsnap=16, uvmax=3e8, sky="ring_disk", alpha=0.2),
 }
 
 
 def make_problem(cfg="cfg1", seed=1, W=None, pix=None, ntel=None, nch=None, nsnap=None, uvmax=None,
-                 sky=None, prior_flux=1.0, noise=True) -> Problem:
+                 sky=None, prior_flux=None, noise=True, vis_model="grid") -> Problem:
     """Seeded synthetic problem.  Named configs fix the sizes; keyword
-    overrides allow small test shapes with the same construction."""
+    overrides allow small test shapes with the same construction.
+
+    vis_model: "grid"     -- the sky is rendered on the reconstruction grid and its
+                             visibilities are the FP64 DFT of that image, so the
+                             truth image reaches chi2/ndof ~ 1 (default);
+               "analytic" -- closed-form visibilities of the continuous sky.
+    prior_flux: total flux of the flat prior (default W*W: mean prior pixel 1).
+    meta["alpha"] is the entropy weight used with this prior scale: J(cI; cM)
+    with alpha/c equals J(I; M) with alpha, so alpha scales inversely with the
+    prior flux (steps of the line search scale with its square)."""
     c = dict(CONFIGS[cfg])
     for k, val in (("W", W), ("pix", pix), ("ntel", ntel), ("nch", nch), ("nsnap", nsnap),
                    ("uvmax", uvmax), ("sky", sky)):
@@ -241,20 +326,27 @@ def make_problem(cfg="cfg1", seed=1, W=None, pix=None, ntel=None, nch=None, nsna
     u = uv[:, 0].astype(np.float64)
     v = uv[:, 1].astype(np.float64)
     W_ = c["W"]
-    fov = W_ * c["pix"]
-    comps = sky_components(c["sky"], rng, fov)
-    # the sky is centred on pixel (W/2, W/2) of the DFT frame
-    xc = (W_ // 2) * float(np.float32(c["pix"]))
-    V = np.zeros(u.shape, np.complex128)
-    for f, fn, x, y in comps:
-        V += f * fn(u, v) * shift(u, v, xc + x, xc + y)
+    pixf = float(np.float32(c["pix"]))
+    fov = W_ * pixf
+    comps = sky_components(c["sky"], rng, fov, pixf)
+    truth = render(comps, W_, pixf)
+    alpha_px = RPMAS * pixf
+    if vis_model == "grid":
+        V = dft64(truth, W_, alpha_px * u, alpha_px * v)
+    elif vis_model == "analytic":
+        xc = (W_ // 2) * pixf     # image centre in the DFT frame
+        V = np.zeros(u.shape, np.complex128)
+        for comp in comps:
+            V += component_vis(dict(comp, x=comp["x"] + xc, y=comp["y"] + xc), u, v)
+    else:
+        raise ValueError(vis_model)
     npow = u.size
     nbis = uvpnt.shape[0]
     v2 = np.abs(V) ** 2
     legs = np.where(sign < 0, np.conj(V[uvpnt]), V[uvpnt])
     T3 = legs.prod(axis=1)
     sig_v2 = np.maximum(0.02 * v2, 1e-4)
-    sig_amp = np.maximum(0.05 * np.abs(T3), 1e-5)
+    sig_amp = np.maximum(0.05 * np.abs(T3), 1e-30)
     sig_phi = np.full(nbis, 1.0)
     if noise:
         v2_obs = v2 + rng.normal(size=npow) * sig_v2
@@ -280,11 +372,14 @@ def make_problem(cfg="cfg1", seed=1, W=None, pix=None, ntel=None, nch=None, nsna
     phasor = np.exp(-1j * phi_obs.astype(np.float64) * PI_REF / 180.0).astype(np.complex64)
     # flat prior of the given total flux (set_model type 0, gpair1/trunk/cpu.c:653-662),
     # start image = max(prior, 1e-8) (gpair1/trunk/gpair.c:217-225)
+    if prior_flux is None:
+        prior_flux = float(W_ * W_)
     model = np.full(W_ * W_, prior_flux / float(W_ * W_), np.float32)
     start = np.maximum(model, np.float32(1e-8))
     meta = dict(cfg=cfg, seed=seed, nbase=nbase, ntri=ntri, sky=c["sky"], W=W_, pix=c["pix"],
-                noise="sigV2=max(0.02*V2,1e-4); sigamp=max(0.05*|T3|,1e-5); sigphi=1deg",
-                uvmax=c["uvmax"], ntel=c["ntel"], nch=c["nch"], nsnap=c["nsnap"])
+                noise="sigV2=max(0.02*V2,1e-4); sigamp=0.05*|T3|; sigphi=1deg",
+                uvmax=c["uvmax"], ntel=c["ntel"], nch=c["nch"], nsnap=c["nsnap"], vis_model=vis_model,
+                prior_flux=prior_flux, components=comps, alpha=c["alpha"])
     return Problem(W=W_, pixellation=c["pix"], u=uv[:, 0].copy(), v=uv[:, 1].copy(), npow=npow,
                    nbis=nbis, data=data, err=err, phasor=phasor, uvpnt=uvpnt, sign=sign,
-                   model=model, start=start, truth_vis=V, meta=meta)
+                   model=model, start=start, truth_vis=V, meta=meta, truth=(truth * prior_flux).astype(np.float32))

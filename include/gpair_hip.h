@@ -174,6 +174,15 @@ void gpair_cg_default_params(gpair_cg_params *p);
 int gpair_cg_run(gpair_ctx *ctx, const gpair_cg_params *params, gpair_cg_result *result, double *trace,
                  int trace_rows);
 
+/* ---- measurement (no reference counterpart; the reference queue has no
+ * profiling, gpair1/trunk/gpu.c:1724) ------------------------------------ */
+enum { GPAIR_TK_FORWARD = 0, GPAIR_TK_ADJOINT = 1, GPAIR_TK_COUNT = 2 };
+/* start (on != 0) or stop recording HIP events around every forward / adjoint
+ * DFT launch on the context stream; enabling clears what was recorded */
+int gpair_timing_enable(gpair_ctx *ctx, int on);
+/* number of timed launches of a kernel and their summed duration (ms) */
+int gpair_timing_read(gpair_ctx *ctx, int kernel, int *count, double *total_ms);
+
 /* problem sizes of a context (0 when not set) */
 int gpair_sizes(const gpair_ctx *ctx, int *W, int *nuv, int *npow, int *nbis);
 /* synchronise the context's stream */
