@@ -29,6 +29,16 @@ class CgResult(C.Structure):
                 ("entropy", _d), ("chi2r", _d), ("converged", _i), ("status", _i)]
 
 
+class PcgParams(C.Structure):
+    _fields_ = [("alpha", _d), ("precond", _i), ("max_iter", _i), ("chi2r_target", _d), ("minvalue", _d),
+                ("wolfe1", _d), ("verbose", _i)]
+
+
+class PcgResult(C.Structure):
+    _fields_ = [("iterations", _i), ("forward_dfts", _i), ("adjoint_dfts", _i), ("ls_batches", _i), ("J", _d),
+                ("chi2", _d), ("entropy", _d), ("chi2r", _d), ("converged", _i), ("stalled", _i), ("status", _i)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "gpair_strerror": (C.c_char_p, [_i]),
@@ -76,6 +86,9 @@ SIGNATURES = {
     "gpair_sync": (_i, [_vp]),
     "gpair_timing_enable": (_i, [_vp, _i]),
     "gpair_timing_read": (_i, [_vp, _i, _pi, _pd]),
+    "gpair_pcg_default_params": (None, [C.POINTER(PcgParams)]),
+    "gpair_pcg_run": (_i, [_vp, C.POINTER(PcgParams), C.POINTER(PcgResult), _vp, _i]),
+    "gpair_set_pfactor": (_i, [_vp, _vp, _i]),
 }
 
 _LIB = None

@@ -222,3 +222,22 @@ class GpairContext:
         rc = self.L.gpair_cg_run(self.h, C.byref(p), C.byref(res), _ptr(trace), trace.shape[0])
         self._check(rc, "gpair_cg_run")
         return res, trace[:res.iterations]
+
+    def set_pfactor(self, pf):
+        if pf is None:
+            self._check(self.L.gpair_set_pfactor(self.h, None, 0), "gpair_set_pfactor")
+            return
+        a = np.ascontiguousarray(pf, np.float32)
+        self._check(self.L.gpair_set_pfactor(self.h, _ptr(a), a.size), "gpair_set_pfactor")
+
+    def pcg_run(self, alpha, precond=1, max_iter=1000, chi2r_target=1.0, verbose=False):
+        """accelerated loop (preconditioned PR-CG with the batched line search)."""
+        p = _lib.PcgParams()
+        self.L.gpair_pcg_default_params(C.byref(p))
+        p.alpha, p.precond, p.max_iter = float(alpha), int(precond), int(max_iter)
+        p.chi2r_target, p.verbose = float(chi2r_target), int(verbose)
+        res = _lib.PcgResult()
+        trace = np.zeros((max(max_iter, 1), 8))
+        rc = self.L.gpair_pcg_run(self.h, C.byref(p), C.byref(res), _ptr(trace), trace.shape[0])
+        self._check(rc, "gpair_pcg_run")
+        return res, trace[:min(res.iterations, trace.shape[0])]

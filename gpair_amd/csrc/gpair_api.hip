@@ -48,6 +48,11 @@ struct gpair_ctx {
     // adjoint split-K partials
     float *adj_part;
     int adj_nsplit, adj_pper;
+    // accelerated loop
+    gpair_buf_s z, zold;
+    float2 *Ud;
+    float *pfac;
+    double *ls_out, *host_ls;   // device / pinned host, 2*LS_K
     // optional per-launch timing of the DFT kernels (HIP events on the context stream)
     int timing;
     std::vector<hipEvent_t> ev[GPAIR_TK_COUNT];
@@ -151,11 +156,13 @@ extern "C" int gpu_init(gpair_ctx **out, int device)
     }
     int rc;
     if ((rc = dalloc(c, &c->scal, GPAIR_NSCAL)) || (rc = dalloc(c, &c->invflux, 4)) ||
-        (rc = dalloc(c, &c->part, (size_t)RED_MAX_BLOCKS * 4)) || (rc = dalloc(c, &c->ticket, 4))) {
+        (rc = dalloc(c, &c->part, (size_t)RED_MAX_BLOCKS * RED_MAX_NV)) || (rc = dalloc(c, &c->ticket, 4)) ||
+        (rc = dalloc(c, &c->ls_out, 2 * LS_K))) {
         gpu_cleanup(c);
         return rc;
     }
-    if (hipHostMalloc((void **)&c->host_scal, sizeof(double) * GPAIR_NSCAL, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&c->host_scal, sizeof(double) * GPAIR_NSCAL, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->host_ls, sizeof(double) * 2 * LS_K, hipHostMallocDefault) != hipSuccess) {
         gpu_cleanup(c);
         return GPAIR_E_NOMEM;
     }
@@ -171,7 +178,8 @@ extern "C" int gpu_init(gpair_ctx **out, int device)
 
 static void free_images(gpair_ctx *c)
 {
-    gpair_buf_s *bufs[] = {&c->img, &c->tmp, &c->model, &c->gdata, &c->gent, &c->gnew, &c->gold, &c->gtmp, &c->dir};
+    gpair_buf_s *bufs[] = {&c->img,  &c->tmp,  &c->model, &c->gdata, &c->gent, &c->gnew,
+                           &c->gold, &c->gtmp, &c->dir,   &c->z,     &c->zold};
     for (auto *b : bufs) {
         dfree(&b->p);
         b->n = 0;
@@ -189,8 +197,11 @@ extern "C" int gpu_cleanup(gpair_ctx *c)
     dfree(&c->data); dfree(&c->err); dfree(&c->phasor); dfree(&c->tri); dfree(&c->csr_ptr); dfree(&c->csr_ent);
     dfree(&c->X); dfree(&c->Y); dfree(&c->vis); dfree(&c->coef); dfree(&c->mock);
     dfree(&c->scal); dfree(&c->invflux); dfree(&c->part); dfree(&c->ticket); dfree(&c->adj_part);
+    dfree(&c->Ud); dfree(&c->pfac); dfree(&c->ls_out);
     if (c->host_scal)
         (void)hipHostFree(c->host_scal);
+    if (c->host_ls)
+        (void)hipHostFree(c->host_ls);
     for (int k = 0; k < GPAIR_TK_COUNT; k++)
         for (hipEvent_t e : c->ev[k])
             (void)hipEventDestroy(e);
@@ -237,7 +248,9 @@ static int alloc_images(gpair_ctx *c, int W)
     if (c->W == W && c->img.p)
         return GPAIR_OK;
     free_images(c);
-    gpair_buf_s *bufs[] = {&c->img, &c->tmp, &c->model, &c->gdata, &c->gent, &c->gnew, &c->gold, &c->gtmp, &c->dir};
+    dfree(&c->pfac);   // sized for the previous width
+    gpair_buf_s *bufs[] = {&c->img,  &c->tmp,  &c->model, &c->gdata, &c->gent, &c->gnew,
+                           &c->gold, &c->gtmp, &c->dir,   &c->z,     &c->zold};
     for (auto *b : bufs) {
         if ((rc = dalloc(c, &b->p, n)))
             return rc;
@@ -294,7 +307,7 @@ extern "C" int gpu_copy_data(gpair_ctx *c, const float *data, const float *data_
     if ((rc = dalloc(c, &c->data, ndof)) || (rc = dalloc(c, &c->err, ndof)) || (rc = dalloc(c, &c->mock, ndof)) ||
         (rc = dalloc(c, &c->phasor, nbis)) || (rc = dalloc(c, &c->tri, nbis)) ||
         (rc = dalloc(c, &c->csr_ptr, nuv + 1)) || (rc = dalloc(c, &c->csr_ent, 3 * (size_t)nbis)) ||
-        (rc = dalloc(c, &c->vis, nuv)) || (rc = dalloc(c, &c->coef, nuv)))
+        (rc = dalloc(c, &c->vis, nuv)) || (rc = dalloc(c, &c->coef, nuv)) || (rc = dalloc(c, &c->Ud, nuv)))
         return rc;
     if ((rc = alloc_images(c, image_width)))
         return rc;
@@ -465,7 +478,8 @@ static int ready(gpair_ctx *c)
 
 static int own_buf(gpair_ctx *c, gpair_buf b)
 {
-    gpair_buf_s *bufs[] = {&c->img, &c->tmp, &c->model, &c->gdata, &c->gent, &c->gnew, &c->gold, &c->gtmp, &c->dir};
+    gpair_buf_s *bufs[] = {&c->img,  &c->tmp,  &c->model, &c->gdata, &c->gent, &c->gnew,
+                           &c->gold, &c->gtmp, &c->dir,   &c->z,     &c->zold};
     for (auto *x : bufs)
         if (x == b && b->p)
             return 1;
@@ -553,14 +567,15 @@ static int launch_flux(gpair_ctx *c, const float *img)
     return GPAIR_OK;
 }
 
-static int launch_forward(gpair_ctx *c, const float *img)
+// vis_out = scale ? (*scale) A img : A img  (A = unnormalised DFT)
+static int launch_forward_to(gpair_ctx *c, const float *img, const float *scale, float2 *vis_out)
 {
     const unsigned grid = (unsigned)((c->nuv + FWD_PT - 1) / FWD_PT);
     switch (c->W / 64) {
 #define FWD_CASE(NB)                                                                                       \
     case NB:                                                                                               \
         TIMED_LAUNCH(c, GPAIR_TK_FORWARD,                                                                  \
-                     (k_vis_fwd_f32<NB><<<grid, 256, 0, c->stream>>>(img, c->X, c->Y, c->nuv, c->invflux, c->vis))); \
+                     (k_vis_fwd_f32<NB><<<grid, 256, 0, c->stream>>>(img, c->X, c->Y, c->nuv, scale, vis_out))); \
         break;
     FWD_CASE(1)
     FWD_CASE(2)
@@ -573,6 +588,8 @@ static int launch_forward(gpair_ctx *c, const float *img)
     LAUNCHCHK(c);
     return GPAIR_OK;
 }
+
+static int launch_forward(gpair_ctx *c, const float *img) { return launch_forward_to(c, img, c->invflux, c->vis); }
 
 static int launch_chi2(gpair_ctx *c, const float *img)
 {
@@ -866,4 +883,250 @@ extern "C" int gpu_linesearch_zoom(gpair_ctx *c, int nuv, int npow, int nbis, in
         J_old = J;
         counter++;
     }
+}
+
+// ---------------------------------------------------------------------------
+// accelerated loop: preconditioned PR-CG with the batched line search
+// (declared in include/gpair_hip.h; restated in FP64 by the oracle's orc_pcg)
+// ---------------------------------------------------------------------------
+extern "C" void gpair_pcg_default_params(gpair_pcg_params *p)
+{
+    memset(p, 0, sizeof(*p));
+    p->alpha = 1.0;
+    p->precond = 1;
+    p->max_iter = 1000;
+    p->chi2r_target = 1.0;
+    p->minvalue = 1e-8;
+    p->wolfe1 = 1e-4;
+    p->verbose = 0;
+}
+
+extern "C" int gpair_set_pfactor(gpair_ctx *c, const float *pf, int size)
+{
+    if (!c)
+        return GPAIR_E_ARG;
+    if (!c->have_data)
+        return fail(c, GPAIR_E_STATE, "gpair_set_pfactor before gpu_copy_data");
+    if (!pf) {
+        dfree(&c->pfac);
+        return GPAIR_OK;
+    }
+    if (size != c->n)
+        return fail(c, GPAIR_E_ARG, "gpair_set_pfactor: %d values, image has %d pixels", size, c->n);
+    int rc;
+    if ((rc = dalloc(c, &c->pfac, c->n)))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->pfac, pf, sizeof(float) * c->n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GPAIR_OK;
+}
+
+static int read_scalars(gpair_ctx *c, int slot, int count, double *out)
+{
+    HIPCHK(c, hipMemcpyAsync(c->host_scal + slot, c->scal + slot, sizeof(double) * count, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < count; i++)
+        out[i] = c->host_scal[slot + i];
+    return GPAIR_OK;
+}
+
+// vertex of the parabola through (a, Ja), (b, Jb), (c, Jc), a < b < c, when it is a
+// minimum inside (a, c); otherwise b
+static double parabola_step(double a, double b, double cc, double Ja, double Jb, double Jc)
+{
+    const double p = (b - a) * (Jb - Jc), q = (b - cc) * (Jb - Ja);
+    const double den = p - q;
+    if (!(den < 0.0 || den > 0.0))
+        return b;
+    const double s = b - 0.5 * ((b - a) * p - (b - cc) * q) / den;
+    const double curv = (Ja - Jb) / (a - b) - (Jb - Jc) / (b - cc);
+    if (!(curv < 0.0) || !(s > a && s < cc))
+        return b;
+    return s;
+}
+
+// chi2 and entropy of an image (leaves vis / mock of that image)
+static int exact_eval(gpair_ctx *c, const float *img, double *chi2, double *S)
+{
+    int rc;
+    if ((rc = launch_chi2(c, img)))
+        return rc;
+    k_entropy<<<red_blocks(c->n), RED_THREADS, 0, c->stream>>>(img, c->model.p, c->n, c->part, c->ticket, c->scal);
+    LAUNCHCHK(c);
+    double v[2];
+    if ((rc = read_scalars(c, SC_CHI2, 2, v)))
+        return rc;
+    *chi2 = v[0];
+    *S = v[1];
+    return GPAIR_OK;
+}
+
+template <typename T>
+static inline void swap_ptr(T *&a, T *&b)
+{
+    T *t = a;
+    a = b;
+    b = t;
+}
+
+extern "C" int gpair_pcg_run(gpair_ctx *c, const gpair_pcg_params *prm, gpair_pcg_result *result, double *trace,
+                             int trace_rows)
+{
+    int rc;
+    if ((rc = ready(c)))
+        return rc;
+    if (!prm)
+        return fail(c, GPAIR_E_ARG, "gpair_pcg_run: NULL parameters");
+    gpair_pcg_result res;
+    memset(&res, 0, sizeof(res));
+    const int n = c->n, ndof = c->ndof;
+    const double alpha = prm->alpha, minv = prm->minvalue, c1 = prm->wolfe1;
+    double chi2, S;
+#define PCG_TRY(x)              \
+    do {                        \
+        if ((rc = (x))) {       \
+            res.status = rc;    \
+            goto done;          \
+        }                       \
+    } while (0)
+    {
+        PCG_TRY(exact_eval(c, c->img.p, &chi2, &S));
+        res.forward_dfts = 1;
+        double J = chi2 - alpha * S, beta = 0.0, sel = 0.0, s_ref = 0.0, zg_old = 0.0;
+        for (int it = 0; it < prm->max_iter; it++) {
+            if (!isfinite(J)) {
+                res.status = fail(c, GPAIR_E_NUMERIC, "gpair_pcg_run: non-finite objective at iteration %d", it);
+                goto done;
+            }
+            if (trace && it < trace_rows) {
+                double *r = trace + 8 * it;
+                r[0] = J; r[1] = chi2; r[2] = S; r[3] = sel; r[4] = beta;
+                r[5] = res.forward_dfts; r[6] = res.adjoint_dfts; r[7] = res.ls_batches;
+            }
+            res.iterations = it + 1;
+            res.J = J; res.chi2 = chi2; res.entropy = S; res.chi2r = chi2 / ndof;
+            if (prm->verbose)
+                printf("pcg it %d J %.10e chi2r %.6f step %.3e beta %.3e dft %d/%d\n", it, J, chi2 / ndof, sel, beta,
+                       res.forward_dfts, res.adjoint_dfts);
+            if (prm->chi2r_target > 0.0 && chi2 / ndof <= prm->chi2r_target) {
+                res.converged = 1;
+                break;
+            }
+            // criterion gradient at the current image
+            PCG_TRY(launch_data_gradient(c, c->img.p));
+            k_entropy_grad<<<ew_blocks(n), 256, 0, c->stream>>>(c->img.p, c->model.p, n, c->gent.p);
+            k_crit_grad<<<ew_blocks(n), 256, 0, c->stream>>>(c->gdata.p, c->gent.p, (float)alpha, n, c->gnew.p);
+            res.adjoint_dfts++;
+            // preconditioned direction
+            k_pcg_precond<<<red_blocks(n), RED_THREADS, 0, c->stream>>>(c->img.p, c->gnew.p, c->gold.p, c->pfac,
+                                                                         prm->precond, n, c->z.p, c->part,
+                                                                         c->ticket, c->scal + SC_ZG);
+            PCG_TRY(hipGetLastError() == hipSuccess ? GPAIR_OK : GPAIR_E_HIP);
+            double sc[GPAIR_NSCAL];
+            PCG_TRY(read_scalars(c, 0, SC_ZGO + 1, sc));
+            const double FI = sc[SC_FLUX], zg = sc[SC_ZG], zgo = sc[SC_ZGO];
+            double Jk[LS_K], sk[LS_K], w1 = 0.0;
+            int kbest = -1;
+            // attempt 0: conjugate direction; attempt 1 (only when attempt 0 used
+            // beta > 0 and found no decrease): restart along -z
+            for (int attempt = 0; attempt < 2 && kbest < 0; attempt++) {
+                double b0 = 0.0;
+                if (attempt == 0 && it > 0) {
+                    b0 = (zg - zgo) / zg_old;
+                    if (!(b0 > 0.0))
+                        b0 = 0.0;
+                }
+                if (attempt == 1 && beta == 0.0)
+                    break;
+                beta = b0;
+                double Fd = 0.0, dabs = 0.0;
+                for (int pass = 0; pass < 2; pass++) {
+                    k_pcg_dir<<<red_blocks(n), RED_THREADS, 0, c->stream>>>(c->img.p, c->z.p, c->gnew.p,
+                                                                             (float)beta, (float)minv, n, c->dir.p,
+                                                                             c->part, c->ticket, c->scal + SC_W1);
+                    PCG_TRY(hipGetLastError() == hipSuccess ? GPAIR_OK : GPAIR_E_HIP);
+                    PCG_TRY(read_scalars(c, SC_W1, 3, sc));
+                    w1 = sc[0]; Fd = sc[1]; dabs = sc[2];
+                    if (w1 < 0.0 || beta == 0.0)
+                        break;
+                    beta = 0.0;   // not a descent direction: restart along -z
+                }
+                (void)Fd;
+                if (!(w1 < 0.0) || !(dabs > 0.0))
+                    break;
+                if (s_ref == 0.0)
+                    s_ref = FI / dabs;
+                // one forward DFT of d, then all candidate steps in one batch
+                PCG_TRY(launch_forward_to(c, c->dir.p, nullptr, c->Ud));
+                res.forward_dfts++;
+                for (int round = 0; round < 4 && kbest < 0; round++) {
+                    res.ls_batches++;
+                    k_ls_chi2<<<red_blocks(c->npow + c->nbis), RED_THREADS, 0, c->stream>>>(
+                        c->vis, c->Ud, c->scal, s_ref, c->data, c->err, c->phasor, c->tri, c->npow, c->nbis,
+                        c->part, c->ticket, c->ls_out);
+                    k_ls_entropy<<<red_blocks(n), RED_THREADS, 0, c->stream>>>(c->img.p, c->dir.p, c->model.p,
+                                                                                s_ref, minv, n, c->part, c->ticket,
+                                                                                c->ls_out + LS_K);
+                    PCG_TRY(hipGetLastError() == hipSuccess ? GPAIR_OK : GPAIR_E_HIP);
+                    if (hipMemcpyAsync(c->host_ls, c->ls_out, sizeof(double) * 2 * LS_K, hipMemcpyDeviceToHost,
+                                       c->stream) != hipSuccess ||
+                        hipStreamSynchronize(c->stream) != hipSuccess) {
+                        res.status = fail(c, GPAIR_E_HIP, "line-search read-back failed");
+                        goto done;
+                    }
+                    int kb = 0;
+                    for (int k = 0; k < LS_K; k++) {
+                        sk[k] = k == 0 ? 0.0 : s_ref * exp2(0.5 * (double)(k - LS_TOP));
+                        Jk[k] = c->host_ls[k] - alpha * c->host_ls[LS_K + k];
+                        if (Jk[k] < Jk[kb])
+                            kb = k;
+                    }
+                    if (kb > 0)
+                        kbest = kb;
+                    else
+                        s_ref *= exp2(-16.0);
+                }
+            }
+            if (kbest < 0) {
+                res.stalled = 1;
+                break;
+            }
+            double s = sk[kbest];
+            if (kbest > 0 && kbest < LS_K - 1)
+                s = parabola_step(sk[kbest - 1], sk[kbest], sk[kbest + 1], Jk[kbest - 1], Jk[kbest], Jk[kbest + 1]);
+            // commit with an exact evaluation and the Armijo test
+            double Jn = 0.0, chi2n = 0.0, Sn = 0.0;
+            int ok = 0;
+            for (int tries = 0; tries < 32; tries++) {
+                k_step<<<ew_blocks(n), 256, 0, c->stream>>>(c->img.p, c->dir.p, (float)s, (float)minv, n, c->tmp.p);
+                PCG_TRY(exact_eval(c, c->tmp.p, &chi2n, &Sn));
+                res.forward_dfts++;
+                Jn = chi2n - alpha * Sn;
+                if (Jn <= J + c1 * s * w1) {
+                    ok = 1;
+                    break;
+                }
+                s = (tries == 0 && s != sk[kbest]) ? sk[kbest] : 0.5 * s;
+            }
+            if (!ok) {
+                res.stalled = 1;
+                break;
+            }
+            swap_ptr(c->img.p, c->tmp.p);     // vis / mock already belong to the new image
+            swap_ptr(c->gnew.p, c->gold.p);
+            swap_ptr(c->z.p, c->zold.p);
+            J = Jn;
+            chi2 = chi2n;
+            S = Sn;
+            sel = s;
+            s_ref = s;
+            zg_old = zg;
+        }
+    }
+done:
+#undef PCG_TRY
+    if (result)
+        *result = res;
+    return res.status;
 }

@@ -18,13 +18,21 @@ enum {
     SC_ENT = 2,
     SC_DOT = 3,
     SC_C0 = 4,
-    SC_DOT2 = 5,
-    SC_DOT3 = 6,
-    GPAIR_NSCAL = 8
+    SC_ZG = 5,     // z.g, z.g_old          (preconditioned CG)
+    SC_ZGO = 6,
+    SC_W1 = 7,     // d.g, sum d, sum |d|
+    SC_FD = 8,
+    SC_DABS = 9,
+    GPAIR_NSCAL = 16
 };
 
 #define RED_THREADS 256
 #define RED_MAX_BLOCKS 2048
+#define RED_MAX_NV 32
+
+// batched line search: LS_K candidate steps, s_0 = 0 and s_k = s_ref * 2^((k - LS_TOP) / 2)
+#define LS_K 32
+#define LS_TOP 26
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 

@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void k_vis_fwd_f32(const float *__restrict__ i
             s.x += red[w][tid].x;
             s.y += red[w][tid].y;
         }
-        const float f = *invflux;
+        const float f = invflux ? *invflux : 1.0f;   // no scale: unnormalised A I
         vis[p0 + tid] = make_float2(s.x * f, s.y * f);
     }
 }
@@ -426,6 +426,143 @@ __global__ void k_step(const float *img, const float *__restrict__ d, float s, f
         float t = img[i] + s * d[i];
         out[i] = t < minval ? minval : t;
     }
+}
+
+// ---------------------------------------------------------------------------
+// accelerated loop: preconditioned direction and batched line search
+// ---------------------------------------------------------------------------
+// z = P g with P = image (MEM metric) or 1, times an optional per-pixel factor;
+// out[0] = z.g, out[1] = z.g_old
+__global__ __launch_bounds__(RED_THREADS) void k_pcg_precond(const float *__restrict__ img,
+                                                             const float *__restrict__ g,
+                                                             const float *__restrict__ g_old,
+                                                             const float *__restrict__ pfac, int precond, int n,
+                                                             float *__restrict__ z, double *part, unsigned *ticket,
+                                                             double *out)
+{
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) {
+        float pf = precond ? img[i] : 1.0f;
+        if (pfac)
+            pf *= pfac[i];
+        const float zi = pf * g[i];
+        z[i] = zi;
+        v[0] += (double)zi * (double)g[i];
+        v[1] += (double)zi * (double)g_old[i];
+    }
+    reduce_finish<2>(v, part, ticket, out, nullptr);
+}
+
+// d = -z + beta d, projected to zero where the pixel sits on the floor and d < 0;
+// out[0] = d.g, out[1] = sum d, out[2] = sum |d|
+__global__ __launch_bounds__(RED_THREADS) void k_pcg_dir(const float *__restrict__ img, const float *__restrict__ z,
+                                                         const float *__restrict__ g, float beta, float minval, int n,
+                                                         float *__restrict__ d, double *part, unsigned *ticket,
+                                                         double *out)
+{
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) {
+        float di = -z[i] + beta * d[i];
+        if (img[i] <= minval && di < 0.f)
+            di = 0.f;
+        d[i] = di;
+        v[0] += (double)di * (double)g[i];
+        v[1] += (double)di;
+        v[2] += fabs((double)di);
+    }
+    reduce_finish<3>(v, part, ticket, out, nullptr);
+}
+
+// candidate 0 is s = 0 (the linearised current image, the common reference)
+__device__ inline double ls_step(double s_ref, int k)
+{
+    return k == 0 ? 0.0 : s_ref * exp2(0.5 * (double)(k - LS_TOP));
+}
+
+// chi2 of the visibilities (U_I + s_k U_d) / (F_I + s_k F_d) for all LS_K candidates,
+// U_I = F_I V_I; out[k] = chi2_k
+__global__ __launch_bounds__(RED_THREADS) void k_ls_chi2(const float2 *__restrict__ vis,
+                                                         const float2 *__restrict__ Ud,
+                                                         const double *__restrict__ scal, double s_ref,
+                                                         const float *__restrict__ data,
+                                                         const float *__restrict__ err,
+                                                         const float2 *__restrict__ phasor,
+                                                         const int4 *__restrict__ tri, int npow, int nbis,
+                                                         double *part, unsigned *ticket, double *out)
+{
+    double v[LS_K];
+#pragma unroll
+    for (int k = 0; k < LS_K; k++)
+        v[k] = 0.0;
+    const double FI = scal[SC_FLUX], Fd = scal[SC_FD];
+    float sk[LS_K], ik[LS_K];
+#pragma unroll
+    for (int k = 0; k < LS_K; k++) {
+        const double s = ls_step(s_ref, k);
+        sk[k] = (float)s;
+        ik[k] = (float)(1.0 / (FI + s * Fd));
+    }
+    const float fI = (float)FI;
+    const int total = npow + nbis;
+    for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < total; i += gridDim.x * RED_THREADS) {
+        if (i < npow) {
+            const float2 a = vis[i], b = Ud[i];
+            const float ax = a.x * fI, ay = a.y * fI, d0 = data[i], e = err[i];
+#pragma unroll
+            for (int k = 0; k < LS_K; k++) {
+                const float x = (ax + sk[k] * b.x) * ik[k], y = (ay + sk[k] * b.y) * ik[k];
+                const double r = ((double)(x * x + y * y) - (double)d0) * (double)e;
+                v[k] += r * r;
+            }
+        } else {
+            const int t = i - npow;
+            const int4 tr = tri[t];
+            const float2 a0 = vis[tr.x], a1 = vis[tr.y], a2 = vis[tr.z];
+            const float2 b0 = Ud[tr.x], b1 = Ud[tr.y], b2 = Ud[tr.z];
+            const float2 rho = phasor[t];
+            const int kk = npow + 2 * t;
+            const float dr = data[kk], di = data[kk + 1], er = err[kk], ei = err[kk + 1];
+            const float c0 = (tr.w & 1) ? -1.f : 1.f, c1 = (tr.w & 2) ? -1.f : 1.f, c2 = (tr.w & 4) ? -1.f : 1.f;
+#pragma unroll
+            for (int k = 0; k < LS_K; k++) {
+                const float s = sk[k], iv = ik[k];
+                const float2 l0 = make_float2((a0.x * fI + s * b0.x) * iv, c0 * (a0.y * fI + s * b0.y) * iv);
+                const float2 l1 = make_float2((a1.x * fI + s * b1.x) * iv, c1 * (a1.y * fI + s * b1.y) * iv);
+                const float2 l2 = make_float2((a2.x * fI + s * b2.x) * iv, c2 * (a2.y * fI + s * b2.y) * iv);
+                const float2 T = cmul(cmul(cmul(l0, l1), l2), rho);
+                const double r0 = ((double)T.x - (double)dr) * (double)er;
+                const double r1 = ((double)T.y - (double)di) * (double)ei;
+                v[k] += r0 * r0 + r1 * r1;
+            }
+        }
+    }
+    reduce_finish<LS_K>(v, part, ticket, out, nullptr);
+}
+
+// entropy of max(I + s_k d, minval) for all candidates; out[k] = S_k
+__global__ __launch_bounds__(RED_THREADS) void k_ls_entropy(const float *__restrict__ img,
+                                                            const float *__restrict__ d,
+                                                            const float *__restrict__ model, double s_ref,
+                                                            double minval, int n, double *part, unsigned *ticket,
+                                                            double *out)
+{
+    double v[LS_K], sk[LS_K];
+#pragma unroll
+    for (int k = 0; k < LS_K; k++) {
+        v[k] = 0.0;
+        sk[k] = ls_step(s_ref, k);
+    }
+    for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) {
+        const double I = img[i], dd = d[i], M = model[i];
+#pragma unroll
+        for (int k = 0; k < LS_K; k++) {
+            double t = I + sk[k] * dd;
+            if (t < minval)
+                t = minval;
+            v[k] += (t > 0.0 && M > 0.0) ? (t - M - t * log(t / M)) : -M;
+        }
+    }
+    reduce_finish<LS_K>(v, part, ticket, out, nullptr);
 }
 
 // explicit instantiations for the supported widths (W = 64 * NB)

@@ -183,6 +183,43 @@ int gpair_timing_enable(gpair_ctx *ctx, int on);
 /* number of timed launches of a kernel and their summed duration (ms) */
 int gpair_timing_read(gpair_ctx *ctx, int kernel, int *count, double *total_ms);
 
+/* ---- accelerated loop (no reference counterpart) ---------------------------
+ * Preconditioned Polak-Ribiere CG: z = P o g with P = image (MEM metric, when
+ * precond != 0) times an optional per-pixel factor (gpair_set_pfactor), the
+ * direction projected to zero on floor pixels moving down, and a batched line
+ * search: with U = A I and A d (A = unnormalised DFT, one extra forward DFT of d)
+ * the visibilities of I + s d are (U_I + s U_d) / (F_I + s F_d), so 32 steps
+ * s_ref 2^((k-26)/2) are scored in one launch; a parabola through the best three
+ * gives s, committed with an exact evaluation that must pass the Armijo test.
+ * Per iteration: 2 forward DFTs, 1 adjoint DFT, about 4 scalar read-backs. */
+typedef struct {
+    double alpha;
+    int precond;           /* 0: P = 1, 1: P = image */
+    int max_iter;
+    double chi2r_target;
+    double minvalue;       /* 1e-8 */
+    double wolfe1;         /* Armijo constant, 1e-4 */
+    int verbose;
+} gpair_pcg_params;
+
+typedef struct {
+    int iterations;        /* trace rows (objective evaluations of accepted images) */
+    int forward_dfts, adjoint_dfts, ls_batches;
+    double J, chi2, entropy, chi2r;
+    int converged;
+    int stalled;           /* no decrease found along the direction */
+    int status;
+} gpair_pcg_result;
+
+void gpair_pcg_default_params(gpair_pcg_params *p);
+/* Runs the accelerated loop on the current image.  trace: 8 doubles per row:
+ * J, chi2, S, step, beta, forward DFTs, adjoint DFTs, line-search batches
+ * (the last three cumulative). */
+int gpair_pcg_run(gpair_ctx *ctx, const gpair_pcg_params *params, gpair_pcg_result *result, double *trace,
+                  int trace_rows);
+/* per-pixel preconditioner factor [W*W] multiplying P (NULL clears it) */
+int gpair_set_pfactor(gpair_ctx *ctx, const float *pfactor, int size);
+
 /* problem sizes of a context (0 when not set) */
 int gpair_sizes(const gpair_ctx *ctx, int *W, int *nuv, int *npow, int *nbis);
 /* synchronise the context's stream */

@@ -500,3 +500,231 @@ int orc_num_threads(void)
     return 1;
 #endif
 }
+
+/* ------------------------------------------------------------------------
+ * Accelerated loop (a build-side method; the reference has no counterpart):
+ * preconditioned Polak-Ribiere CG with a batched line search.
+ *   z = P o g (P = 1, or the MEM image metric P = I, optionally times a
+ *   per-pixel factor), beta = max(0, z.(g - g_old) / z_old.g_old), d = -z + beta d,
+ *   restart d = -z when d.g >= 0.
+ * Line search: the DFT is linear, so with U_I = A I, U_d = A d (A = unnormalised
+ * DFT) the visibilities of I + s d are (U_I + s U_d) / (F_I + s F_d).  All
+ * LS_K candidate steps (s_0 = 0, s_k = s_ref 2^((k - LS_TOP)/2)) are scored at
+ * once from that (the pixel floor is applied to the entropy term; in V it is
+ * ignored); a step is taken when some s_k > 0 scores below s_0, otherwise the
+ * window moves down 16 octaves (4 rounds), then the direction restarts as -z,
+ * a parabola through the best candidate and its neighbours gives s*, and s* is
+ * committed with an exact evaluation that must pass the Armijo test
+ * (J <= J0 + c1 s w1), falling back to the best grid step and then halving.
+ * ---------------------------------------------------------------------- */
+#define LS_K 32
+#define LS_TOP 26
+
+void orc_dft_raw(int W, int nuv, const double *au, const double *av, const double *img, double *out)
+{
+#pragma omp parallel
+    {
+        double complex *xr = malloc(sizeof(double complex) * (size_t)W);
+#pragma omp for schedule(static)
+        for (int p = 0; p < nuv; p++) {
+            for (int i = 0; i < W; i++)
+                xr[i] = turn_exp(au[p] * (double)i);
+            double complex acc = 0.0;
+            for (int y = 0; y < W; y++) {
+                double complex t = 0.0;
+                const double *row = img + (size_t)y * W;
+                for (int x = 0; x < W; x++)
+                    t += row[x] * xr[x];
+                acc += t * turn_exp(-av[p] * (double)y);
+            }
+            out[2 * p] = creal(acc);
+            out[2 * p + 1] = cimag(acc);
+        }
+        free(xr);
+    }
+}
+
+/* chi2 of the visibilities (U_I + s U_d) / (F_I + s F_d) */
+static double chi2_along(orc_prob *P, const double *UI, const double *Ud, double FI, double Fd, double s,
+                         double *vis_tmp, double *mock_tmp)
+{
+    double inv = 1.0 / (FI + s * Fd);
+    for (int p = 0; p < P->nuv; p++) {
+        vis_tmp[2 * p] = (UI[2 * p] + s * Ud[2 * p]) * inv;
+        vis_tmp[2 * p + 1] = (UI[2 * p + 1] + s * Ud[2 * p + 1]) * inv;
+    }
+    orc_vis2data(P->npow, P->nbis, vis_tmp, P->uvpnt, P->sign, P->phasor, mock_tmp);
+    return orc_data2chi2(P->npow + 2 * P->nbis, P->data, P->err, mock_tmp);
+}
+
+static double entropy_along(orc_prob *P, const double *img, const double *d, double s, double minv)
+{
+    int n = P->W * P->W;
+    double S = 0.0;
+    for (int i = 0; i < n; i++) {
+        double t = img[i] + s * d[i];
+        if (t < minv) t = minv;
+        double M = P->model[i];
+        S += (t > 0.0 && M > 0.0) ? (t - M - t * log(t / M)) : -M;
+    }
+    return S;
+}
+
+/* parabola through (a, Ja), (b, Jb), (c, Jc) with a < b < c: vertex if it is a
+ * minimum inside (a, c), else b */
+double orc_parabola_step(double a, double b, double c, double Ja, double Jb, double Jc)
+{
+    double p = (b - a) * (Jb - Jc), q = (b - c) * (Jb - Ja);
+    double den = p - q;
+    if (!(den < 0.0 || den > 0.0))
+        return b;
+    double s = b - 0.5 * ((b - a) * p - (b - c) * q) / den;
+    double curv = (Ja - Jb) / (a - b) - (Jb - Jc) / (b - c);   /* >0 for a minimum (a<b<c) */
+    if (!(curv < 0.0) || !(s > a && s < c))
+        return b;
+    return s;
+}
+
+/* trace: 8 doubles per outer iteration: J, chi2, S, step, beta, forward DFTs
+ * (cumulative), adjoint DFTs (cumulative), line-search batches (cumulative). */
+int orc_pcg(int W, int nuv, int npow, int nbis, const double *au, const double *av, const double *data,
+            const double *err, const int *uvpnt, const int *sign, const double *phasor, const double *model,
+            double alpha, int precond, const double *pfactor, int max_iter, double chi2r_target, double *img,
+            double *trace)
+{
+    const double c1 = 1e-4, minv = 1e-8;
+    int n = W * W, ndof = npow + 2 * nbis;
+    orc_prob P = {W, nuv, npow, nbis, au, av, data, err, phasor, model, uvpnt, sign, alpha,
+                  NULL, NULL, NULL, NULL, NULL, 0, 0};
+    P.vis = malloc(sizeof(double) * 2 * (size_t)nuv);
+    P.coef = malloc(sizeof(double) * 2 * (size_t)nuv);
+    P.mock = malloc(sizeof(double) * (size_t)ndof);
+    P.gdata = malloc(sizeof(double) * (size_t)n);
+    P.gent = calloc((size_t)n, sizeof(double));
+    double *g = calloc((size_t)n, sizeof(double)), *g_old = calloc((size_t)n, sizeof(double));
+    double *z = calloc((size_t)n, sizeof(double)), *z_old = calloc((size_t)n, sizeof(double));
+    double *d = calloc((size_t)n, sizeof(double)), *tmp = malloc(sizeof(double) * (size_t)n);
+    double *UI = malloc(sizeof(double) * 2 * (size_t)nuv), *Ud = malloc(sizeof(double) * 2 * (size_t)nuv);
+    double *vt = malloc(sizeof(double) * 2 * (size_t)nuv), *mt = malloc(sizeof(double) * (size_t)ndof);
+    int nfwd = 0, nadj = 0, nbatch = 0, rows = 0;
+    double chi2, S, beta = 0.0, sel = 0.0, s_ref = 0.0, zg_old = 0.0;
+    double J = eval_J(&P, img, &chi2, &S);
+    nfwd++;
+    for (int it = 0; it < max_iter; it++) {
+        if (trace) {
+            double *r = trace + 8 * rows;
+            r[0] = J; r[1] = chi2; r[2] = S; r[3] = sel; r[4] = beta; r[5] = nfwd; r[6] = nadj; r[7] = nbatch;
+        }
+        rows++;
+        if (chi2 / ndof <= chi2r_target)
+            break;
+        eval_grad(&P, img, g);
+        nadj++;
+        for (int i = 0; i < n; i++) {
+            double pf = precond ? img[i] : 1.0;
+            if (pfactor)
+                pf *= pfactor[i];
+            z[i] = pf * g[i];
+        }
+        const double zg = dot(n, z, g), zgo = dot(n, z, g_old);
+        const double FI = orc_flux(n, img);
+        for (int p = 0; p < nuv; p++) {
+            UI[2 * p] = P.vis[2 * p] * FI;
+            UI[2 * p + 1] = P.vis[2 * p + 1] * FI;
+        }
+        double Jk[LS_K], sk[LS_K], w1 = 0.0;
+        int kbest = -1;
+        /* attempt 0: conjugate direction; attempt 1 (only when attempt 0 used
+         * beta > 0 and found no decrease): restart along -z */
+        for (int attempt = 0; attempt < 2 && kbest < 0; attempt++) {
+            double b0 = 0.0;
+            if (attempt == 0 && it > 0) {
+                b0 = (zg - zgo) / zg_old;
+                if (!(b0 > 0.0)) b0 = 0.0;
+            }
+            if (attempt == 1 && beta == 0.0)
+                break;
+            beta = b0;
+            /* pixels on the floor do not move further down: the direction is
+             * projected onto the feasible side there, so d.g is the slope of
+             * the clamped path */
+            for (int pass = 0; pass < 2; pass++) {
+                for (int i = 0; i < n; i++) {
+                    d[i] = -z[i] + beta * d[i];
+                    if (img[i] <= minv && d[i] < 0.0)
+                        d[i] = 0.0;
+                }
+                w1 = dot(n, d, g);
+                if (w1 < 0.0 || beta == 0.0)
+                    break;
+                beta = 0.0;
+            }
+            if (!(w1 < 0.0))
+                break;
+            double Fd = 0.0, dabs = 0.0;
+            for (int i = 0; i < n; i++) {
+                Fd += d[i];
+                dabs += fabs(d[i]);
+            }
+            if (s_ref == 0.0)
+                s_ref = FI / dabs;
+            orc_dft_raw(W, nuv, au, av, d, Ud);
+            nfwd++;
+            /* candidate 0 is s = 0: every candidate is compared with the same
+             * linearised evaluation of the current image */
+            for (int round = 0; round < 4 && kbest < 0; round++) {
+                nbatch++;
+                for (int k = 0; k < LS_K; k++) {
+                    sk[k] = k == 0 ? 0.0 : s_ref * pow(2.0, 0.5 * (k - LS_TOP));
+                    double c2 = chi2_along(&P, UI, Ud, FI, Fd, sk[k], vt, mt);
+                    Jk[k] = c2 - alpha * entropy_along(&P, img, d, sk[k], minv);
+                }
+                int kb = 0;
+                for (int k = 1; k < LS_K; k++)
+                    if (Jk[k] < Jk[kb]) kb = k;
+                if (kb > 0)
+                    kbest = kb;
+                else
+                    s_ref *= pow(2.0, -16.0);
+            }
+        }
+        if (kbest < 0) {
+            if (orc_verbose)
+                fprintf(stderr, "pcg it %d: no decrease (J %.10e, w1 %.3e, s_ref %.3e)\n", it, J, w1, s_ref);
+            break;   /* no decrease along d at any scale: stalled */
+        }
+        double s = sk[kbest];
+        if (kbest > 0 && kbest < LS_K - 1)
+            s = orc_parabola_step(sk[kbest - 1], sk[kbest], sk[kbest + 1], Jk[kbest - 1], Jk[kbest], Jk[kbest + 1]);
+        /* commit with an exact evaluation and the Armijo test */
+        double Jn = 0.0, chi2n = 0.0, Sn = 0.0;
+        int ok = 0;
+        for (int tries = 0; tries < 32; tries++) {
+            step_image(n, img, d, s, minv, tmp);
+            Jn = eval_J(&P, tmp, &chi2n, &Sn);
+            nfwd++;
+            if (Jn <= J + c1 * s * w1) {
+                ok = 1;
+                break;
+            }
+            s = (tries == 0 && s != sk[kbest]) ? sk[kbest] : 0.5 * s;
+        }
+        if (!ok) {
+            if (orc_verbose)
+                fprintf(stderr, "pcg it %d: commit failed (J %.10e, Jn %.10e, s %.3e, w1 %.3e)\n", it, J, Jn, s, w1);
+            break;
+        }
+        memcpy(img, tmp, sizeof(double) * (size_t)n);
+        J = Jn;
+        chi2 = chi2n;
+        S = Sn;
+        sel = s;
+        s_ref = s;
+        memcpy(g_old, g, sizeof(double) * (size_t)n);
+        memcpy(z_old, z, sizeof(double) * (size_t)n);
+        zg_old = zg;
+    }
+    free(P.vis); free(P.coef); free(P.mock); free(P.gdata); free(P.gent);
+    free(g); free(g_old); free(z); free(z_old); free(d); free(tmp); free(UI); free(Ud); free(vt); free(mt);
+    return rows;
+}

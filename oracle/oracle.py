@@ -68,6 +68,12 @@ class Oracle:
         L.orc_cg.argtypes = [C.c_int] * 4 + [_dp, _dp, _dp, _dp, _ip, _ip, _dp, _dp, C.c_double, C.c_int,
                                              C.c_int, C.c_double, _dp, C.c_void_p]
         L.orc_cg.restype = C.c_int
+        L.orc_pcg.argtypes = [C.c_int] * 4 + [_dp, _dp, _dp, _dp, _ip, _ip, _dp, _dp, C.c_double, C.c_int,
+                                              C.c_void_p, C.c_int, C.c_double, _dp, C.c_void_p]
+        L.orc_pcg.restype = C.c_int
+        L.orc_dft_raw.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.orc_parabola_step.argtypes = [C.c_double] * 6
+        L.orc_parabola_step.restype = C.c_double
         L.orc_num_threads.restype = C.c_int
 
     # -- problem helpers ----------------------------------------------------
@@ -158,6 +164,22 @@ class Oracle:
                              P["uvpnt"], P["sign"], P["phasor"], P["model"], float(alpha), int(method),
                              int(max_iter), float(chi2r_target), img, trace.ctypes.data)
         return img, trace[:rows]
+
+    def pcg(self, P, img, alpha, precond=1, pfactor=None, max_iter=200, chi2r_target=1.0):
+        """accelerated loop (preconditioned PR-CG, batched line search)."""
+        img = _c64(img).copy()
+        trace = np.zeros((max_iter + 1, 8))
+        pf = None if pfactor is None else _c64(pfactor)
+        rows = self.L.orc_pcg(P["W"], P["nuv"], P["npow"], P["nbis"], P["au"], P["av"], P["data"], P["err"],
+                              P["uvpnt"], P["sign"], P["phasor"], P["model"], float(alpha), int(precond),
+                              None if pf is None else pf.ctypes.data, int(max_iter), float(chi2r_target), img,
+                              trace.ctypes.data)
+        return img, trace[:rows]
+
+    def dft_raw(self, W, au, av, img):
+        out = np.empty(2 * au.size)
+        self.L.orc_dft_raw(W, au.size, _c64(au), _c64(av), _c64(img), out)
+        return out.view(np.complex128)
 
 
 # ---------------------------------------------------------------------------

@@ -162,3 +162,37 @@ def test_cg_loop_matches_oracle(orc):
     # the objective of the GPU's final image, re-evaluated in FP64
     J_o = orc.objective(P, img_gpu, ALPHA)[0]
     assert abs(J_gpu - J_o) / abs(J_o) <= 2e-5
+
+
+def test_pcg_loop_matches_oracle(orc):
+    """accelerated loop (gpair_pcg_run) against the FP64 oracle's orc_pcg, same rules."""
+    prob = synth.make_problem("cfg1", seed=1)
+    P = orc.unpack(prob)
+    alpha = prob.meta["alpha"]
+    nit = 12
+    with GpairContext.from_problem(prob) as ctx:
+        res, tr = ctx.pcg_run(alpha, precond=1, max_iter=nit, chi2r_target=0.0)
+    img_o, tr_o = orc.pcg(P, prob.start.astype(np.float64), alpha, precond=1, max_iter=nit, chi2r_target=0.0)
+    print("gpu J", tr[:, 0])
+    print("orc J", tr_o[:, 0])
+    assert res.iterations == nit and len(tr_o) == nit
+    rel = np.abs(tr[:, 0] - tr_o[:, 0]) / np.abs(tr_o[:, 0])
+    print("trajectory rel diff", rel)
+    assert rel.max() <= 1e-3
+    # same DFT / batch counts (the rules took the same branches)
+    assert np.array_equal(tr[:, 5:8], tr_o[:, 5:8])
+
+
+def test_pcg_reaches_target_and_objective_is_exact(orc):
+    """cfg1 to chi2/ndof <= 1; the final objective re-evaluated in FP64 agrees."""
+    prob = synth.make_problem("cfg1", seed=1)
+    P = orc.unpack(prob)
+    alpha = prob.meta["alpha"]
+    with GpairContext.from_problem(prob) as ctx:
+        res, tr = ctx.pcg_run(alpha, precond=1, max_iter=3000, chi2r_target=1.0)
+        img = ctx.get("ci").astype(np.float64)
+    print("iterations", res.iterations, "chi2r", res.chi2r, "dfts", res.forward_dfts, res.adjoint_dfts)
+    assert res.converged and res.chi2r <= 1.0
+    J_o, chi2_o, *_ = orc.objective(P, img, alpha)
+    assert abs(res.J - J_o) / abs(J_o) <= 2e-5
+    assert chi2_o / prob.ndof <= 1.0 + 1e-5
