@@ -1,12 +1,20 @@
 """Benchmark of the MEM-CG reconstruction loop on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--cfg cfg2] [--to-target] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--cfg cfg2] [--no-cpu] [--plain-budget-s S]
 
-A step is one CG iteration of the reference's GPU-path loop (objective,
-gradient, beta, descent direction, bracket + zoom line search, image update),
-driven by the C host loop gpair_cg_run over the C ABI, on the 256x256 headline
-problem (36,864 V2 + 86,016 T3, synthetic, seeded).  Inputs are resident in HBM
-before the timed region.  Prints one JSON line.
+Workload: the 256x256 headline problem (36,864 V2 + 86,016 T3, synthetic and
+seeded, Gaussian prior of FWHM 5 mas), all inputs resident in HBM before any
+timed region.  A step is one CG iteration.
+
+* value: CG iterations per second of the accelerated loop (gpair_pcg_run:
+  image-metric preconditioned PR-CG, batched line search with FP64 objective
+  changes) over K timed iterations after W warm-up iterations.
+* to_target: wall time to chi2/ndof <= 1 from the prior for the accelerated
+  loop in FP32 and in BF16x3, and for the plain loop with the reference's rules
+  (gpair_cg_run, FP32 DFT), the latter bounded by --plain-budget-s.
+* roofline: the forward DFT, timed with HIP events on the context stream.
+* cpu_baseline: the reference's own CPU model (oracle/_ref) on one core.
+Prints one JSON line on stdout.
 """
 from __future__ import annotations
 
@@ -22,24 +30,23 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-METRIC = ("wall-time & CG iters/s to chi2_r=1 at 256², BF16x3/learned vs plain FP32; err vs FP64")
-PEAK_FP32_MFMA = 157.3   # TFLOP/s, MI355X dense FP32 matrix
-PEAK_HBM = 8000.0        # GB/s
+METRIC = "wall-time & CG iters/s to chi2_r=1 at 256², BF16x3/learned vs plain FP32; err vs FP64"
+PEAK = {"fp32": 157.3, "bf16x3": 2500.0 / 3.0}   # TFLOP/s: FP32 MFMA; BF16 MFMA / 3 products
 PROFILES = os.path.join(ROOT, "profiles")
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cfg", default="cfg2")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--alpha", type=float, default=None)
-    ap.add_argument("--method", type=int, default=2)
-    ap.add_argument("--ls-max-evals", type=int, default=64)
-    ap.add_argument("--to-target", action="store_true", help="also time the run to chi2_r = 1")
-    ap.add_argument("--max-iter", type=int, default=20000)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3"])
+    ap.add_argument("--max-iter", type=int, default=30000)
+    ap.add_argument("--plain-budget-s", type=float, default=30.0)
+    ap.add_argument("--no-target", action="store_true", help="skip the runs to chi2_r = 1")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-kind", default="auto", choices=["auto", "reference", "port"])
     ap.add_argument("--calib-copy", action="store_true",
@@ -60,9 +67,9 @@ def calib_copy():
     print(json.dumps({"calib_copy_bytes": n * 4}))
 
 
-def read_traffic(W):
-    """HBM bytes per forward-DFT launch from the committed rocprofv3 --pmc summary, if present."""
-    path = os.path.join(PROFILES, f"forward_dft_W{W}_pmc.txt")
+def read_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if present."""
+    path = os.path.join(PROFILES, name)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -72,7 +79,8 @@ def read_traffic(W):
 
 
 def cpu_baseline(prob, evals_J, evals_G, kind):
-    """Reference CPU model (oracle/_ref, 1 core) or the FP64 port, on a bounded sample."""
+    """The reference's CPU model (oracle/_ref, one core) or the FP64 port, on a bounded
+    sample scaled to CG iterations of the reference rules."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     if kind == "auto":
@@ -80,16 +88,15 @@ def cpu_baseline(prob, evals_J, evals_G, kind):
     if kind == "reference":
         from gpair_amd import synth
         R = O.RefCPU(prob)
-        img = prob.start.copy()
         devnull = os.open(os.devnull, os.O_WRONLY)
         saved = os.dup(1)
         sys.stdout.flush()
-        os.dup2(devnull, 1)        # the reference prints "CPU Flux" per call
+        os.dup2(devnull, 1)        # the reference prints a line per image2vis call
         try:
             t0 = time.perf_counter()
-            R.image2chi2(img)
+            R.image2chi2(prob.start)
             tJ = time.perf_counter() - t0
-            # gradient: same uv set on a 64x64 grid, cost is linear in pixel count
+            # gradient: the same uv set on a 64x64 grid; its cost is linear in the pixel count
             small = synth.make_problem(prob.meta["cfg"], seed=prob.meta["seed"], W=64,
                                        pix=prob.pixellation * prob.W / 64)
             Rs = O.RefCPU(small)
@@ -101,22 +108,31 @@ def cpu_baseline(prob, evals_J, evals_G, kind):
             os.dup2(saved, 1)
             os.close(devnull)
         per_iter = evals_J * tJ + evals_G * tG
-        sample = (f"reference cpu.c (image2chi2 at {prob.W}^2 = {tJ:.2f} s; compute_data_gradient on the same uv "
-                  f"at 64^2 x{prob.W * prob.W // 4096} = {tG:.1f} s) x ({evals_J:.1f} J + {evals_G:.1f} grad) "
-                  f"evals per CG iteration from the GPU run")
+        sample = (f"reference cpu.c: image2chi2 at {prob.W}^2 {tJ:.2f} s, compute_data_gradient on the same uv "
+                  f"at 64^2 x{prob.W * prob.W // 4096} = {tG:.1f} s; per CG iteration of the reference rules "
+                  f"{evals_J:.1f} J + {evals_G:.1f} gradient evaluations (counts from the GPU run)")
         return {"value": 1.0 / per_iter, "unit": "CG iter/s", "cores": 1, "kind": "reference", "sample": sample}
     orc = O.Oracle()
     P = orc.unpack(prob)
     img = prob.start.astype(np.float64)
     t0 = time.perf_counter()
-    J = orc.objective(P, img, 100.0)
+    J = orc.objective(P, img, 1.0)
     tJ = time.perf_counter() - t0
     t0 = time.perf_counter()
-    orc.gradient(P, img, 100.0, J[3], J[4])
+    orc.gradient(P, img, 1.0, J[3], J[4])
     tG = time.perf_counter() - t0
     per_iter = evals_J * tJ + evals_G * tG
     return {"value": 1.0 / per_iter, "unit": "CG iter/s", "cores": orc.threads(), "kind": "port",
-            "sample": f"FP64 oracle: one J ({tJ:.2f} s) + one gradient ({tG:.2f} s) scaled by the evals per iteration"}
+            "sample": f"FP64 oracle: one J ({tJ:.2f} s) + one gradient ({tG:.2f} s) x evals per iteration"}
+
+
+def timing(L, h):
+    out = {}
+    for k, name in ((0, "forward"), (1, "adjoint")):
+        cnt, tot = C.c_int(), C.c_double()
+        L.gpair_timing_read(h, k, C.byref(cnt), C.byref(tot))
+        out[name] = (cnt.value, tot.value)
+    return out
 
 
 def main():
@@ -127,76 +143,115 @@ def main():
     if a.calib_copy:
         calib_copy()
         return
-    from gpair_amd import _lib, synth
+    from gpair_amd import synth
     from gpair_amd.api import GpairContext
 
     prob = synth.make_problem(a.cfg, seed=a.seed)
     alpha = a.alpha if a.alpha is not None else prob.meta["alpha"]
-    print(f"# problem {a.cfg}: W={prob.W} nuv={prob.nuv} nbis={prob.nbis} ndof={prob.ndof} alpha={alpha}",
-          file=sys.stderr, flush=True)
+    prec = {"fp32": 0, "bf16x3": 1}[a.precision]
+    W, nuv = prob.W, prob.nuv
+    print(f"# problem {a.cfg}: W={W} nuv={nuv} nbis={prob.nbis} ndof={prob.ndof} alpha={alpha} "
+          f"prior={prob.meta['prior']}", file=sys.stderr, flush=True)
+
+    # --- headline: K timed iterations of the accelerated loop after W warm-up ones ---
     ctx = GpairContext.from_problem(prob)
+    ctx.set_precision(prec)
     L = ctx.L
     if a.warmup > 0:
-        ctx.cg_run(alpha, method=a.method, max_iter=a.warmup, chi2r_target=0.0, ls_max_evals=a.ls_max_evals)
+        ctx.pcg_run(alpha, precond=1, max_iter=a.warmup, chi2r_target=0.0)
     L.gpair_sync(ctx.h)
     L.gpair_timing_enable(ctx.h, 1)
     t0 = time.perf_counter()
-    res, tr = ctx.cg_run(alpha, method=a.method, max_iter=a.steps, chi2r_target=0.0, ls_max_evals=a.ls_max_evals)
+    res, tr = ctx.pcg_run(alpha, precond=1, max_iter=a.steps, chi2r_target=0.0)
     L.gpair_sync(ctx.h)
     dt = time.perf_counter() - t0
-    cnt, tot = C.c_int(), C.c_double()
-    L.gpair_timing_read(ctx.h, 0, C.byref(cnt), C.byref(tot))
-    fwd_n, fwd_ms = cnt.value, tot.value
-    L.gpair_timing_read(ctx.h, 1, C.byref(cnt), C.byref(tot))
-    adj_n, adj_ms = cnt.value, tot.value
+    tk = timing(L, ctx.h)
     L.gpair_timing_enable(ctx.h, 0)
-    print(f"# timed {a.steps} CG iterations in {dt:.3f} s: J evals {res.criterion_evals}, grad evals "
-          f"{res.grad_evals}, chi2r {res.chi2r:.4f}", file=sys.stderr, flush=True)
+    ctx.close()
+    steps_done = res.iterations
+    print(f"# accelerated ({a.precision}): {steps_done} iterations in {dt:.3f} s, chi2r {res.chi2r:.5f}",
+          file=sys.stderr, flush=True)
 
-    W, nuv = prob.W, prob.nuv
-    flop_fwd = 4.0 * nuv * W * W + 8.0 * nuv * W
+    flop = 4.0 * nuv * W * W + 8.0 * nuv * W
+    fwd_n, fwd_ms = tk["forward"]
+    adj_n, adj_ms = tk["adjoint"]
     avg_fwd = fwd_ms / max(fwd_n, 1) * 1e-3
-    achieved = flop_fwd / avg_fwd / 1e12
-    avg_adj = adj_ms / max(adj_n, 1) * 1e-3
+    achieved = flop / avg_fwd / 1e12
+    kname = "k_vis_fwd_f32<4>" if prec == 0 else "k_vis_fwd_b3"
     out = {
         "metric": METRIC,
-        "value": a.steps / dt,
+        "value": steps_done / dt,
         "unit": "CG iter/s",
         "n_gpus": 1,
-        "steps": a.steps,
+        "steps": steps_done,
         "warmup": a.warmup,
-        "ms_per_step": dt / a.steps * 1e3,
+        "ms_per_step": dt / steps_done * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded array + analytic sky, noise per BASELINE configs)",
-        "config": {"workload": f"{a.cfg}: plain MEM-CG, FP32 MFMA DFT, FP64 scalars; {W}x{W} px, "
-                               f"{prob.npow} V2 + {prob.nbis} T3",
+        "dtype": "f32" if prec == 0 else "bf16x3 (f32 accumulate)",
+        "data": "synthetic: seeded 9-telescope array, sky rendered on the grid, noise per BASELINE configs",
+        "config": {"workload": f"{a.cfg}: MEM-CG reconstruction {W}x{W} px, {prob.npow} V2 + {prob.nbis} T3, "
+                               f"accelerated loop (preconditioned PR-CG, batched line search), {a.precision} DFT",
                    "W": W, "nuv": nuv, "npow": prob.npow, "nbis": prob.nbis, "alpha": alpha,
-                   "method": ["SD", "FR", "PR", "HS"][a.method], "seed": a.seed, "precision": "fp32"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_MFMA, "traffic": read_traffic(W),
-                     "kernel": "k_vis_fwd_f32 (forward DFT)", "avg_us": avg_fwd * 1e6, "launches": fwd_n,
-                     "flop_per_launch": flop_fwd},
-        "adjoint": {"kernel": "k_adj_f32", "avg_us": avg_adj * 1e6, "launches": adj_n,
-                    "achieved_tflops": flop_fwd / avg_adj / 1e12 if adj_n else None},
-        "evals_per_iter": {"J": res.criterion_evals / a.steps, "grad": res.grad_evals / a.steps},
-        "chi2r_end": res.chi2r,
+                   "prior": list(prob.meta["prior"]), "seed": a.seed, "precision": a.precision},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK[a.precision], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK[a.precision],
+                     "traffic": read_traffic(f"forward_dft_W{W}_{a.precision}_pmc.txt"),
+                     "kernel": f"{kname} (forward DFT)", "avg_us": avg_fwd * 1e6, "launches": fwd_n,
+                     "flop_per_launch": flop},
+        "adjoint": {"kernel": "k_adj_f32", "avg_us": adj_ms / max(adj_n, 1) * 1e3, "launches": adj_n,
+                    "achieved_tflops": flop / (adj_ms / max(adj_n, 1) * 1e-3) / 1e12 if adj_n else None},
+        "dfts_per_iter": {"forward": (res.forward_dfts) / max(steps_done, 1),
+                          "adjoint": res.adjoint_dfts / max(steps_done, 1)},
     }
-    if a.to_target:
-        ctx2 = GpairContext.from_problem(prob)
+
+    # --- wall time to chi2_r = 1 ------------------------------------------------
+    plain_counts = None
+    if not a.no_target:
+        tt = {}
+        for name, p in (("accelerated_fp32", 0), ("accelerated_bf16x3", 1)):
+            c2 = GpairContext.from_problem(prob)
+            c2.set_precision(p)
+            t0 = time.perf_counter()
+            r2, _ = c2.pcg_run(alpha, precond=1, max_iter=a.max_iter, chi2r_target=1.0)
+            c2.L.gpair_sync(c2.h)
+            wall = time.perf_counter() - t0
+            tt[name] = {"wall_s": wall, "iterations": r2.iterations, "iters_per_s": r2.iterations / wall,
+                        "converged": bool(r2.converged), "chi2r": r2.chi2r, "J": r2.J,
+                        "forward_dfts": r2.forward_dfts, "adjoint_dfts": r2.adjoint_dfts}
+            print(f"# {name}: {r2.iterations} its {wall:.2f} s chi2r {r2.chi2r:.6f}", file=sys.stderr, flush=True)
+            c2.close()
+        # plain loop with the reference's rules, bounded by the budget (chunks of 500 iterations)
+        c3 = GpairContext.from_problem(prob)
         t0 = time.perf_counter()
-        r2, tr2 = ctx2.cg_run(alpha, method=a.method, max_iter=a.max_iter, chi2r_target=1.0,
-                              ls_max_evals=a.ls_max_evals)
-        ctx2.L.gpair_sync(ctx2.h)
-        out["to_target"] = {"wall_s": time.perf_counter() - t0, "iterations": r2.iterations,
-                            "converged": bool(r2.converged), "chi2r": r2.chi2r, "J": r2.J}
-        ctx2.close()
+        its = jev = gev = 0
+        conv = False
+        r3 = None
+        while time.perf_counter() - t0 < a.plain_budget_s and its < a.max_iter:
+            r3, _ = c3.cg_run(alpha, method=2, max_iter=500, chi2r_target=1.0)
+            its += r3.iterations
+            jev += r3.criterion_evals
+            gev += r3.grad_evals
+            if r3.converged:
+                conv = True
+                break
+        c3.L.gpair_sync(c3.h)
+        wall = time.perf_counter() - t0
+        c3.close()
+        tt["plain_fp32"] = {"wall_s": wall, "iterations": its, "iters_per_s": its / wall, "converged": conv,
+                            "chi2r": r3.chi2r, "J": r3.J, "budget_s": a.plain_budget_s,
+                            "J_evals_per_iter": jev / its, "grad_evals_per_iter": gev / its}
+        plain_counts = (jev / its, gev / its)
+        print(f"# plain_fp32: {its} its {wall:.2f} s chi2r {r3.chi2r:.6f} converged {conv}", file=sys.stderr,
+              flush=True)
+        best = min(tt["accelerated_fp32"]["wall_s"], tt["accelerated_bf16x3"]["wall_s"])
+        tt["speedup_vs_plain_fp32"] = ((">=" if not conv else "") +
+                                       f"{tt['plain_fp32']['wall_s'] / best:.1f}")
+        out["to_target"] = tt
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(prob, res.criterion_evals / a.steps, res.grad_evals / a.steps,
-                                           a.cpu_kind)
-    ctx.close()
+        counts = plain_counts if plain_counts else (10.0, 2.0)
+        out["cpu_baseline"] = cpu_baseline(prob, counts[0], counts[1], a.cpu_kind)
     print(json.dumps(out), flush=True)
 
 

@@ -89,6 +89,8 @@ SIGNATURES = {
     "gpair_pcg_default_params": (None, [C.POINTER(PcgParams)]),
     "gpair_pcg_run": (_i, [_vp, C.POINTER(PcgParams), C.POINTER(PcgResult), _vp, _i]),
     "gpair_set_pfactor": (_i, [_vp, _vp, _i]),
+    "gpair_set_model": (_i, [_vp, _i, C.c_float, _i, C.c_float, C.c_float, _i]),
+    "gpair_init_data": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _pi]),
 }
 
 _LIB = None

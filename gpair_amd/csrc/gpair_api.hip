@@ -48,9 +48,14 @@ struct gpair_ctx {
     // adjoint split-K partials
     float *adj_part;
     int adj_nsplit, adj_pper;
+    // split-precision (BF16x3) operands
+    unsigned short *Xh, *Xl, *Ih, *Il;
+    int split_valid;
     // accelerated loop
-    gpair_buf_s z, zold;
-    float2 *Ud;
+    gpair_buf_s z, zold, delta;
+    float2 *Ud, *vis2;
+    double2 *Vd, *Vn;
+    float *mock2;
     float *pfac;
     double *ls_out, *host_ls;   // device / pinned host, 2*LS_K
     // optional per-launch timing of the DFT kernels (HIP events on the context stream)
@@ -179,7 +184,7 @@ extern "C" int gpu_init(gpair_ctx **out, int device)
 static void free_images(gpair_ctx *c)
 {
     gpair_buf_s *bufs[] = {&c->img,  &c->tmp,  &c->model, &c->gdata, &c->gent, &c->gnew,
-                           &c->gold, &c->gtmp, &c->dir,   &c->z,     &c->zold};
+                           &c->gold, &c->gtmp, &c->dir,   &c->z,     &c->zold, &c->delta};
     for (auto *b : bufs) {
         dfree(&b->p);
         b->n = 0;
@@ -198,6 +203,8 @@ extern "C" int gpu_cleanup(gpair_ctx *c)
     dfree(&c->X); dfree(&c->Y); dfree(&c->vis); dfree(&c->coef); dfree(&c->mock);
     dfree(&c->scal); dfree(&c->invflux); dfree(&c->part); dfree(&c->ticket); dfree(&c->adj_part);
     dfree(&c->Ud); dfree(&c->pfac); dfree(&c->ls_out);
+    dfree(&c->vis2); dfree(&c->Vd); dfree(&c->Vn); dfree(&c->mock2);
+    dfree(&c->Xh); dfree(&c->Xl); dfree(&c->Ih); dfree(&c->Il);
     if (c->host_scal)
         (void)hipHostFree(c->host_scal);
     if (c->host_ls)
@@ -250,7 +257,7 @@ static int alloc_images(gpair_ctx *c, int W)
     free_images(c);
     dfree(&c->pfac);   // sized for the previous width
     gpair_buf_s *bufs[] = {&c->img,  &c->tmp,  &c->model, &c->gdata, &c->gent, &c->gnew,
-                           &c->gold, &c->gtmp, &c->dir,   &c->z,     &c->zold};
+                           &c->gold, &c->gtmp, &c->dir,   &c->z,     &c->zold, &c->delta};
     for (auto *b : bufs) {
         if ((rc = dalloc(c, &b->p, n)))
             return rc;
@@ -307,7 +314,9 @@ extern "C" int gpu_copy_data(gpair_ctx *c, const float *data, const float *data_
     if ((rc = dalloc(c, &c->data, ndof)) || (rc = dalloc(c, &c->err, ndof)) || (rc = dalloc(c, &c->mock, ndof)) ||
         (rc = dalloc(c, &c->phasor, nbis)) || (rc = dalloc(c, &c->tri, nbis)) ||
         (rc = dalloc(c, &c->csr_ptr, nuv + 1)) || (rc = dalloc(c, &c->csr_ent, 3 * (size_t)nbis)) ||
-        (rc = dalloc(c, &c->vis, nuv)) || (rc = dalloc(c, &c->coef, nuv)) || (rc = dalloc(c, &c->Ud, nuv)))
+        (rc = dalloc(c, &c->vis, nuv)) || (rc = dalloc(c, &c->coef, nuv)) || (rc = dalloc(c, &c->Ud, nuv)) ||
+        (rc = dalloc(c, &c->vis2, nuv)) || (rc = dalloc(c, &c->Vd, nuv)) || (rc = dalloc(c, &c->Vn, nuv)) ||
+        (rc = dalloc(c, &c->mock2, ndof)))
         return rc;
     if ((rc = alloc_images(c, image_width)))
         return rc;
@@ -382,6 +391,7 @@ extern "C" int gpair_set_uv(gpair_ctx *c, const float *u, const float *v, int nu
     dfree(&dv);
     if (!err)
         c->have_tables = 1;
+    c->split_valid = 0;
     return err;
 }
 
@@ -415,6 +425,7 @@ extern "C" int gpu_copy_dft(gpair_ctx *c, const float *dft_x, const float *dft_y
     dfree(&tmp);
     if (!err)
         c->have_tables = 1;
+    c->split_valid = 0;
     return err;
 }
 
@@ -449,9 +460,25 @@ extern "C" int gpair_set_precision(gpair_ctx *c, int precision)
 {
     if (!c)
         return GPAIR_E_ARG;
-    if (precision != GPAIR_PREC_FP32)
+    if (precision != GPAIR_PREC_FP32 && precision != GPAIR_PREC_BF16X3)
         return fail(c, GPAIR_E_ARG, "precision %d not available", precision);
     c->precision = precision;
+    return GPAIR_OK;
+}
+
+// hi/lo BF16 copies of the X table for the split-precision forward DFT
+static int ensure_split_tables(gpair_ctx *c)
+{
+    if (c->split_valid)
+        return GPAIR_OK;
+    const size_t tsz = (size_t)c->nuv * c->W;
+    int rc;
+    if ((rc = dalloc(c, &c->Xh, 2 * tsz)) || (rc = dalloc(c, &c->Xl, 2 * tsz)) ||
+        (rc = dalloc(c, &c->Ih, (size_t)c->n)) || (rc = dalloc(c, &c->Il, (size_t)c->n)))
+        return rc;
+    k_split_tables<<<(unsigned)((tsz + 255) / 256), 256, 0, c->stream>>>(c->X, c->nuv, c->W, c->Xh, c->Xl);
+    LAUNCHCHK(c);
+    c->split_valid = 1;
     return GPAIR_OK;
 }
 
@@ -479,7 +506,7 @@ static int ready(gpair_ctx *c)
 static int own_buf(gpair_ctx *c, gpair_buf b)
 {
     gpair_buf_s *bufs[] = {&c->img,  &c->tmp,  &c->model, &c->gdata, &c->gent, &c->gnew,
-                           &c->gold, &c->gtmp, &c->dir,   &c->z,     &c->zold};
+                           &c->gold, &c->gtmp, &c->dir,   &c->z,     &c->zold, &c->delta};
     for (auto *x : bufs)
         if (x == b && b->p)
             return 1;
@@ -570,6 +597,19 @@ static int launch_flux(gpair_ctx *c, const float *img)
 // vis_out = scale ? (*scale) A img : A img  (A = unnormalised DFT)
 static int launch_forward_to(gpair_ctx *c, const float *img, const float *scale, float2 *vis_out)
 {
+    if (c->precision == GPAIR_PREC_BF16X3) {
+        int rc;
+        if ((rc = ensure_split_tables(c)))
+            return rc;
+        k_split_image<<<ew_blocks(c->n), 256, 0, c->stream>>>(img, c->n, c->Ih, c->Il);
+        LAUNCHCHK(c);
+        const unsigned g3 = (unsigned)((c->nuv + B3_PW - 1) / B3_PW);
+        TIMED_LAUNCH(c, GPAIR_TK_FORWARD,
+                     (k_vis_fwd_b3<<<g3, 256, 0, c->stream>>>(c->Ih, c->Il, c->Xh, c->Xl, c->Y, c->nuv, c->W, scale,
+                                                              vis_out)));
+        LAUNCHCHK(c);
+        return GPAIR_OK;
+    }
     const unsigned grid = (unsigned)((c->nuv + FWD_PT - 1) / FWD_PT);
     switch (c->W / 64) {
 #define FWD_CASE(NB)                                                                                       \
@@ -969,164 +1009,4 @@ static inline void swap_ptr(T *&a, T *&b)
     a = b;
     b = t;
 }
-
-extern "C" int gpair_pcg_run(gpair_ctx *c, const gpair_pcg_params *prm, gpair_pcg_result *result, double *trace,
-                             int trace_rows)
-{
-    int rc;
-    if ((rc = ready(c)))
-        return rc;
-    if (!prm)
-        return fail(c, GPAIR_E_ARG, "gpair_pcg_run: NULL parameters");
-    gpair_pcg_result res;
-    memset(&res, 0, sizeof(res));
-    const int n = c->n, ndof = c->ndof;
-    const double alpha = prm->alpha, minv = prm->minvalue, c1 = prm->wolfe1;
-    double chi2, S;
-#define PCG_TRY(x)              \
-    do {                        \
-        if ((rc = (x))) {       \
-            res.status = rc;    \
-            goto done;          \
-        }                       \
-    } while (0)
-    {
-        PCG_TRY(exact_eval(c, c->img.p, &chi2, &S));
-        res.forward_dfts = 1;
-        double J = chi2 - alpha * S, beta = 0.0, sel = 0.0, s_ref = 0.0, zg_old = 0.0;
-        for (int it = 0; it < prm->max_iter; it++) {
-            if (!isfinite(J)) {
-                res.status = fail(c, GPAIR_E_NUMERIC, "gpair_pcg_run: non-finite objective at iteration %d", it);
-                goto done;
-            }
-            if (trace && it < trace_rows) {
-                double *r = trace + 8 * it;
-                r[0] = J; r[1] = chi2; r[2] = S; r[3] = sel; r[4] = beta;
-                r[5] = res.forward_dfts; r[6] = res.adjoint_dfts; r[7] = res.ls_batches;
-            }
-            res.iterations = it + 1;
-            res.J = J; res.chi2 = chi2; res.entropy = S; res.chi2r = chi2 / ndof;
-            if (prm->verbose)
-                printf("pcg it %d J %.10e chi2r %.6f step %.3e beta %.3e dft %d/%d\n", it, J, chi2 / ndof, sel, beta,
-                       res.forward_dfts, res.adjoint_dfts);
-            if (prm->chi2r_target > 0.0 && chi2 / ndof <= prm->chi2r_target) {
-                res.converged = 1;
-                break;
-            }
-            // criterion gradient at the current image
-            PCG_TRY(launch_data_gradient(c, c->img.p));
-            k_entropy_grad<<<ew_blocks(n), 256, 0, c->stream>>>(c->img.p, c->model.p, n, c->gent.p);
-            k_crit_grad<<<ew_blocks(n), 256, 0, c->stream>>>(c->gdata.p, c->gent.p, (float)alpha, n, c->gnew.p);
-            res.adjoint_dfts++;
-            // preconditioned direction
-            k_pcg_precond<<<red_blocks(n), RED_THREADS, 0, c->stream>>>(c->img.p, c->gnew.p, c->gold.p, c->pfac,
-                                                                         prm->precond, n, c->z.p, c->part,
-                                                                         c->ticket, c->scal + SC_ZG);
-            PCG_TRY(hipGetLastError() == hipSuccess ? GPAIR_OK : GPAIR_E_HIP);
-            double sc[GPAIR_NSCAL];
-            PCG_TRY(read_scalars(c, 0, SC_ZGO + 1, sc));
-            const double FI = sc[SC_FLUX], zg = sc[SC_ZG], zgo = sc[SC_ZGO];
-            double Jk[LS_K], sk[LS_K], w1 = 0.0;
-            int kbest = -1;
-            // attempt 0: conjugate direction; attempt 1 (only when attempt 0 used
-            // beta > 0 and found no decrease): restart along -z
-            for (int attempt = 0; attempt < 2 && kbest < 0; attempt++) {
-                double b0 = 0.0;
-                if (attempt == 0 && it > 0) {
-                    b0 = (zg - zgo) / zg_old;
-                    if (!(b0 > 0.0))
-                        b0 = 0.0;
-                }
-                if (attempt == 1 && beta == 0.0)
-                    break;
-                beta = b0;
-                double Fd = 0.0, dabs = 0.0;
-                for (int pass = 0; pass < 2; pass++) {
-                    k_pcg_dir<<<red_blocks(n), RED_THREADS, 0, c->stream>>>(c->img.p, c->z.p, c->gnew.p,
-                                                                             (float)beta, (float)minv, n, c->dir.p,
-                                                                             c->part, c->ticket, c->scal + SC_W1);
-                    PCG_TRY(hipGetLastError() == hipSuccess ? GPAIR_OK : GPAIR_E_HIP);
-                    PCG_TRY(read_scalars(c, SC_W1, 3, sc));
-                    w1 = sc[0]; Fd = sc[1]; dabs = sc[2];
-                    if (w1 < 0.0 || beta == 0.0)
-                        break;
-                    beta = 0.0;   // not a descent direction: restart along -z
-                }
-                (void)Fd;
-                if (!(w1 < 0.0) || !(dabs > 0.0))
-                    break;
-                if (s_ref == 0.0)
-                    s_ref = FI / dabs;
-                // one forward DFT of d, then all candidate steps in one batch
-                PCG_TRY(launch_forward_to(c, c->dir.p, nullptr, c->Ud));
-                res.forward_dfts++;
-                for (int round = 0; round < 4 && kbest < 0; round++) {
-                    res.ls_batches++;
-                    k_ls_chi2<<<red_blocks(c->npow + c->nbis), RED_THREADS, 0, c->stream>>>(
-                        c->vis, c->Ud, c->scal, s_ref, c->data, c->err, c->phasor, c->tri, c->npow, c->nbis,
-                        c->part, c->ticket, c->ls_out);
-                    k_ls_entropy<<<red_blocks(n), RED_THREADS, 0, c->stream>>>(c->img.p, c->dir.p, c->model.p,
-                                                                                s_ref, minv, n, c->part, c->ticket,
-                                                                                c->ls_out + LS_K);
-                    PCG_TRY(hipGetLastError() == hipSuccess ? GPAIR_OK : GPAIR_E_HIP);
-                    if (hipMemcpyAsync(c->host_ls, c->ls_out, sizeof(double) * 2 * LS_K, hipMemcpyDeviceToHost,
-                                       c->stream) != hipSuccess ||
-                        hipStreamSynchronize(c->stream) != hipSuccess) {
-                        res.status = fail(c, GPAIR_E_HIP, "line-search read-back failed");
-                        goto done;
-                    }
-                    int kb = 0;
-                    for (int k = 0; k < LS_K; k++) {
-                        sk[k] = k == 0 ? 0.0 : s_ref * exp2(0.5 * (double)(k - LS_TOP));
-                        Jk[k] = c->host_ls[k] - alpha * c->host_ls[LS_K + k];
-                        if (Jk[k] < Jk[kb])
-                            kb = k;
-                    }
-                    if (kb > 0)
-                        kbest = kb;
-                    else
-                        s_ref *= exp2(-16.0);
-                }
-            }
-            if (kbest < 0) {
-                res.stalled = 1;
-                break;
-            }
-            double s = sk[kbest];
-            if (kbest > 0 && kbest < LS_K - 1)
-                s = parabola_step(sk[kbest - 1], sk[kbest], sk[kbest + 1], Jk[kbest - 1], Jk[kbest], Jk[kbest + 1]);
-            // commit with an exact evaluation and the Armijo test
-            double Jn = 0.0, chi2n = 0.0, Sn = 0.0;
-            int ok = 0;
-            for (int tries = 0; tries < 32; tries++) {
-                k_step<<<ew_blocks(n), 256, 0, c->stream>>>(c->img.p, c->dir.p, (float)s, (float)minv, n, c->tmp.p);
-                PCG_TRY(exact_eval(c, c->tmp.p, &chi2n, &Sn));
-                res.forward_dfts++;
-                Jn = chi2n - alpha * Sn;
-                if (Jn <= J + c1 * s * w1) {
-                    ok = 1;
-                    break;
-                }
-                s = (tries == 0 && s != sk[kbest]) ? sk[kbest] : 0.5 * s;
-            }
-            if (!ok) {
-                res.stalled = 1;
-                break;
-            }
-            swap_ptr(c->img.p, c->tmp.p);     // vis / mock already belong to the new image
-            swap_ptr(c->gnew.p, c->gold.p);
-            swap_ptr(c->z.p, c->zold.p);
-            J = Jn;
-            chi2 = chi2n;
-            S = Sn;
-            sel = s;
-            s_ref = s;
-            zg_old = zg;
-        }
-    }
-done:
-#undef PCG_TRY
-    if (result)
-        *result = res;
-    return res.status;
-}
+#include "gpair_pcg.inc"

@@ -23,6 +23,9 @@ enum {
     SC_W1 = 7,     // d.g, sum d, sum |d|
     SC_FD = 8,
     SC_DABS = 9,
+    SC_FDELTA = 10,  // sum of the committed image change, entropy change, chi2 change
+    SC_DSENT = 11,
+    SC_DCHI2 = 12,
     GPAIR_NSCAL = 16
 };
 

@@ -479,72 +479,97 @@ __device__ inline double ls_step(double s_ref, int k)
     return k == 0 ? 0.0 : s_ref * exp2(0.5 * (double)(k - LS_TOP));
 }
 
-// chi2 of the visibilities (U_I + s_k U_d) / (F_I + s_k F_d) for all LS_K candidates,
-// U_I = F_I V_I; out[k] = chi2_k
-__global__ __launch_bounds__(RED_THREADS) void k_ls_chi2(const float2 *__restrict__ vis,
-                                                         const float2 *__restrict__ Ud,
-                                                         const double *__restrict__ scal, double s_ref,
-                                                         const float *__restrict__ data,
-                                                         const float *__restrict__ err,
-                                                         const float2 *__restrict__ phasor,
-                                                         const int4 *__restrict__ tri, int npow, int nbis,
-                                                         double *part, unsigned *ticket, double *out)
+// The line search and the commit work on CHANGES of the objective in FP64: the
+// current visibilities are kept in FP64 (Vd) and only DFTs of the direction d
+// and of the image change are taken in FP32 / BF16x3, so the decision
+// variables keep FP64 resolution however small the improvement is.
+__device__ inline double2 dcmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ inline double2 dleg(double2 v, int conj_bit) { return conj_bit ? make_double2(v.x, -v.y) : v; }
+
+__device__ inline double pix_entropy(double t, double M)
+{
+    return (t > 0.0 && M > 0.0) ? (t - M - t * log(t / M)) : -M;
+}
+
+// Vd[p] = vis[p] (FP64 copy of freshly evaluated visibilities)
+__global__ void k_vis_to_double(const float2 *__restrict__ vis, int nuv, double2 *__restrict__ Vd)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nuv)
+        Vd[p] = make_double2(vis[p].x, vis[p].y);
+}
+
+// change of chi2 along the line for all LS_K candidates:
+// V_k = (F_I V + s_k U_d) / (F_I + s_k F_d); out[k] = sum r(V_k)^2 - r(V)^2
+__global__ __launch_bounds__(RED_THREADS) void k_ls_dchi2(const double2 *__restrict__ Vd,
+                                                          const float2 *__restrict__ Ud,
+                                                          const double *__restrict__ scal, double s_ref,
+                                                          const float *__restrict__ data,
+                                                          const float *__restrict__ err,
+                                                          const float2 *__restrict__ phasor,
+                                                          const int4 *__restrict__ tri, int npow, int nbis,
+                                                          double *part, unsigned *ticket, double *out)
 {
     double v[LS_K];
 #pragma unroll
     for (int k = 0; k < LS_K; k++)
         v[k] = 0.0;
     const double FI = scal[SC_FLUX], Fd = scal[SC_FD];
-    float sk[LS_K], ik[LS_K];
-#pragma unroll
-    for (int k = 0; k < LS_K; k++) {
-        const double s = ls_step(s_ref, k);
-        sk[k] = (float)s;
-        ik[k] = (float)(1.0 / (FI + s * Fd));
-    }
-    const float fI = (float)FI;
     const int total = npow + nbis;
     for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < total; i += gridDim.x * RED_THREADS) {
         if (i < npow) {
-            const float2 a = vis[i], b = Ud[i];
-            const float ax = a.x * fI, ay = a.y * fI, d0 = data[i], e = err[i];
+            const double2 a = Vd[i];
+            const double bx = Ud[i].x, by = Ud[i].y, d0 = data[i], e = err[i];
+            const double r0 = (a.x * a.x + a.y * a.y - d0) * e;
 #pragma unroll
             for (int k = 0; k < LS_K; k++) {
-                const float x = (ax + sk[k] * b.x) * ik[k], y = (ay + sk[k] * b.y) * ik[k];
-                const double r = ((double)(x * x + y * y) - (double)d0) * (double)e;
-                v[k] += r * r;
+                const double s = ls_step(s_ref, k), iv = 1.0 / (FI + s * Fd);
+                const double x = (FI * a.x + s * bx) * iv, y = (FI * a.y + s * by) * iv;
+                const double r = (x * x + y * y - d0) * e;
+                v[k] += r * r - r0 * r0;
             }
         } else {
             const int t = i - npow;
             const int4 tr = tri[t];
-            const float2 a0 = vis[tr.x], a1 = vis[tr.y], a2 = vis[tr.z];
-            const float2 b0 = Ud[tr.x], b1 = Ud[tr.y], b2 = Ud[tr.z];
-            const float2 rho = phasor[t];
+            const int idx[3] = {tr.x, tr.y, tr.z};
+            double2 a[3], b[3];
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                a[j] = Vd[idx[j]];
+                b[j] = make_double2(Ud[idx[j]].x, Ud[idx[j]].y);
+            }
+            const double2 rho = make_double2(phasor[t].x, phasor[t].y);
             const int kk = npow + 2 * t;
-            const float dr = data[kk], di = data[kk + 1], er = err[kk], ei = err[kk + 1];
-            const float c0 = (tr.w & 1) ? -1.f : 1.f, c1 = (tr.w & 2) ? -1.f : 1.f, c2 = (tr.w & 4) ? -1.f : 1.f;
+            const double dr = data[kk], di = data[kk + 1], er = err[kk], ei = err[kk + 1];
+            double2 T0 = dcmul(dcmul(dcmul(dleg(a[0], tr.w & 1), dleg(a[1], tr.w & 2)), dleg(a[2], tr.w & 4)), rho);
+            const double q0 = ((T0.x - dr) * er) * ((T0.x - dr) * er) + ((T0.y - di) * ei) * ((T0.y - di) * ei);
 #pragma unroll
             for (int k = 0; k < LS_K; k++) {
-                const float s = sk[k], iv = ik[k];
-                const float2 l0 = make_float2((a0.x * fI + s * b0.x) * iv, c0 * (a0.y * fI + s * b0.y) * iv);
-                const float2 l1 = make_float2((a1.x * fI + s * b1.x) * iv, c1 * (a1.y * fI + s * b1.y) * iv);
-                const float2 l2 = make_float2((a2.x * fI + s * b2.x) * iv, c2 * (a2.y * fI + s * b2.y) * iv);
-                const float2 T = cmul(cmul(cmul(l0, l1), l2), rho);
-                const double r0 = ((double)T.x - (double)dr) * (double)er;
-                const double r1 = ((double)T.y - (double)di) * (double)ei;
-                v[k] += r0 * r0 + r1 * r1;
+                const double s = ls_step(s_ref, k), iv = 1.0 / (FI + s * Fd);
+                double2 l[3];
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    l[j] = make_double2((FI * a[j].x + s * b[j].x) * iv, (FI * a[j].y + s * b[j].y) * iv);
+                const double2 T =
+                    dcmul(dcmul(dcmul(dleg(l[0], tr.w & 1), dleg(l[1], tr.w & 2)), dleg(l[2], tr.w & 4)), rho);
+                const double r0 = (T.x - dr) * er, r1 = (T.y - di) * ei;
+                v[k] += r0 * r0 + r1 * r1 - q0;
             }
         }
     }
     reduce_finish<LS_K>(v, part, ticket, out, nullptr);
 }
 
-// entropy of max(I + s_k d, minval) for all candidates; out[k] = S_k
-__global__ __launch_bounds__(RED_THREADS) void k_ls_entropy(const float *__restrict__ img,
-                                                            const float *__restrict__ d,
-                                                            const float *__restrict__ model, double s_ref,
-                                                            double minval, int n, double *part, unsigned *ticket,
-                                                            double *out)
+// change of the entropy along the line: out[k] = sum S(max(I + s_k d, minval)) - S(I)
+__global__ __launch_bounds__(RED_THREADS) void k_ls_dentropy(const float *__restrict__ img,
+                                                             const float *__restrict__ d,
+                                                             const float *__restrict__ model, double s_ref,
+                                                             double minval, int n, double *part, unsigned *ticket,
+                                                             double *out)
 {
     double v[LS_K], sk[LS_K];
 #pragma unroll
@@ -554,15 +579,296 @@ __global__ __launch_bounds__(RED_THREADS) void k_ls_entropy(const float *__restr
     }
     for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) {
         const double I = img[i], dd = d[i], M = model[i];
+        const double s0 = pix_entropy(I, M);
 #pragma unroll
         for (int k = 0; k < LS_K; k++) {
             double t = I + sk[k] * dd;
             if (t < minval)
                 t = minval;
-            v[k] += (t > 0.0 && M > 0.0) ? (t - M - t * log(t / M)) : -M;
+            v[k] += pix_entropy(t, M) - s0;
         }
     }
     reduce_finish<LS_K>(v, part, ticket, out, nullptr);
+}
+
+// trial image I_t = max(I + s d, minval), the change delta = I_t - I, and
+// out[0] = sum delta, out[1] = S(I_t) - S(I)
+__global__ __launch_bounds__(RED_THREADS) void k_commit_step(const float *__restrict__ img,
+                                                             const float *__restrict__ d,
+                                                             const float *__restrict__ model, float s, float minval,
+                                                             int n, float *__restrict__ trial,
+                                                             float *__restrict__ delta, double *part,
+                                                             unsigned *ticket, double *out)
+{
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) {
+        const float I = img[i];
+        float t = I + s * d[i];
+        if (t < minval)
+            t = minval;
+        trial[i] = t;
+        const float dl = t - I;
+        delta[i] = dl;
+        const double M = model[i];
+        v[0] += (double)dl;
+        v[1] += pix_entropy((double)t, M) - pix_entropy((double)I, M);
+    }
+    reduce_finish<2>(v, part, ticket, out, nullptr);
+}
+
+// visibilities after the change: Vn = (F_I V + U_delta) / (F_I + F_delta) (FP64),
+// also as floats for the adjoint
+__global__ void k_vis_update(const double2 *__restrict__ Vd, const float2 *__restrict__ Udelta,
+                             const double *__restrict__ scal, int nuv, double2 *__restrict__ Vn,
+                             float2 *__restrict__ vis_f)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nuv)
+        return;
+    const double FI = scal[SC_FLUX], Fdl = scal[SC_FDELTA];
+    const double iv = 1.0 / (FI + Fdl);
+    const double2 a = Vd[p];
+    const double2 r = make_double2((FI * a.x + Udelta[p].x) * iv, (FI * a.y + Udelta[p].y) * iv);
+    Vn[p] = r;
+    vis_f[p] = make_float2((float)r.x, (float)r.y);
+}
+
+// exact chi2 change between Vd and Vn, mock data of Vn; out[0] = sum r(Vn)^2 - r(Vd)^2
+__global__ __launch_bounds__(RED_THREADS) void k_commit_dchi2(const double2 *__restrict__ Vd,
+                                                              const double2 *__restrict__ Vn,
+                                                              const float *__restrict__ data,
+                                                              const float *__restrict__ err,
+                                                              const float2 *__restrict__ phasor,
+                                                              const int4 *__restrict__ tri, int npow, int nbis,
+                                                              float *__restrict__ mock_new, double *part,
+                                                              unsigned *ticket, double *out)
+{
+    double v[1] = {0.0};
+    const int total = npow + nbis;
+    for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < total; i += gridDim.x * RED_THREADS) {
+        if (i < npow) {
+            const double2 a = Vd[i], b = Vn[i];
+            const double d0 = data[i], e = err[i];
+            const double m = b.x * b.x + b.y * b.y;
+            const double r0 = (a.x * a.x + a.y * a.y - d0) * e, r1 = (m - d0) * e;
+            mock_new[i] = (float)m;
+            v[0] += r1 * r1 - r0 * r0;
+        } else {
+            const int t = i - npow;
+            const int4 tr = tri[t];
+            const double2 rho = make_double2(phasor[t].x, phasor[t].y);
+            const int kk = npow + 2 * t;
+            const double dr = data[kk], di = data[kk + 1], er = err[kk], ei = err[kk + 1];
+            const double2 T0 =
+                dcmul(dcmul(dcmul(dleg(Vd[tr.x], tr.w & 1), dleg(Vd[tr.y], tr.w & 2)), dleg(Vd[tr.z], tr.w & 4)), rho);
+            const double2 T1 =
+                dcmul(dcmul(dcmul(dleg(Vn[tr.x], tr.w & 1), dleg(Vn[tr.y], tr.w & 2)), dleg(Vn[tr.z], tr.w & 4)), rho);
+            mock_new[kk] = (float)T1.x;
+            mock_new[kk + 1] = (float)T1.y;
+            const double a0 = (T0.x - dr) * er, a1 = (T0.y - di) * ei;
+            const double b0 = (T1.x - dr) * er, b1 = (T1.y - di) * ei;
+            v[0] += (b0 * b0 + b1 * b1) - (a0 * a0 + a1 * a1);
+        }
+    }
+    reduce_finish<1>(v, part, ticket, out, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// split precision BF16x3: x = hi + lo with hi = bf16(x), lo = bf16(x - hi);
+// a*b ~ ah*bh + ah*bl + al*bh (the al*bl term, ~2^-16 |ab|, is dropped),
+// every product exact in the FP32 MFMA accumulator.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ inline unsigned short bf16_bits_rne(float x)
+{
+    unsigned u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+__device__ inline void split_bf16(float x, unsigned short &hi, unsigned short &lo)
+{
+    hi = bf16_bits_rne(x);
+    lo = bf16_bits_rne(x - __uint_as_float((unsigned)hi << 16));
+}
+
+// X float2 [nuv][W] -> Xh, Xl bf16 [nuv][2][W] (re plane, im plane per uv point)
+__global__ void k_split_tables(const float2 *__restrict__ X, int nuv, int W, unsigned short *__restrict__ Xh,
+                               unsigned short *__restrict__ Xl)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nuv * W)
+        return;
+    const size_t p = i / W, x = i % W;
+    const float2 v = X[i];
+    unsigned short h, l;
+    split_bf16(v.x, h, l);
+    Xh[(2 * p) * W + x] = h;
+    Xl[(2 * p) * W + x] = l;
+    split_bf16(v.y, h, l);
+    Xh[(2 * p + 1) * W + x] = h;
+    Xl[(2 * p + 1) * W + x] = l;
+}
+
+// image float [W*W] -> Ih, Il bf16 [W*W], same [y][x] layout
+__global__ void k_split_image(const float *__restrict__ img, int n, unsigned short *__restrict__ Ih,
+                              unsigned short *__restrict__ Il)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    unsigned short h, l;
+    split_bf16(img[i], h, l);
+    Ih[i] = h;
+    Il[i] = l;
+}
+
+// Forward DFT on v_mfma_f32_16x16x32_bf16 (A: lane l holds A[l%16][8(l/16)..+7],
+// B: B[8(l/16)..+7][l%16], D: D[4(l/16)+r][l%16]).
+// One workgroup = 64 uv points; wave w owns 16 of them as two 16-row MFMA blocks
+// (rows 0-7 = Re X of 8 points, rows 8-15 = Im X of the same points) and sweeps
+// all W image rows in groups of 256 (the N dimension); K = x in LDS-staged chunks
+// of 64 columns of the split image [y][x] (row stride 72 bf16: conflict-free
+// 16-byte reads).  The y reduction with the FP32 Y table happens in registers.
+#define B3_PW 64
+#define B3_KC 64
+#define B3_LDS_STRIDE 72
+#define B3_NG 256
+
+__global__ __launch_bounds__(256) void k_vis_fwd_b3(const unsigned short *__restrict__ Ih,
+                                                    const unsigned short *__restrict__ Il,
+                                                    const unsigned short *__restrict__ Xh,
+                                                    const unsigned short *__restrict__ Xl,
+                                                    const float2 *__restrict__ Y, int nuv, int W,
+                                                    const float *__restrict__ scale, float2 *__restrict__ vis)
+{
+    __shared__ __attribute__((aligned(16))) unsigned short sBh[B3_NG * B3_LDS_STRIDE];
+    __shared__ __attribute__((aligned(16))) unsigned short sBl[B3_NG * B3_LDS_STRIDE];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    const int p0 = blockIdx.x * B3_PW + wave * 16;
+    // the A rows this lane feeds: point p0 + 8 j + (li & 7), component li >> 3
+    const int comp = li >> 3;
+    int prow[2];
+    bool pval[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int p = p0 + 8 * j + (li & 7);
+        pval[j] = p < nuv;
+        prow[j] = pval[j] ? (2 * p + comp) : 0;
+    }
+    float vr[2][4], vi[2][4];
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            vr[j][r] = 0.f;
+            vi[j][r] = 0.f;
+        }
+    const int ngroups = (W + B3_NG - 1) / B3_NG;
+    for (int grp = 0; grp < ngroups; grp++) {
+        const int y0 = grp * B3_NG;
+        const int rows = min(B3_NG, W - y0);
+        const int nbk = rows / 16;
+        v4f acc[2][16];
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int b = 0; b < 16; b++)
+                acc[j][b] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int x0 = 0; x0 < W; x0 += B3_KC) {
+            __syncthreads();
+            // stage image rows y0..y0+rows-1, columns x0..x0+63 (hi and lo)
+            for (int e = tid; e < rows * (B3_KC / 8); e += 256) {
+                const int yy = e / (B3_KC / 8), q = e % (B3_KC / 8);
+                const size_t g = (size_t)(y0 + yy) * W + x0 + 8 * q;
+                *reinterpret_cast<uint4 *>(&sBh[yy * B3_LDS_STRIDE + 8 * q]) = *reinterpret_cast<const uint4 *>(Ih + g);
+                *reinterpret_cast<uint4 *>(&sBl[yy * B3_LDS_STRIDE + 8 * q]) = *reinterpret_cast<const uint4 *>(Il + g);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ks = 0; ks < B3_KC / 32; ks++) {
+                const int kx = x0 + ks * 32 + 8 * lg;
+                bf16x8 ah[2], al[2];
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    uint4 h = make_uint4(0, 0, 0, 0), l = h;
+                    if (pval[j]) {
+                        h = *reinterpret_cast<const uint4 *>(Xh + (size_t)prow[j] * W + kx);
+                        l = *reinterpret_cast<const uint4 *>(Xl + (size_t)prow[j] * W + kx);
+                    }
+                    ah[j] = *reinterpret_cast<bf16x8 *>(&h);
+                    al[j] = *reinterpret_cast<bf16x8 *>(&l);
+                }
+#pragma unroll
+                for (int b = 0; b < 16; b++) {
+                    if (b < nbk) {
+                        const int o = (b * 16 + li) * B3_LDS_STRIDE + ks * 32 + 8 * lg;
+                        const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&sBh[o]);
+                        const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&sBl[o]);
+#pragma unroll
+                        for (int j = 0; j < 2; j++) {
+                            acc[j][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[j], bh, acc[j][b], 0, 0, 0);
+                            acc[j][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[j], bl, acc[j][b], 0, 0, 0);
+                            acc[j][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[j], bh, acc[j][b], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+        }
+        // epilogue of this row group: rows 4 lg + r of block j; lg < 2 hold Re T, lg >= 2 Im T
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int p = p0 + 8 * j + 4 * (lg & 1) + r;
+                if (p < nuv) {
+                    const float2 *yrow = Y + (size_t)p * W + y0 + li;
+#pragma unroll
+                    for (int b = 0; b < 16; b++) {
+                        if (b < nbk) {
+                            const float2 yv = yrow[b * 16];
+                            const float t = acc[j][b][r];
+                            if (lg < 2) {
+                                vr[j][r] += t * yv.x;
+                                vi[j][r] += t * yv.y;
+                            } else {
+                                vr[j][r] -= t * yv.y;
+                                vi[j][r] += t * yv.x;
+                            }
+                        }
+                    }
+                }
+            }
+    }
+    // sum over the 16 y lanes, then Re-row lanes (lg 0,1) + Im-row lanes (lg 2,3)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            float a = vr[j][r], b = vi[j][r];
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                a += __shfl_xor(a, o, 64);
+                b += __shfl_xor(b, o, 64);
+            }
+            a += __shfl_xor(a, 32, 64);
+            b += __shfl_xor(b, 32, 64);
+            vr[j][r] = a;
+            vi[j][r] = b;
+        }
+    if (li == 0 && lg < 2) {
+        const float f = scale ? *scale : 1.0f;
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int p = p0 + 8 * j + 4 * lg + r;
+                if (p < nuv)
+                    vis[p] = make_float2(vr[j][r] * f, vi[j][r] * f);
+            }
+    }
 }
 
 // explicit instantiations for the supported widths (W = 64 * NB)

@@ -297,14 +297,29 @@ CONFIGS = {
     # reference CPU-runnable case: 64x64 @ 0.1 mas, 6 telescopes, 8 ch x 4 snapshots
     "cfg1": dict(W=64, pix=0.1, ntel=6, nch=8, nsnap=4, uvmax=3e8, sky="disk_spots", alpha=0.025),
     # headline: 256x256 @ 0.05 mas, 9 telescopes, 64 ch x 16 snapshots
-    "cfg2": dict(W=256, pix=0.05, ntel=9, nch=64, nsnap=16, uvmax=3e8, sky="ld_disk_spots_companion", alpha=0.2),
+    # prior: the reference's default model, a Gaussian of FWHM 5 mas (gpair1/trunk/gpair.c:82-84), centred
+    "cfg2": dict(W=256, pix=0.05, ntel=9, nch=64, nsnap=16, uvmax=3e8, sky="ld_disk_spots_companion", alpha=0.3,
+                 prior=("gauss", 5.0)),
     # precision variant: 512x512, 256 channels
-    "cfg3": dict(W=512, pix=0.05, ntel=9, nch=256, nsnap=16, uvmax=3e8, sky="ring_disk", alpha=0.2),
+    "cfg3": dict(W=512, pix=0.05, ntel=9, nch=256, nsnap=16, uvmax=3e8, sky="ring_disk", alpha=0.3,
+                 prior=("gauss", 10.0)),
 }
 
 
+def set_model(W, pix, mtype, width, flux, centered=True):
+    """prior image through the library's gpair_set_model (gpair1/trunk/cpu.c:645-749)."""
+    import ctypes as C
+    from . import _lib
+    out = np.zeros(W * W, np.float32)
+    rc = _lib.load().gpair_set_model(out.ctypes.data, int(W), C.c_float(pix), int(mtype), C.c_float(width),
+                                     C.c_float(flux), int(bool(centered)))
+    if rc != 0:
+        raise ValueError(f"gpair_set_model: bad arguments (type {mtype})")
+    return out
+
+
 def make_problem(cfg="cfg1", seed=1, W=None, pix=None, ntel=None, nch=None, nsnap=None, uvmax=None,
-                 sky=None, prior_flux=None, noise=True, vis_model="grid") -> Problem:
+                 sky=None, prior_flux=None, noise=True, vis_model="grid", prior=None) -> Problem:
     """Seeded synthetic problem.  Named configs fix the sizes; keyword
     overrides allow small test shapes with the same construction.
 
@@ -374,12 +389,18 @@ def make_problem(cfg="cfg1", seed=1, W=None, pix=None, ntel=None, nch=None, nsna
     # start image = max(prior, 1e-8) (gpair1/trunk/gpair.c:217-225)
     if prior_flux is None:
         prior_flux = float(W_ * W_)
-    model = np.full(W_ * W_, prior_flux / float(W_ * W_), np.float32)
+    if prior is None:
+        prior = c.get("prior", ("flat", 0.0))
+    if prior[0] == "flat":
+        model = np.full(W_ * W_, prior_flux / float(W_ * W_), np.float32)
+    else:
+        model = set_model(W_, pixf, {"dirac": 1, "disk": 2, "gauss": 3, "lorentz": 4}[prior[0]], prior[1],
+                          prior_flux, centered=True)
     start = np.maximum(model, np.float32(1e-8))
     meta = dict(cfg=cfg, seed=seed, nbase=nbase, ntri=ntri, sky=c["sky"], W=W_, pix=c["pix"],
                 noise="sigV2=max(0.02*V2,1e-4); sigamp=0.05*|T3|; sigphi=1deg",
                 uvmax=c["uvmax"], ntel=c["ntel"], nch=c["nch"], nsnap=c["nsnap"], vis_model=vis_model,
-                prior_flux=prior_flux, components=comps, alpha=c["alpha"])
+                prior_flux=prior_flux, components=comps, alpha=c["alpha"], prior=tuple(prior))
     return Problem(W=W_, pixellation=c["pix"], u=uv[:, 0].copy(), v=uv[:, 1].copy(), npow=npow,
                    nbis=nbis, data=data, err=err, phasor=phasor, uvpnt=uvpnt, sign=sign,
                    model=model, start=start, truth_vis=V, meta=meta, truth=(truth * prior_flux).astype(np.float32))

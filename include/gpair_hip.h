@@ -220,6 +220,22 @@ int gpair_pcg_run(gpair_ctx *ctx, const gpair_pcg_params *params, gpair_pcg_resu
 /* per-pixel preconditioner factor [W*W] multiplying P (NULL clears it) */
 int gpair_set_pfactor(gpair_ctx *ctx, const float *pfactor, int size);
 
+/* ---- host-side data preparation (no device needed) ----------------------- */
+/* set_model, gpair1/trunk/cpu.c:645-749: prior image [W*W] of total flux `flux`.
+ * type 0 flat, 1 centred Dirac, 2 uniform disk (radius width mas), 3 Gaussian
+ * (FWHM width mas), 4 Lorentzian.  The reference draws the Gaussian 80 px off
+ * centre in y (cpu.c:714-715); centered != 0 removes that offset. */
+int gpair_set_model(float *model, int W, float pixellation, int type, float width, float flux, int centered);
+/* init_data, gpair1/trunk/gpair.c:997-1066: packs data [npow + 2 nbis],
+ * inverse errors and phasors (re, im per T3) from V2 / T3 arrays; uvpnt holds 3
+ * uv indices per T3.  With do_extrapolation == 1, triple amplitudes whose error
+ * is <= 0 or > 1e3 are rebuilt from the three V2 points (bisamp / bisamperr are
+ * updated in place, as the reference updates its oi_data); *ndof is decremented
+ * for each T3 that cannot be rebuilt. */
+int gpair_init_data(int npow, int nbis, const float *pow, const float *powerr, float *bisamp, float *bisamperr,
+                    const float *bisphs, const float *bisphserr, const int64_t *uvpnt, int do_extrapolation,
+                    float *data, float *data_err, float *phasor, int *ndof);
+
 /* problem sizes of a context (0 when not set) */
 int gpair_sizes(const gpair_ctx *ctx, int *W, int *nuv, int *npow, int *nbis);
 /* synchronise the context's stream */

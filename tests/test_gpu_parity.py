@@ -196,3 +196,22 @@ def test_pcg_reaches_target_and_objective_is_exact(orc):
     J_o, chi2_o, *_ = orc.objective(P, img, alpha)
     assert abs(res.J - J_o) / abs(J_o) <= 2e-5
     assert chi2_o / prob.ndof <= 1.0 + 1e-5
+
+
+@pytest.mark.parametrize("name,kw", _cases(), ids=[c[0] for c in _cases()])
+def test_bf16x3_forward(name, kw, orc):
+    """split-precision forward DFT (3 BF16 MFMA products, FP32 accumulation)
+    against the FP64 oracle.  Bounds: |dV| <= 3e-5 max|V|, chi2 relative 2e-5."""
+    prob = _problem(name, kw)
+    P = orc.unpack(prob)
+    img = _image(prob, 7)
+    with GpairContext.from_problem(prob, image=img) as ctx:
+        ctx.set_precision(1)
+        chi2 = ctx.gpu_get_chi2_curr()
+        vis = ctx.vis().astype(np.complex128)
+    J, chi2_o, S_o, vis_o, mock_o, F = orc.objective(P, img.astype(np.float64), ALPHA)
+    ev = np.abs(vis - vis_o).max() / np.abs(vis_o).max()
+    ec = abs(chi2 - chi2_o) / chi2_o
+    print(f"  bf16x3 {name}: vis {ev:.2e} chi2 {ec:.2e}", flush=True)
+    assert ev <= 3e-5
+    assert ec <= 2e-5
