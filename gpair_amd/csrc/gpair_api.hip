@@ -47,10 +47,13 @@ struct gpair_ctx {
     double *host_scal;   // pinned
     // adjoint split-K partials
     float *adj_part;
-    int adj_nsplit, adj_pper;
+    int adj_nsplit, adj_pper, adj_cap;
     // split-precision (BF16x3) operands
     unsigned short *Xh, *Xl, *Ih, *Il;
+    unsigned short *Xth, *Xtl;   // x-major split X for the adjoint, points padded to nuvp
+    int nuvp;
     int split_valid;
+    int adj_b3_nsplit, adj_b3_pper;
     // accelerated loop
     gpair_buf_s z, zold, delta;
     float2 *Ud, *vis2;
@@ -127,11 +130,14 @@ static void dfree(T **p)
     }
 }
 
+// one element per thread, at most 128 workgroups: every workgroup of a reduction
+// takes the same ticket with an atomic add, and past ~128 of them that serialised
+// atomic costs more than the extra elements per thread
 static int red_blocks(int n)
 {
-    int b = (n + RED_THREADS * 4 - 1) / (RED_THREADS * 4);
+    int b = (n + RED_THREADS - 1) / RED_THREADS;
     if (b < 1) b = 1;
-    if (b > 1024) b = 1024;
+    if (b > 128) b = 128;
     return b;
 }
 
@@ -204,7 +210,7 @@ extern "C" int gpu_cleanup(gpair_ctx *c)
     dfree(&c->scal); dfree(&c->invflux); dfree(&c->part); dfree(&c->ticket); dfree(&c->adj_part);
     dfree(&c->Ud); dfree(&c->pfac); dfree(&c->ls_out);
     dfree(&c->vis2); dfree(&c->Vd); dfree(&c->Vn); dfree(&c->mock2);
-    dfree(&c->Xh); dfree(&c->Xl); dfree(&c->Ih); dfree(&c->Il);
+    dfree(&c->Xh); dfree(&c->Xl); dfree(&c->Ih); dfree(&c->Il); dfree(&c->Xth); dfree(&c->Xtl);
     if (c->host_scal)
         (void)hipHostFree(c->host_scal);
     if (c->host_ls)
@@ -452,6 +458,8 @@ extern "C" int gpu_build_kernels(gpair_ctx *c, int data_size, int image_width, i
         return rc;
     c->adj_nsplit = nsplit;
     c->adj_pper = pper;
+    c->adj_cap = nsplit;
+    c->split_valid = 0;   // the split-precision adjoint sizes its partials on first use
     c->built = 1;
     return GPAIR_OK;
 }
@@ -478,6 +486,28 @@ static int ensure_split_tables(gpair_ctx *c)
         return rc;
     k_split_tables<<<(unsigned)((tsz + 255) / 256), 256, 0, c->stream>>>(c->X, c->nuv, c->W, c->Xh, c->Xl);
     LAUNCHCHK(c);
+    if (c->W % AB3_XT == 0) {   // the split-precision adjoint tiles the image in 64 x 128
+        c->nuvp = (c->nuv + AB3_PR - 1) / AB3_PR * AB3_PR;
+        const size_t tt = (size_t)c->nuvp * c->W;
+        if ((rc = dalloc(c, &c->Xth, 2 * tt)) || (rc = dalloc(c, &c->Xtl, 2 * tt)))
+            return rc;
+        k_split_tables_t<<<(unsigned)((tt + 255) / 256), 256, 0, c->stream>>>(c->X, c->nuv, c->nuvp, c->W, c->Xth,
+                                                                             c->Xtl);
+        LAUNCHCHK(c);
+        const int tiles = (c->W / AB3_YT) * (c->W / AB3_XT);
+        int nsplit = 512 / tiles;
+        if (nsplit < 1) nsplit = 1;
+        int pper = (c->nuv + nsplit - 1) / nsplit;
+        pper = (pper + AB3_PR - 1) / AB3_PR * AB3_PR;
+        nsplit = (c->nuv + pper - 1) / pper;
+        if (nsplit > c->adj_cap) {   // one partial buffer serves both adjoints
+            if ((rc = dalloc(c, &c->adj_part, (size_t)nsplit * c->n)))
+                return rc;
+            c->adj_cap = nsplit;
+        }
+        c->adj_b3_nsplit = nsplit;
+        c->adj_b3_pper = pper;
+    }
     c->split_valid = 1;
     return GPAIR_OK;
 }
@@ -604,9 +634,21 @@ static int launch_forward_to(gpair_ctx *c, const float *img, const float *scale,
         k_split_image<<<ew_blocks(c->n), 256, 0, c->stream>>>(img, c->n, c->Ih, c->Il);
         LAUNCHCHK(c);
         const unsigned g3 = (unsigned)((c->nuv + B3_PW - 1) / B3_PW);
-        TIMED_LAUNCH(c, GPAIR_TK_FORWARD,
-                     (k_vis_fwd_b3<<<g3, 256, 0, c->stream>>>(c->Ih, c->Il, c->Xh, c->Xl, c->Y, c->nuv, c->W, scale,
-                                                              vis_out)));
+        const int nbk = (c->W < B3_ROWS ? c->W : B3_ROWS) / 32;
+        switch (nbk) {
+#define B3_CASE(N)                                                                                          \
+    case N:                                                                                                 \
+        TIMED_LAUNCH(c, GPAIR_TK_FORWARD,                                                                   \
+                     (k_vis_fwd_b3<N><<<g3, 256, 0, c->stream>>>(c->Ih, c->Il, c->Xh, c->Xl, c->Y, c->nuv, c->W, \
+                                                                 scale, vis_out)));                         \
+        break;
+            B3_CASE(2)
+            B3_CASE(4)
+            B3_CASE(6)
+            B3_CASE(8)
+#undef B3_CASE
+        default: return fail(c, GPAIR_E_ARG, "unsupported width %d", c->W);
+        }
         LAUNCHCHK(c);
         return GPAIR_OK;
     }
@@ -653,11 +695,23 @@ static int launch_data_gradient(gpair_ctx *c, const float *img)
                                                                c->csr_ptr, c->csr_ent, c->npow, c->nuv, c->coef,
                                                                c->part, c->ticket, c->scal);
     LAUNCHCHK(c);
-    dim3 grid((c->W / 64) * (c->W / 64), c->adj_nsplit);
-    TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,
-                 (k_adj_f32<<<grid, 256, 0, c->stream>>>(c->coef, c->X, c->Y, c->nuv, c->W, c->adj_pper, c->adj_part)));
+    int nsplit = c->adj_nsplit;
+    if (c->precision == GPAIR_PREC_BF16X3 && c->W % AB3_XT == 0) {
+        if ((rc = ensure_split_tables(c)))
+            return rc;
+        nsplit = c->adj_b3_nsplit;
+        dim3 grid((c->W / AB3_YT) * (c->W / AB3_XT), nsplit);
+        TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,
+                     (k_adj_b3<<<grid, 256, 0, c->stream>>>(c->coef, c->Y, c->Xth, c->Xtl, c->nuv, c->nuvp, c->W,
+                                                            c->adj_b3_pper, c->adj_part)));
+    } else {
+        dim3 grid((c->W / 64) * (c->W / 64), nsplit);
+        TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,
+                     (k_adj_f32<<<grid, 256, 0, c->stream>>>(c->coef, c->X, c->Y, c->nuv, c->W, c->adj_pper,
+                                                             c->adj_part)));
+    }
     LAUNCHCHK(c);
-    k_grad_finish<<<ew_blocks(c->n), 256, 0, c->stream>>>(c->adj_part, c->adj_nsplit, c->n, c->scal, c->gdata.p);
+    k_grad_finish<<<ew_blocks(c->n), 256, 0, c->stream>>>(c->adj_part, nsplit, c->n, c->scal, c->gdata.p);
     LAUNCHCHK(c);
     return GPAIR_OK;
 }

@@ -38,6 +38,24 @@ enum {
 #define LS_TOP 26
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// split precision BF16x3: x = hi + lo with hi = bf16(x), lo = bf16(x - hi), both
+// rounded to nearest even; a*b ~ ah*bh + ah*bl + al*bh (al*bl, ~2^-16 |ab|, dropped),
+// every product exact in the FP32 MFMA accumulator
+__device__ inline unsigned short bf16_bits_rne(float x)
+{
+    unsigned u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+__device__ inline void split_bf16(float x, unsigned short &hi, unsigned short &lo)
+{
+    hi = bf16_bits_rne(x);
+    lo = bf16_bits_rne(x - __uint_as_float((unsigned)hi << 16));
+}
 
 // ---------------------------------------------------------------------------
 // deterministic block reduction + last-block finish (fixed summation order)

@@ -379,10 +379,136 @@ __global__ void k_grad_finish(const float *__restrict__ part, int nsplit, int n,
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
+    // loads issued 8 at a time, added in split order (the result does not depend on
+    // the grouping because the FP64 sum of 64 FP32 values is exact to ~2^-29 relative
+    // either way, and the order is fixed)
     double s = 0.0;
-    for (int k = 0; k < nsplit; k++)
+    int k = 0;
+    for (; k + 8 <= nsplit; k += 8) {
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            v[j] = part[(size_t)(k + j) * n + i];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            s += (double)v[j];
+    }
+    for (; k < nsplit; k++)
         s += (double)part[(size_t)k * n + i];
     gdata[i] = (float)((s - scal[SC_C0]) / scal[SC_FLUX]);
+}
+
+// ---------------------------------------------------------------------------
+// adjoint DFT in split precision (BF16x3) on v_mfma_f32_32x32x16_bf16.
+// G[y][x] = sum_k A[y][k] B[k][x], k = 2 p + c:
+//   A[y][2p] = Re(a_p Y[p][y]), A[y][2p+1] = -Im(a_p Y[p][y])   (built per call in LDS)
+//   B[2p][x] = Re X[p][x],      B[2p+1][x] = Im X[p][x]          (static split table [x][p][2])
+// One workgroup = a 64 (y) x 128 (x) tile of one uv split; wave w owns columns
+// 32w..32w+31 and both 32-row blocks.  Every round covers 16 uv points (two
+// MFMA k-steps): the 256 threads build the 64 x 32 A tile (hi and lo planes)
+// in LDS, then each wave runs 2 x 2 x 3 MFMAs.
+// ---------------------------------------------------------------------------
+#define AB3_YT 64
+#define AB3_XT 128
+#define AB3_PR 16                 // uv points per round
+#define AB3_STRIDE (2 * AB3_PR + 8) // bf16 per A row in LDS: 32 k + 8 padding
+
+// X float2 [nuv][W] -> Xt hi/lo bf16 [W][nuvp][2] (x-major, (Re, Im) per point),
+// points nuv..nuvp-1 zero
+__global__ void k_split_tables_t(const float2 *__restrict__ X, int nuv, int nuvp, int W,
+                                 unsigned short *__restrict__ Xth, unsigned short *__restrict__ Xtl)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nuvp * W)
+        return;
+    const int x = (int)(i / nuvp), p = (int)(i % nuvp);
+    float2 v = make_float2(0.f, 0.f);
+    if (p < nuv)
+        v = X[(size_t)p * W + x];
+    unsigned short h, l;
+    split_bf16(v.x, h, l);
+    Xth[2 * i] = h;
+    Xtl[2 * i] = l;
+    split_bf16(v.y, h, l);
+    Xth[2 * i + 1] = h;
+    Xtl[2 * i + 1] = l;
+}
+
+__global__ __launch_bounds__(256) void k_adj_b3(const float2 *__restrict__ coef, const float2 *__restrict__ Y,
+                                                const unsigned short *__restrict__ Xth,
+                                                const unsigned short *__restrict__ Xtl, int nuv, int nuvp, int W,
+                                                int p_per_split, float *__restrict__ part)
+{
+    __shared__ __attribute__((aligned(16))) unsigned short sAh[AB3_YT * AB3_STRIDE];
+    __shared__ __attribute__((aligned(16))) unsigned short sAl[AB3_YT * AB3_STRIDE];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 31, lh = lane >> 5;
+    const int tilesx = W / AB3_XT;
+    const int y0 = (blockIdx.x / tilesx) * AB3_YT, x0 = (blockIdx.x % tilesx) * AB3_XT;
+    const int xw = x0 + 32 * wave + lr;   // this lane's B column
+    const int pstart = blockIdx.y * p_per_split;
+    const int pend = min(pstart + p_per_split, nuv);
+    v16f acc[2];
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+            acc[m][r] = 0.f;
+    // A-tile builder: thread -> point pl = tid / 16 of the round, rows yy = 4 (tid % 16) .. +3
+    const int bpl = tid >> 4, byy = (tid & 15) * 4;
+    for (int pr = pstart; pr < pend; pr += AB3_PR) {
+        {
+            const int p = pr + bpl;
+            float2 a = make_float2(0.f, 0.f);
+            float4 y01 = make_float4(0.f, 0.f, 0.f, 0.f), y23 = y01;
+            if (p < pend) {
+                a = coef[p];
+                const float4 *yr = reinterpret_cast<const float4 *>(Y + (size_t)p * W + y0 + byy);
+                y01 = yr[0];
+                y23 = yr[1];
+            }
+            const float2 yv[4] = {make_float2(y01.x, y01.y), make_float2(y01.z, y01.w), make_float2(y23.x, y23.y),
+                                  make_float2(y23.z, y23.w)};
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float2 ay = cmul(a, yv[j]);
+                unsigned short h0, l0, h1, l1;
+                split_bf16(ay.x, h0, l0);
+                split_bf16(-ay.y, h1, l1);
+                const int o = (byy + j) * AB3_STRIDE + 2 * bpl;
+                sAh[o] = h0;
+                sAh[o + 1] = h1;
+                sAl[o] = l0;
+                sAl[o + 1] = l1;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            // B: points pr + 8 ks + 4 lh .. +3, (Re, Im) interleaved = 8 bf16 at column xw
+            const size_t bo = ((size_t)xw * nuvp + pr + 8 * ks + 4 * lh) * 2;
+            const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(Xth + bo);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(Xtl + bo);
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+                const int o = (32 * m + lr) * AB3_STRIDE + 16 * ks + 8 * lh;
+                const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(&sAh[o]);
+                const bf16x8 al = *reinterpret_cast<const bf16x8 *>(&sAl[o]);
+                acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[m], 0, 0, 0);
+                acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[m], 0, 0, 0);
+                acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[m], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    float *out = part + (size_t)blockIdx.y * W * W;
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int y = y0 + 32 * m + 8 * (r >> 2) + 4 * lh + (r & 3);
+            out[(size_t)y * W + xw] = acc[m][r];
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -578,14 +704,22 @@ __global__ __launch_bounds__(RED_THREADS) void k_ls_dentropy(const float *__rest
         sk[k] = ls_step(s_ref, k);
     }
     for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) {
-        const double I = img[i], dd = d[i], M = model[i];
-        const double s0 = pix_entropy(I, M);
+        const float I = img[i], dd = d[i], M = model[i];
+        if (!(I > 0.f && M > 0.f))
+            continue;   // the floor keeps I > 0; S = -M there whatever the step
+        // S(t) - S(I) = dl (1 - ln(I/M)) - t log1p(dl / I), dl = t - I: every term is
+        // proportional to the change, so FP32 resolves it relative to the change itself
+        const float L = logf(I / M), invI = 1.0f / I;
 #pragma unroll
         for (int k = 0; k < LS_K; k++) {
-            double t = I + sk[k] * dd;
-            if (t < minval)
-                t = minval;
-            v[k] += pix_entropy(t, M) - s0;
+            float t = I + (float)sk[k] * dd;
+            if (t < (float)minval)
+                t = (float)minval;
+            const float dl = t - I, q = dl * invI;
+            // ln(t/I): log1p while the change is small, log of the ratio once t is far
+            // below I (there dl rounds to -I and log1p(-1) would be -inf)
+            const float lr = q > -0.5f ? log1pf(q) : logf(t * invI);
+            v[k] += (double)(dl * (1.0f - L) - t * lr);
         }
     }
     reduce_finish<LS_K>(v, part, ticket, out, nullptr);
@@ -674,25 +808,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_commit_dchi2(const double2 *__r
 }
 
 // ---------------------------------------------------------------------------
-// split precision BF16x3: x = hi + lo with hi = bf16(x), lo = bf16(x - hi);
-// a*b ~ ah*bh + ah*bl + al*bh (the al*bl term, ~2^-16 |ab|, is dropped),
-// every product exact in the FP32 MFMA accumulator.
+// split precision BF16x3 forward DFT (split helpers in gpair_kernels.h)
 // ---------------------------------------------------------------------------
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-
-__device__ inline unsigned short bf16_bits_rne(float x)
-{
-    unsigned u = __float_as_uint(x);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
-
-__device__ inline void split_bf16(float x, unsigned short &hi, unsigned short &lo)
-{
-    hi = bf16_bits_rne(x);
-    lo = bf16_bits_rne(x - __uint_as_float((unsigned)hi << 16));
-}
-
 // X float2 [nuv][W] -> Xh, Xl bf16 [nuv][2][W] (re plane, im plane per uv point)
 __global__ void k_split_tables(const float2 *__restrict__ X, int nuv, int W, unsigned short *__restrict__ Xh,
                                unsigned short *__restrict__ Xl)
@@ -724,152 +841,188 @@ __global__ void k_split_image(const float *__restrict__ img, int n, unsigned sho
     Il[i] = l;
 }
 
-// Forward DFT on v_mfma_f32_16x16x32_bf16 (A: lane l holds A[l%16][8(l/16)..+7],
-// B: B[8(l/16)..+7][l%16], D: D[4(l/16)+r][l%16]).
-// One workgroup = 64 uv points; wave w owns 16 of them as two 16-row MFMA blocks
-// (rows 0-7 = Re X of 8 points, rows 8-15 = Im X of the same points) and sweeps
-// all W image rows in groups of 256 (the N dimension); K = x in LDS-staged chunks
-// of 64 columns of the split image [y][x] (row stride 72 bf16: conflict-free
-// 16-byte reads).  The y reduction with the FP32 Y table happens in registers.
+// Forward DFT on v_mfma_f32_32x32x16_bf16 (gfx950).  Operand layouts: lane l
+// holds A[l%32][8(l/32)..+7] and B[8(l/32)..+7][l%32]; accumulator register r
+// holds D[8(r/4) + 4(l/32) + r%4][l%32].
+// One workgroup = 64 uv points; wave w owns 16 of them as one 32-row block
+// (rows 0-15 Re X, rows 16-31 Im X of the same points, so Re and Im of a point
+// meet in one lane as accumulator registers r and r+8) and sweeps the image
+// rows in groups of up to 256 (NBK blocks of 32).  K = x runs through LDS in
+// chunks of 32 columns of the split image [y][x], double-buffered: the global
+// loads of chunk c+1 are in flight while chunk c feeds the MFMAs.  The A operand
+// (split X rows) comes straight from global memory, one k-step ahead.
 #define B3_PW 64
-#define B3_KC 64
-#define B3_LDS_STRIDE 72
-#define B3_NG 256
+#define B3_KC 32
+#define B3_STRIDE 40   // bf16 per LDS row: 32 + 8 padding, conflict-free 16-byte reads
+#define B3_ROWS 256
 
-__global__ __launch_bounds__(256) void k_vis_fwd_b3(const unsigned short *__restrict__ Ih,
-                                                    const unsigned short *__restrict__ Il,
-                                                    const unsigned short *__restrict__ Xh,
-                                                    const unsigned short *__restrict__ Xl,
-                                                    const float2 *__restrict__ Y, int nuv, int W,
-                                                    const float *__restrict__ scale, float2 *__restrict__ vis)
+
+// cooperative copy of one 32-column chunk of the split image rows into registers,
+// then into LDS (row stride B3_STRIDE); up to 4 pieces of 16 bytes per thread, kept
+// in named registers (an indexed array would be demoted to scratch memory)
+template <int Q, int CH>
+__device__ __forceinline__ void b3_fetch1(uint4 &h, uint4 &l, const unsigned short *__restrict__ Ih,
+                                          const unsigned short *__restrict__ Il, int tid, int y0, int W, int cc)
 {
-    __shared__ __attribute__((aligned(16))) unsigned short sBh[B3_NG * B3_LDS_STRIDE];
-    __shared__ __attribute__((aligned(16))) unsigned short sBl[B3_NG * B3_LDS_STRIDE];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int li = lane & 15, lg = lane >> 4;
-    const int p0 = blockIdx.x * B3_PW + wave * 16;
-    // the A rows this lane feeds: point p0 + 8 j + (li & 7), component li >> 3
-    const int comp = li >> 3;
-    int prow[2];
-    bool pval[2];
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
-        const int p = p0 + 8 * j + (li & 7);
-        pval[j] = p < nuv;
-        prow[j] = pval[j] ? (2 * p + comp) : 0;
-    }
-    float vr[2][4], vi[2][4];
-#pragma unroll
-    for (int j = 0; j < 2; j++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            vr[j][r] = 0.f;
-            vi[j][r] = 0.f;
-        }
-    const int ngroups = (W + B3_NG - 1) / B3_NG;
-    for (int grp = 0; grp < ngroups; grp++) {
-        const int y0 = grp * B3_NG;
-        const int rows = min(B3_NG, W - y0);
-        const int nbk = rows / 16;
-        v4f acc[2][16];
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-#pragma unroll
-            for (int b = 0; b < 16; b++)
-                acc[j][b] = v4f{0.f, 0.f, 0.f, 0.f};
-        for (int x0 = 0; x0 < W; x0 += B3_KC) {
-            __syncthreads();
-            // stage image rows y0..y0+rows-1, columns x0..x0+63 (hi and lo)
-            for (int e = tid; e < rows * (B3_KC / 8); e += 256) {
-                const int yy = e / (B3_KC / 8), q = e % (B3_KC / 8);
-                const size_t g = (size_t)(y0 + yy) * W + x0 + 8 * q;
-                *reinterpret_cast<uint4 *>(&sBh[yy * B3_LDS_STRIDE + 8 * q]) = *reinterpret_cast<const uint4 *>(Ih + g);
-                *reinterpret_cast<uint4 *>(&sBl[yy * B3_LDS_STRIDE + 8 * q]) = *reinterpret_cast<const uint4 *>(Il + g);
-            }
-            __syncthreads();
-#pragma unroll
-            for (int ks = 0; ks < B3_KC / 32; ks++) {
-                const int kx = x0 + ks * 32 + 8 * lg;
-                bf16x8 ah[2], al[2];
-#pragma unroll
-                for (int j = 0; j < 2; j++) {
-                    uint4 h = make_uint4(0, 0, 0, 0), l = h;
-                    if (pval[j]) {
-                        h = *reinterpret_cast<const uint4 *>(Xh + (size_t)prow[j] * W + kx);
-                        l = *reinterpret_cast<const uint4 *>(Xl + (size_t)prow[j] * W + kx);
-                    }
-                    ah[j] = *reinterpret_cast<bf16x8 *>(&h);
-                    al[j] = *reinterpret_cast<bf16x8 *>(&l);
-                }
-#pragma unroll
-                for (int b = 0; b < 16; b++) {
-                    if (b < nbk) {
-                        const int o = (b * 16 + li) * B3_LDS_STRIDE + ks * 32 + 8 * lg;
-                        const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&sBh[o]);
-                        const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&sBl[o]);
-#pragma unroll
-                        for (int j = 0; j < 2; j++) {
-                            acc[j][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[j], bh, acc[j][b], 0, 0, 0);
-                            acc[j][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[j], bl, acc[j][b], 0, 0, 0);
-                            acc[j][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[j], bh, acc[j][b], 0, 0, 0);
-                        }
-                    }
-                }
-            }
-        }
-        // epilogue of this row group: rows 4 lg + r of block j; lg < 2 hold Re T, lg >= 2 Im T
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int p = p0 + 8 * j + 4 * (lg & 1) + r;
-                if (p < nuv) {
-                    const float2 *yrow = Y + (size_t)p * W + y0 + li;
-#pragma unroll
-                    for (int b = 0; b < 16; b++) {
-                        if (b < nbk) {
-                            const float2 yv = yrow[b * 16];
-                            const float t = acc[j][b][r];
-                            if (lg < 2) {
-                                vr[j][r] += t * yv.x;
-                                vi[j][r] += t * yv.y;
-                            } else {
-                                vr[j][r] -= t * yv.y;
-                                vi[j][r] += t * yv.x;
-                            }
-                        }
-                    }
-                }
-            }
-    }
-    // sum over the 16 y lanes, then Re-row lanes (lg 0,1) + Im-row lanes (lg 2,3)
-#pragma unroll
-    for (int j = 0; j < 2; j++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            float a = vr[j][r], b = vi[j][r];
-#pragma unroll
-            for (int o = 1; o < 16; o <<= 1) {
-                a += __shfl_xor(a, o, 64);
-                b += __shfl_xor(b, o, 64);
-            }
-            a += __shfl_xor(a, 32, 64);
-            b += __shfl_xor(b, 32, 64);
-            vr[j][r] = a;
-            vi[j][r] = b;
-        }
-    if (li == 0 && lg < 2) {
-        const float f = scale ? *scale : 1.0f;
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int p = p0 + 8 * j + 4 * lg + r;
-                if (p < nuv)
-                    vis[p] = make_float2(vr[j][r] * f, vi[j][r] * f);
-            }
+    const int e = tid + 256 * Q;
+    if (CH % 256 == 0 || e < CH) {
+        const size_t g = (size_t)(y0 + e / (B3_KC / 8)) * W + cc * B3_KC + 8 * (e % (B3_KC / 8));
+        h = *reinterpret_cast<const uint4 *>(Ih + g);
+        l = *reinterpret_cast<const uint4 *>(Il + g);
     }
 }
+
+template <int Q, int CH>
+__device__ __forceinline__ void b3_store1(const uint4 &h, const uint4 &l, unsigned short *sh, unsigned short *sl,
+                                          int tid)
+{
+    const int e = tid + 256 * Q;
+    if (CH % 256 == 0 || e < CH) {
+        const int o = (e / (B3_KC / 8)) * B3_STRIDE + 8 * (e % (B3_KC / 8));
+        *reinterpret_cast<uint4 *>(sh + o) = h;
+        *reinterpret_cast<uint4 *>(sl + o) = l;
+    }
+}
+
+template <int PER, int CH>
+__device__ __forceinline__ void b3_fetch(uint4 &h0, uint4 &h1, uint4 &h2, uint4 &h3, uint4 &l0, uint4 &l1,
+                                         uint4 &l2, uint4 &l3, const unsigned short *__restrict__ Ih,
+                                         const unsigned short *__restrict__ Il, int tid, int y0, int W, int cc)
+{
+    if constexpr (PER > 0) b3_fetch1<0, CH>(h0, l0, Ih, Il, tid, y0, W, cc);
+    if constexpr (PER > 1) b3_fetch1<1, CH>(h1, l1, Ih, Il, tid, y0, W, cc);
+    if constexpr (PER > 2) b3_fetch1<2, CH>(h2, l2, Ih, Il, tid, y0, W, cc);
+    if constexpr (PER > 3) b3_fetch1<3, CH>(h3, l3, Ih, Il, tid, y0, W, cc);
+}
+
+template <int PER, int CH>
+__device__ __forceinline__ void b3_store(const uint4 &h0, const uint4 &h1, const uint4 &h2, const uint4 &h3,
+                                         const uint4 &l0, const uint4 &l1, const uint4 &l2, const uint4 &l3,
+                                         unsigned short *sh, unsigned short *sl, int tid)
+{
+    if constexpr (PER > 0) b3_store1<0, CH>(h0, l0, sh, sl, tid);
+    if constexpr (PER > 1) b3_store1<1, CH>(h1, l1, sh, sl, tid);
+    if constexpr (PER > 2) b3_store1<2, CH>(h2, l2, sh, sl, tid);
+    if constexpr (PER > 3) b3_store1<3, CH>(h3, l3, sh, sl, tid);
+}
+
+template <int NBK>
+__global__ __launch_bounds__(256) void k_vis_fwd_b3(const unsigned short *__restrict__ Ih,
+                                                       const unsigned short *__restrict__ Il,
+                                                       const unsigned short *__restrict__ Xh,
+                                                       const unsigned short *__restrict__ Xl,
+                                                       const float2 *__restrict__ Y, int nuv, int W,
+                                                       const float *__restrict__ scale, float2 *__restrict__ vis)
+{
+    constexpr int ROWS = NBK * 32;              // image rows per group
+    constexpr int CH = ROWS * (B3_KC / 8);      // 16-byte pieces per array per chunk
+    constexpr int PER = (CH + 255) / 256;       // pieces per thread
+    __shared__ __attribute__((aligned(16))) unsigned short sB[2][2][ROWS * B3_STRIDE];   // [buf][hi/lo]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 31, lh = lane >> 5;
+    const int p0 = blockIdx.x * B3_PW + wave * 16;
+    const int pa = p0 + (lr & 15);               // A row of this lane: point pa, component lr >> 4
+    const bool aval = pa < nuv;
+    const size_t arow = aval ? (size_t)(2 * pa + (lr >> 4)) * W : 0;
+    float vr[8], vi[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        vr[r] = 0.f;
+        vi[r] = 0.f;
+    }
+    const int nchunk = W / B3_KC;
+    for (int y0 = 0; y0 < W; y0 += ROWS) {
+        v16f acc[NBK];
+#pragma unroll
+        for (int b = 0; b < NBK; b++)
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+                acc[b][r] = 0.f;
+        static_assert(PER <= 4, "staging holds up to 4 pieces per thread");
+        uint4 h0 = make_uint4(0, 0, 0, 0), h1 = h0, h2 = h0, h3 = h0, l0 = h0, l1 = h0, l2 = h0, l3 = h0;
+        b3_fetch<PER, CH>(h0, h1, h2, h3, l0, l1, l2, l3, Ih, Il, tid, y0, W, 0);
+        __syncthreads();   // the previous group's readers are done with sB
+        b3_store<PER, CH>(h0, h1, h2, h3, l0, l1, l2, l3, sB[0][0], sB[0][1], tid);
+        __syncthreads();
+        uint4 ah_n = make_uint4(0, 0, 0, 0), al_n = ah_n;
+        if (aval) {
+            ah_n = *reinterpret_cast<const uint4 *>(Xh + arow + 8 * lh);
+            al_n = *reinterpret_cast<const uint4 *>(Xl + arow + 8 * lh);
+        }
+        for (int c = 0; c < nchunk; c++) {
+            const int buf = c & 1;
+            if (c + 1 < nchunk)
+                b3_fetch<PER, CH>(h0, h1, h2, h3, l0, l1, l2, l3, Ih, Il, tid, y0, W, c + 1);
+#pragma unroll
+            for (int ks = 0; ks < B3_KC / 16; ks++) {
+                const uint4 ah_c = ah_n, al_c = al_n;
+                const int kn = c * B3_KC + ks * 16 + 16;   // next k-step's first column
+                if (aval && kn < W) {
+                    ah_n = *reinterpret_cast<const uint4 *>(Xh + arow + kn + 8 * lh);
+                    al_n = *reinterpret_cast<const uint4 *>(Xl + arow + kn + 8 * lh);
+                }
+                const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(&ah_c);
+                const bf16x8 al = *reinterpret_cast<const bf16x8 *>(&al_c);
+#pragma unroll
+                for (int b = 0; b < NBK; b++) {
+                    const int o = (b * 32 + lr) * B3_STRIDE + ks * 16 + 8 * lh;
+                    const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&sB[buf][0][o]);
+                    const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&sB[buf][1][o]);
+                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[b], 0, 0, 0);
+                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[b], 0, 0, 0);
+                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[b], 0, 0, 0);
+                }
+            }
+            if (c + 1 < nchunk) {
+                // the other buffer was last read in chunk c-1, before the barrier below
+                b3_store<PER, CH>(h0, h1, h2, h3, l0, l1, l2, l3, sB[buf ^ 1][0], sB[buf ^ 1][1], tid);
+                __syncthreads();
+            }
+        }
+        // epilogue of this row group: point 8(r/4) + 4 lh + r%4 (r < 8) has Re T in
+        // register r and Im T in register r+8, column y = y0 + 32 b + lr
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const int p = p0 + 8 * (r >> 2) + 4 * lh + (r & 3);
+            if (p < nuv) {
+                const float2 *yrow = Y + (size_t)p * W + y0 + lr;
+#pragma unroll
+                for (int b = 0; b < NBK; b++) {
+                    const float2 yv = yrow[b * 32];
+                    const float tr = acc[b][r], ti = acc[b][r + 8];
+                    vr[r] += tr * yv.x - ti * yv.y;
+                    vi[r] += tr * yv.y + ti * yv.x;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            vr[r] += __shfl_xor(vr[r], o, 64);
+            vi[r] += __shfl_xor(vi[r], o, 64);
+        }
+    }
+    if (lr == 0) {
+        const float f = scale ? *scale : 1.0f;
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const int p = p0 + 8 * (r >> 2) + 4 * lh + (r & 3);
+            if (p < nuv)
+                vis[p] = make_float2(vr[r] * f, vi[r] * f);
+        }
+    }
+}
+
+template __global__ void k_vis_fwd_b3<2>(const unsigned short *, const unsigned short *, const unsigned short *,
+                                         const unsigned short *, const float2 *, int, int, const float *, float2 *);
+template __global__ void k_vis_fwd_b3<4>(const unsigned short *, const unsigned short *, const unsigned short *,
+                                         const unsigned short *, const float2 *, int, int, const float *, float2 *);
+template __global__ void k_vis_fwd_b3<6>(const unsigned short *, const unsigned short *, const unsigned short *,
+                                         const unsigned short *, const float2 *, int, int, const float *, float2 *);
+template __global__ void k_vis_fwd_b3<8>(const unsigned short *, const unsigned short *, const unsigned short *,
+                                         const unsigned short *, const float2 *, int, int, const float *, float2 *);
 
 // explicit instantiations for the supported widths (W = 64 * NB)
 template __global__ void k_vis_fwd_f32<1>(const float *, const float2 *, const float2 *, int, const float *, float2 *);

@@ -209,9 +209,14 @@ def test_bf16x3_forward(name, kw, orc):
         ctx.set_precision(1)
         chi2 = ctx.gpu_get_chi2_curr()
         vis = ctx.vis().astype(np.complex128)
+        ctx.gpu_compute_data_gradient_curr()
+        gd = ctx.get("dg").astype(np.float64)
     J, chi2_o, S_o, vis_o, mock_o, F = orc.objective(P, img.astype(np.float64), ALPHA)
+    gd_o = orc.gradient(P, img.astype(np.float64), ALPHA, vis_o, mock_o)[1]
     ev = np.abs(vis - vis_o).max() / np.abs(vis_o).max()
     ec = abs(chi2 - chi2_o) / chi2_o
-    print(f"  bf16x3 {name}: vis {ev:.2e} chi2 {ec:.2e}", flush=True)
+    eg = np.abs(gd - gd_o).max() / np.abs(gd_o).max()
+    print(f"  bf16x3 {name}: vis {ev:.2e} chi2 {ec:.2e} grad {eg:.2e}", flush=True)
     assert ev <= 3e-5
     assert ec <= 2e-5
+    assert eg <= 2e-4   # widths that are not a multiple of 128 use the FP32 adjoint
