@@ -11,12 +11,15 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgpair_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "gpair_hip.h")
+HEADERS = [HEADER, os.path.join(os.path.dirname(HERE), "include", "gpair_nn.h")]
 
 _vp = C.c_void_p
 _i = C.c_int
 _d = C.c_double
 _pd = C.POINTER(C.c_double)
 _pi = C.POINTER(C.c_int)
+_l = C.c_long
+_f = C.c_float
 
 
 class CgParams(C.Structure):
@@ -89,8 +92,28 @@ SIGNATURES = {
     "gpair_pcg_default_params": (None, [C.POINTER(PcgParams)]),
     "gpair_pcg_run": (_i, [_vp, C.POINTER(PcgParams), C.POINTER(PcgResult), _vp, _i]),
     "gpair_set_pfactor": (_i, [_vp, _vp, _i]),
+    "gpair_autotune": (_i, [_vp, _i, _vp, _i]),
     "gpair_set_model": (_i, [_vp, _i, C.c_float, _i, C.c_float, C.c_float, _i]),
     "gpair_init_data": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _pi]),
+    # learned start (include/gpair_nn.h)
+    "gpair_nn_alloc": (_vp, [_l]),
+    "gpair_nn_free": (_i, [_vp]),
+    "gpair_nn_upload": (_i, [_vp, _vp, _l]),
+    "gpair_nn_download": (_i, [_vp, _vp, _l]),
+    "gpair_nn_conv3_fwd": (_i, [_vp, _l, _i, _vp, _vp, _vp, _l, _i, _i, _i, _i, _i]),
+    "gpair_nn_conv3_bwd": (_i, [_vp, _l, _i, _vp, _vp, _l, _i, _i, _i, _i, _vp, _l, _vp, _vp]),
+    "gpair_nn_pool2": (_i, [_vp, _l, _vp, _l, _i, _i, _i, _i, _i]),
+    "gpair_nn_up2": (_i, [_vp, _l, _vp, _l, _i, _i, _i, _i, _i]),
+    "gpair_unet_create": (_i, [C.POINTER(_vp), _i, _i, _i, _i, _i, _i, C.c_ulonglong]),
+    "gpair_unet_destroy": (_i, [_vp]),
+    "gpair_unet_num_params": (_l, [_vp]),
+    "gpair_unet_get_params": (_i, [_vp, _vp, _l]),
+    "gpair_unet_set_params": (_i, [_vp, _vp, _l]),
+    "gpair_unet_get_grads": (_i, [_vp, _vp, _l]),
+    "gpair_unet_forward": (_i, [_vp, _vp, _i, _vp]),
+    "gpair_unet_backward": (_i, [_vp, _vp, _i]),
+    "gpair_unet_adam": (_i, [_vp, _f]),
+    "gpair_unet_train_step": (_i, [_vp, _vp, _vp, _i, _f, _pd]),
 }
 
 _LIB = None
@@ -112,10 +135,10 @@ def load(path: str = LIB_PATH):
     return L
 
 
-def header_symbols(path: str = HEADER):
-    """function names declared in include/gpair_hip.h"""
+def header_symbols(paths=None):
+    """function names declared in include/gpair_hip.h and include/gpair_nn.h"""
     import re
-    txt = open(path).read()
+    txt = "\n".join(open(p).read() for p in (paths or HEADERS))
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt)
     skip = {"if", "defined"}

@@ -35,7 +35,7 @@ def _run(cmd):
 def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    for src in ("gpair_api.hip",):
+    for src in ("gpair_api.hip", "gpair_nn.hip"):
         obj = os.path.join(BUILD, src + ".o")
         _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

@@ -61,6 +61,8 @@ struct gpair_ctx {
     float *mock2;
     float *pfac;
     double *ls_out, *host_ls;   // device / pinned host, 2*LS_K
+    // launch shapes chosen by gpair_autotune (0: the built-in default)
+    int ls_blocks_chi2, ls_blocks_ent, coef_blocks;
     // optional per-launch timing of the DFT kernels (HIP events on the context stream)
     int timing;
     std::vector<hipEvent_t> ev[GPAIR_TK_COUNT];
@@ -142,6 +144,8 @@ static int red_blocks(int n)
 }
 
 static inline int ew_blocks(int n) { return (n + 255) / 256; }
+
+static inline int tuned_blocks(int v, int dflt) { return v > 0 ? v : dflt; }
 
 // ---------------------------------------------------------------------------
 // setup / teardown
@@ -435,6 +439,36 @@ extern "C" int gpu_copy_dft(gpair_ctx *c, const float *dft_x, const float *dft_y
     return err;
 }
 
+// split of the adjoint over uv for about `groups` workgroups in total (b3: the
+// split-precision adjoint's 64 x 128 tiles, else the FP32 adjoint's 64 x 64 tiles);
+// grows the shared partial buffer when needed
+static int set_adj_split(gpair_ctx *c, int b3, int groups)
+{
+    const int tiles = b3 ? (c->W / AB3_YT) * (c->W / AB3_XT) : (c->W / 64) * (c->W / 64);
+    const int gran = b3 ? AB3_PR : 8;
+    int nsplit = groups / tiles;
+    if (nsplit < 1) nsplit = 1;
+    const int maxsplit = (c->nuv + gran - 1) / gran;
+    if (nsplit > maxsplit) nsplit = maxsplit;
+    int pper = (c->nuv + nsplit - 1) / nsplit;
+    pper = (pper + gran - 1) / gran * gran;
+    nsplit = (c->nuv + pper - 1) / pper;
+    if (nsplit > c->adj_cap) {   // one partial buffer serves both adjoints
+        int rc;
+        if ((rc = dalloc(c, &c->adj_part, (size_t)nsplit * c->n)))
+            return rc;
+        c->adj_cap = nsplit;
+    }
+    if (b3) {
+        c->adj_b3_nsplit = nsplit;
+        c->adj_b3_pper = pper;
+    } else {
+        c->adj_nsplit = nsplit;
+        c->adj_pper = pper;
+    }
+    return GPAIR_OK;
+}
+
 extern "C" int gpu_build_kernels(gpair_ctx *c, int data_size, int image_width, int image_size)
 {
     if (!c)
@@ -445,20 +479,11 @@ extern "C" int gpu_build_kernels(gpair_ctx *c, int data_size, int image_width, i
         return fail(c, GPAIR_E_ARG, "gpu_build_kernels: sizes (%d, %d, %d) differ from the data (%d, %d, %d)",
                     data_size, image_width, image_size, c->ndof, c->W, c->n);
     // adjoint split over uv: about 1024 workgroups in total
-    const int tiles = (c->W / 64) * (c->W / 64);
-    int nsplit = 1024 / tiles;
-    if (nsplit < 1) nsplit = 1;
-    int maxsplit = (c->nuv + 7) / 8;
-    if (nsplit > maxsplit) nsplit = maxsplit;
-    int pper = (c->nuv + nsplit - 1) / nsplit;
-    pper = (pper + 7) / 8 * 8;
-    nsplit = (c->nuv + pper - 1) / pper;
     int rc;
-    if ((rc = dalloc(c, &c->adj_part, (size_t)nsplit * c->n)))
+    c->adj_cap = 0;
+    c->adj_b3_nsplit = 0;
+    if ((rc = set_adj_split(c, 0, 1024)))
         return rc;
-    c->adj_nsplit = nsplit;
-    c->adj_pper = pper;
-    c->adj_cap = nsplit;
     c->split_valid = 0;   // the split-precision adjoint sizes its partials on first use
     c->built = 1;
     return GPAIR_OK;
@@ -494,19 +519,8 @@ static int ensure_split_tables(gpair_ctx *c)
         k_split_tables_t<<<(unsigned)((tt + 255) / 256), 256, 0, c->stream>>>(c->X, c->nuv, c->nuvp, c->W, c->Xth,
                                                                              c->Xtl);
         LAUNCHCHK(c);
-        const int tiles = (c->W / AB3_YT) * (c->W / AB3_XT);
-        int nsplit = 512 / tiles;
-        if (nsplit < 1) nsplit = 1;
-        int pper = (c->nuv + nsplit - 1) / nsplit;
-        pper = (pper + AB3_PR - 1) / AB3_PR * AB3_PR;
-        nsplit = (c->nuv + pper - 1) / pper;
-        if (nsplit > c->adj_cap) {   // one partial buffer serves both adjoints
-            if ((rc = dalloc(c, &c->adj_part, (size_t)nsplit * c->n)))
-                return rc;
-            c->adj_cap = nsplit;
-        }
-        c->adj_b3_nsplit = nsplit;
-        c->adj_b3_pper = pper;
+        if (!c->adj_b3_nsplit && (rc = set_adj_split(c, 1, 512)))
+            return rc;
     }
     c->split_valid = 1;
     return GPAIR_OK;
@@ -686,15 +700,24 @@ static int launch_chi2(gpair_ctx *c, const float *img)
     return GPAIR_OK;
 }
 
+static int launch_adjoint(gpair_ctx *c);
+
 static int launch_data_gradient(gpair_ctx *c, const float *img)
 {
     int rc;
     if ((rc = launch_flux(c, img)))
         return rc;
-    k_coef<<<red_blocks(c->nuv), RED_THREADS, 0, c->stream>>>(c->vis, c->mock, c->data, c->err, c->phasor, c->tri,
+    k_coef<<<tuned_blocks(c->coef_blocks, red_blocks(c->nuv)), RED_THREADS, 0, c->stream>>>(c->vis, c->mock, c->data, c->err, c->phasor, c->tri,
                                                                c->csr_ptr, c->csr_ent, c->npow, c->nuv, c->coef,
                                                                c->part, c->ticket, c->scal);
     LAUNCHCHK(c);
+    return launch_adjoint(c);
+}
+
+// adjoint DFT of the coefficients in c->coef and the fixed-order sum of its partials
+static int launch_adjoint(gpair_ctx *c)
+{
+    int rc;
     int nsplit = c->adj_nsplit;
     if (c->precision == GPAIR_PREC_BF16X3 && c->W % AB3_XT == 0) {
         if ((rc = ensure_split_tables(c)))
@@ -1064,3 +1087,4 @@ static inline void swap_ptr(T *&a, T *&b)
     b = t;
 }
 #include "gpair_pcg.inc"
+#include "gpair_tune.inc"

@@ -215,6 +215,41 @@ def sky_components(kind, rng, fov_mas, pix):
                               y=r * math.sin(th)))
         r, th = rng.uniform(0.3, 0.42) * fov_mas, rng.uniform(0, 2 * math.pi)
         comps.append(dict(kind="point", flux=fc, x=snap(r * math.cos(th)), y=snap(r * math.sin(th))))
+    elif kind == "learn_mix":
+        # family for the learned start: a disk, limb-darkened disk, ring (with an
+        # inner disk) or 2-4 Gaussian blobs, 0-3 Gaussian spots on disks, 0-3 point
+        # sources in the outer field
+        R = rng.uniform(0.12, 0.22) * fov_mas
+        x0, y0 = rng.uniform(-0.03, 0.03, size=2) * fov_mas
+        base = ("disk", "ld_disk", "ring", "blobs")[int(rng.integers(4))]
+        nsp = int(rng.integers(0, 4)) if base in ("disk", "ld_disk") else 0
+        npt = int(rng.integers(0, 4))
+        fs = rng.uniform(0.03, 0.10, size=nsp)
+        fp = rng.uniform(0.01, 0.05, size=npt)
+        fb = 1.0 - fs.sum() - fp.sum()
+        if base == "disk":
+            comps.append(dict(kind="disk", flux=fb, R=R, x=x0, y=y0))
+        elif base == "ld_disk":
+            comps.append(dict(kind="ld_disk", flux=fb, R=R, mu=rng.uniform(0.2, 0.6), x=x0, y=y0))
+        elif base == "ring":
+            fr = rng.uniform(0.4, 0.8)
+            comps.append(dict(kind="ring", flux=fb * fr, R=R, width=rng.uniform(0.04, 0.1) * R, x=x0, y=y0))
+            comps.append(dict(kind="disk", flux=fb * (1 - fr), R=rng.uniform(0.2, 0.5) * R,
+                              x=x0 + rng.uniform(-0.2, 0.2) * R, y=y0 + rng.uniform(-0.2, 0.2) * R))
+        else:
+            nb = int(rng.integers(2, 5))
+            w = rng.uniform(0.5, 1.0, size=nb)
+            for f in fb * w / w.sum():
+                r, th = rng.uniform(0.0, 1.0) * R, rng.uniform(0, 2 * math.pi)
+                comps.append(dict(kind="gauss", flux=f, fwhm=rng.uniform(0.3, 0.8) * R, x=x0 + r * math.cos(th),
+                                  y=y0 + r * math.sin(th)))
+        for f in fs:
+            r, th = rng.uniform(0.1, 0.8) * R, rng.uniform(0, 2 * math.pi)
+            comps.append(dict(kind="gauss", flux=f, fwhm=rng.uniform(0.15, 0.4) * R, x=x0 + r * math.cos(th),
+                              y=y0 + r * math.sin(th)))
+        for f in fp:
+            r, th = rng.uniform(0.25, 0.42) * fov_mas, rng.uniform(0, 2 * math.pi)
+            comps.append(dict(kind="point", flux=f, x=snap(r * math.cos(th)), y=snap(r * math.sin(th))))
     elif kind == "ring_disk":
         R = 0.2 * fov_mas
         fr = rng.uniform(0.3, 0.6)
@@ -301,6 +336,12 @@ CONFIGS = {
     "cfg2": dict(W=256, pix=0.05, ntel=9, nch=64, nsnap=16, uvmax=3e8, sky="ld_disk_spots_companion", alpha=0.3,
                  prior=("gauss", 5.0)),
     # precision variant: 512x512, 256 channels
+    # learned start: skies from the learn_mix family on the headline array and grid
+    # (closure triangles with |T3| < 1e-3 are not observed: smooth skies have closure
+    # amplitudes far below any instrument's reach, and the reference weights the
+    # closure phase by 1 / |observed amplitude|)
+    "cfg5": dict(W=256, pix=0.05, ntel=9, nch=64, nsnap=16, uvmax=3e8, sky="learn_mix", alpha=0.3,
+                 prior=("gauss", 5.0), t3_min=1e-3),
     "cfg3": dict(W=512, pix=0.05, ntel=9, nch=256, nsnap=16, uvmax=3e8, sky="ring_disk", alpha=0.3,
                  prior=("gauss", 10.0)),
 }
@@ -319,7 +360,7 @@ def set_model(W, pix, mtype, width, flux, centered=True):
 
 
 def make_problem(cfg="cfg1", seed=1, W=None, pix=None, ntel=None, nch=None, nsnap=None, uvmax=None,
-                 sky=None, prior_flux=None, noise=True, vis_model="grid", prior=None) -> Problem:
+                 sky=None, prior_flux=None, noise=True, vis_model="grid", prior=None, dft=None) -> Problem:
     """Seeded synthetic problem.  Named configs fix the sizes; keyword
     overrides allow small test shapes with the same construction.
 
@@ -328,6 +369,7 @@ def make_problem(cfg="cfg1", seed=1, W=None, pix=None, ntel=None, nch=None, nsna
                              truth image reaches chi2/ndof ~ 1 (default);
                "analytic" -- closed-form visibilities of the continuous sky.
     prior_flux: total flux of the flat prior (default W*W: mean prior pixel 1).
+    dft: replacement for dft64 with the same signature (e.g. an FP64 device DFT).
     meta["alpha"] is the entropy weight used with this prior scale: J(cI; cM)
     with alpha/c equals J(I; M) with alpha, so alpha scales inversely with the
     prior flux (steps of the line search scale with its square)."""
@@ -347,7 +389,7 @@ def make_problem(cfg="cfg1", seed=1, W=None, pix=None, ntel=None, nch=None, nsna
     truth = render(comps, W_, pixf)
     alpha_px = RPMAS * pixf
     if vis_model == "grid":
-        V = dft64(truth, W_, alpha_px * u, alpha_px * v)
+        V = (dft or dft64)(truth, W_, alpha_px * u, alpha_px * v)
     elif vis_model == "analytic":
         xc = (W_ // 2) * pixf     # image centre in the DFT frame
         V = np.zeros(u.shape, np.complex128)
@@ -360,6 +402,10 @@ def make_problem(cfg="cfg1", seed=1, W=None, pix=None, ntel=None, nch=None, nsna
     v2 = np.abs(V) ** 2
     legs = np.where(sign < 0, np.conj(V[uvpnt]), V[uvpnt])
     T3 = legs.prod(axis=1)
+    if c.get("t3_min", 0.0) > 0.0:
+        keep = np.abs(T3) >= c["t3_min"]
+        uvpnt, sign, T3 = uvpnt[keep], sign[keep], T3[keep]
+        nbis = uvpnt.shape[0]
     sig_v2 = np.maximum(0.02 * v2, 1e-4)
     sig_amp = np.maximum(0.05 * np.abs(T3), 1e-30)
     sig_phi = np.full(nbis, 1.0)
@@ -399,6 +445,7 @@ def make_problem(cfg="cfg1", seed=1, W=None, pix=None, ntel=None, nch=None, nsna
     start = np.maximum(model, np.float32(1e-8))
     meta = dict(cfg=cfg, seed=seed, nbase=nbase, ntri=ntri, sky=c["sky"], W=W_, pix=c["pix"],
                 noise="sigV2=max(0.02*V2,1e-4); sigamp=0.05*|T3|; sigphi=1deg",
+                t3_min=c.get("t3_min", 0.0),
                 uvmax=c["uvmax"], ntel=c["ntel"], nch=c["nch"], nsnap=c["nsnap"], vis_model=vis_model,
                 prior_flux=prior_flux, components=comps, alpha=c["alpha"], prior=tuple(prior))
     return Problem(W=W_, pixellation=c["pix"], u=uv[:, 0].copy(), v=uv[:, 1].copy(), npow=npow,

@@ -220,6 +220,17 @@ int gpair_pcg_run(gpair_ctx *ctx, const gpair_pcg_params *params, gpair_pcg_resu
 /* per-pixel preconditioner factor [W*W] multiplying P (NULL clears it) */
 int gpair_set_pfactor(gpair_ctx *ctx, const float *pfactor, int size);
 
+/* Autotuner (no reference counterpart: the reference fixes its work-group sizes
+ * when it builds the kernels, gpair1/trunk/gpu.c:413-498).  Times candidate
+ * launch shapes of the size-dependent kernels on the context's data and current
+ * image -- the grids of the batched line-search reductions and of the adjoint
+ * coefficients, the uv split of the FP32 and BF16x3 adjoints -- and keeps the
+ * fastest for the context (reps launches per candidate, <= 0: 5).  chosen, if not
+ * NULL, receives up to 5 values: line-search chi2 grid, line-search entropy grid,
+ * coefficient grid, FP32 adjoint workgroups, BF16x3 adjoint workgroups.  Call it
+ * after gpu_build_kernels; a new gpu_build_kernels restores the default split. */
+int gpair_autotune(gpair_ctx *ctx, int reps, int *chosen, int nchosen);
+
 /* ---- host-side data preparation (no device needed) ----------------------- */
 /* set_model, gpair1/trunk/cpu.c:645-749: prior image [W*W] of total flux `flux`.
  * type 0 flat, 1 centred Dirac, 2 uniform disk (radius width mas), 3 Gaussian

@@ -1,0 +1,121 @@
+"""Network kernels (include/gpair_nn.h) against the FP64 network oracle (MI355X only).
+
+Tolerances (FP32 kernels against FP64): layer outputs and data gradients
+max|d| <= 1e-5 max|ref|; weight gradients (sums over the batch and pixels)
+max|d| <= 1e-5 max|ref|; whole U-Net output and every parameter gradient
+max|d| <= 1e-4 max|ref|.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import nn_oracle as NN
+from gpair_amd import _lib
+from gpair_amd.learned import DeviceArray, UNet
+
+pytestmark = pytest.mark.gpu
+
+
+def _err(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+@pytest.mark.parametrize("N,Ci,Co,H,W,pad_in,pad_out", [(2, 3, 10, 16, 16, 0, 0), (1, 5, 8, 12, 20, 2, 1),
+                                                        (3, 17, 3, 8, 8, 1, 3)])
+def test_conv3_forward_and_backward(N, Ci, Co, H, W, pad_in, pad_out):
+    """batch strides wider than C*H*W (a tensor inside a concatenation)."""
+    print(f"conv N={N} Ci={Ci} Co={Co} H={H} W={W}", flush=True)
+    L = _lib.load()
+    r = np.random.default_rng(Ci * 100 + Co)
+    HW = H * W
+    in_bs, out_bs = (Ci + pad_in) * HW, (Co + pad_out) * HW
+    x = r.standard_normal((N, Ci, H, W))
+    w = r.standard_normal((Co, Ci, 3, 3)) * 0.5
+    b = r.standard_normal(Co)
+    dy = r.standard_normal((N, Co, H, W))
+    xb = np.zeros((N, Ci + pad_in, H, W), np.float32)
+    xb[:, pad_in:] = x
+    dyb = np.zeros((N, Co + pad_out, H, W), np.float32)
+    dyb[:, :Co] = dy
+    dw_, db_ = DeviceArray(Co * Ci * 9), DeviceArray(Co)
+    d_x, d_w, d_b = DeviceArray(N * in_bs, xb), DeviceArray(w.size, w), DeviceArray(Co, b)
+    d_y, d_dy, d_dx = DeviceArray(N * out_bs), DeviceArray(N * out_bs, dyb), DeviceArray(N * in_bs)
+    for relu in (0, 1):
+        rc = L.gpair_nn_conv3_fwd(d_x.at(pad_in * HW), in_bs, Ci, d_w.at(0), d_b.at(0), d_y.at(0), out_bs, Co, N, H,
+                                  W, relu)
+        assert rc == 0
+        y = d_y.download().reshape(N, Co + pad_out, H, W)[:, :Co]
+        ref = NN.conv3_fwd(x, w, b)
+        if relu:
+            ref = np.maximum(ref, 0)
+        assert _err(y, ref) <= 1e-5
+    rc = L.gpair_nn_conv3_bwd(d_x.at(pad_in * HW), in_bs, Ci, d_w.at(0), d_dy.at(0), out_bs, Co, N, H, W,
+                              d_dx.at(pad_in * HW), in_bs, dw_.at(0), db_.at(0))
+    assert rc == 0
+    rdx, rdw, rdb = NN.conv3_bwd(x, w, dy)
+    gdx = d_dx.download().reshape(N, Ci + pad_in, H, W)[:, pad_in:]
+    e = (_err(gdx, rdx), _err(dw_.download().reshape(rdw.shape), rdw), _err(db_.download(), rdb))
+    print(f"  dx {e[0]:.2e} dw {e[1]:.2e} db {e[2]:.2e}", flush=True)
+    assert max(e) <= 1e-5
+    # invalid shapes are rejected before any launch
+    assert L.gpair_nn_conv3_fwd(d_x.at(0), 1, Ci, d_w.at(0), d_b.at(0), d_y.at(0), out_bs, Co, N, H, W, 0) != 0
+
+
+def test_pool_and_upsample():
+    L = _lib.load()
+    r = np.random.default_rng(7)
+    N, Cc, H, W = 2, 3, 8, 12
+    x = r.standard_normal((N, Cc, H, W))
+    a, o = DeviceArray(x.size, x), DeviceArray(x.size)
+    assert L.gpair_nn_pool2(a.at(0), Cc * H * W, o.at(0), Cc * H * W // 4, N, Cc, H, W, 0) == 0
+    assert _err(o.download(x.size // 4).reshape(N, Cc, H // 2, W // 2), NN.pool2(x)) <= 1e-6
+    assert L.gpair_nn_pool2(o.at(0), Cc * H * W // 4, a.at(0), Cc * H * W, N, Cc, H, W, 1) == 0
+    assert _err(a.download().reshape(x.shape), NN.pool2_bwd(NN.pool2(x).astype(np.float32))) <= 1e-6
+    u, v = DeviceArray(x.size, x), DeviceArray(4 * x.size)
+    assert L.gpair_nn_up2(u.at(0), Cc * H * W, v.at(0), 4 * Cc * H * W, N, Cc, H, W, 0) == 0
+    assert _err(v.download().reshape(N, Cc, 2 * H, 2 * W), NN.up2(x)) <= 1e-6
+    assert L.gpair_nn_up2(v.at(0), 4 * Cc * H * W, u.at(0), Cc * H * W, N, Cc, H, W, 1) == 0
+    assert _err(u.download().reshape(x.shape), NN.up2_bwd(NN.up2(x))) <= 1e-6
+
+
+@pytest.mark.parametrize("W,base,levels,N", [(32, 4, 3, 2), (64, 16, 4, 2)])
+def test_unet_forward_backward_against_oracle(W, base, levels, N):
+    print(f"unet W={W} base={base} levels={levels} N={N}", flush=True)
+    net = UNet(W, base=base, levels=levels, max_batch=N, seed=3)
+    r = np.random.default_rng(11)
+    flat = net.params.astype(np.float64)
+    x = r.standard_normal((N, 2, W, W)).astype(np.float32)
+    y = net.forward(x)
+    ref, cache = NN.unet_forward(x.astype(np.float64), flat, 2, 2, base, levels)
+    ey = _err(y, ref)
+    dy = r.standard_normal(ref.shape).astype(np.float32)
+    net.backward(dy)
+    g = net.grads()
+    _, rg = NN.unet_backward(dy.astype(np.float64), flat, cache, 2, 2, base, levels)
+    eg = _err(g, rg)
+    # per layer, so a wrong layer cannot hide behind a large one
+    worst, o = 0.0, 0
+    for ci, co in NN.unet_layers(2, 2, base, levels):
+        k = co * ci * 9 + co
+        worst = max(worst, _err(g[o:o + k], rg[o:o + k]))
+        o += k
+    print(f"  y {ey:.2e} grads {eg:.2e} worst layer {worst:.2e} ({net.nparam} params)", flush=True)
+    assert ey <= 1e-4 and eg <= 1e-4 and worst <= 1e-4
+
+
+def test_training_is_deterministic_and_reduces_the_loss():
+    W, N = 32, 4
+    r = np.random.default_rng(5)
+    x = r.standard_normal((N, 2, W, W)).astype(np.float32)
+    t = np.tanh(x[:, ::-1] * 0.5).astype(np.float32)    # a learnable local map
+    xd, td = DeviceArray(x.size, x), DeviceArray(t.size, t)
+    runs = []
+    for rep in range(2):
+        net = UNet(W, base=8, levels=3, max_batch=N, seed=9)
+        losses = [net.train_step(xd.at(0), td.at(0), N, 2e-3) for _ in range(60)]
+        runs.append((losses, net.params))
+        net.close()
+    print(f"loss {runs[0][0][0]:.4f} -> {runs[0][0][-1]:.4f}", flush=True)
+    assert runs[0][0][-1] < 0.3 * runs[0][0][0]
+    assert np.array_equal(runs[0][1], runs[1][1])

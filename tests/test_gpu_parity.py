@@ -220,3 +220,37 @@ def test_bf16x3_forward(name, kw, orc):
     assert ev <= 3e-5
     assert ec <= 2e-5
     assert eg <= 2e-4   # widths that are not a multiple of 128 use the FP32 adjoint
+
+
+def test_autotuned_shapes_keep_parity(orc):
+    """after gpair_autotune (launch shapes only) the objective, both adjoints and the
+    accelerated loop still meet the bounds above."""
+    prob = synth.make_problem("cfg1", seed=2, W=128, ntel=7, nch=6, nsnap=5)
+    P = orc.unpack(prob)
+    img = _image(prob, 7)
+    J, chi2_o, S_o, vis_o, mock_o, F = orc.objective(P, img.astype(np.float64), ALPHA)
+    gd_o = orc.gradient(P, img.astype(np.float64), ALPHA, vis_o, mock_o)[1]
+    with GpairContext.from_problem(prob, image=img) as ctx:
+        chosen = ctx.autotune(reps=3)
+        print("autotune", chosen, flush=True)
+        assert all(v > 0 for v in chosen)
+        for prec, vb in ((0, 2e-6), (1, 3e-5)):
+            ctx.set_precision(prec)
+            ctx.gpu_copy_image(img)
+            chi2 = ctx.gpu_get_chi2_curr()
+            vis = ctx.vis().astype(np.complex128)
+            ctx.gpu_compute_data_gradient_curr()
+            gd = ctx.get("dg").astype(np.float64)
+            ev = np.abs(vis - vis_o).max() / np.abs(vis_o).max()
+            eg = np.abs(gd - gd_o).max() / np.abs(gd_o).max()
+            print(f"  prec {prec}: vis {ev:.2e} chi2 {abs(chi2 - chi2_o) / chi2_o:.2e} grad {eg:.2e}", flush=True)
+            assert ev <= vb and abs(chi2 - chi2_o) / chi2_o <= 2e-5 and eg <= 2e-4
+    prob = synth.make_problem("cfg1", seed=1)
+    P = orc.unpack(prob)
+    alpha = prob.meta["alpha"]
+    with GpairContext.from_problem(prob) as ctx:
+        ctx.autotune(reps=3)
+        res, tr = ctx.pcg_run(alpha, precond=1, max_iter=12, chi2r_target=0.0)
+    img_o, tr_o = orc.pcg(P, prob.start.astype(np.float64), alpha, precond=1, max_iter=12, chi2r_target=0.0)
+    assert np.abs(tr[:, 0] - tr_o[:, 0]).max() / np.abs(tr_o[:, 0]).min() <= 1e-3
+    assert np.array_equal(tr[:, 5:8], tr_o[:, 5:8])

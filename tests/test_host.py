@@ -87,3 +87,85 @@ def test_bench_refuses_multi_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stdout
+
+
+GOLDEN_DIR = os.path.join(ROOT, "tests", "golden")
+
+
+@pytest.mark.parametrize("name", ["ref_cpu_w64.npz", "ref_cpu_w64_7tel.npz"])
+def test_set_model_matches_reference_priors(name):
+    """gpair_set_model types 0-4 against the reference's set_model output
+    (gpair1/trunk/cpu.c:645-749, width 1 mas, flux 1, the reference's own offsets)."""
+    z = np.load(os.path.join(GOLDEN_DIR, name), allow_pickle=False)
+    W, pix = int(z["W"]), float(z["pix"])
+    for t in range(5):
+        got = synth.set_model(W, pix, t, 1.0, 1.0, centered=False)
+        ref = z["ref_priors"][t]
+        assert np.array_equal(got, ref), (t, np.abs(got - ref).max())
+    with pytest.raises(ValueError):
+        synth.set_model(W, pix, 7, 1.0, 1.0)
+
+
+def _init_data(pow_, powerr, amp, amperr, phs, phserr, uvpnt, extrap=1):
+    L = _lib.load()
+    npow, nbis = pow_.size, amp.size
+    with np.errstate(over="ignore"):        # 1e99 is +inf in float, as in the reference
+        amp, amperr = amp.astype(np.float32).copy(), amperr.astype(np.float32).copy()
+    data = np.zeros(npow + 2 * nbis, np.float32)
+    err = np.zeros_like(data)
+    ph = np.zeros(2 * nbis, np.float32)
+    nd = C.c_int()
+    f32 = lambda a: np.ascontiguousarray(a, np.float32)
+    args = [f32(pow_), f32(powerr), amp, amperr, f32(phs), f32(phserr), np.ascontiguousarray(uvpnt, np.int64)]
+    rc = L.gpair_init_data(npow, nbis, *[a.ctypes.data for a in args], extrap, data.ctypes.data, err.ctypes.data,
+                           ph.ctypes.data, C.byref(nd))
+    assert rc == 0
+    return data, err, ph, nd.value, amp, amperr
+
+
+def test_init_data_closed_forms():
+    """gpair_init_data (gpair1/trunk/gpair.c:997-1066): packing, and the T3
+    amplitude extrapolation from the three V2 points checked through the
+    quadratic its estimate solves and its small-error limit |V_ab||V_bc||V_ca|."""
+    r = np.random.default_rng(1)
+    npow, nbis = 12, 5
+    pow_ = r.uniform(0.05, 0.9, npow)
+    powerr = 0.02 * pow_
+    amp = r.uniform(0.01, 0.5, nbis)
+    amperr = 0.05 * amp
+    phs = r.uniform(-180, 180, nbis)
+    phserr = np.full(nbis, 1.0)
+    uvpnt = r.integers(0, npow, size=(nbis, 3))
+    amperr[1] = -1.0          # missing amplitude: extrapolated
+    amperr[3] = 1e99          # NaN amplitudes arrive as 1 +- 1e99
+    uvpnt[4, 2] = npow + 3    # a leg without V2: no amplitude weight, one dof less
+    amperr[4] = 0.0
+    data, err, ph, nd, a2, e2 = _init_data(pow_, powerr, amp, amperr, phs, phserr, uvpnt)
+    p32, e32 = pow_.astype(np.float32), powerr.astype(np.float32)
+    assert np.array_equal(data[:npow], p32) and np.allclose(err[:npow], 1 / e32, rtol=1e-6)
+    assert np.all(data[npow + 1::2] == 0)
+    assert np.allclose(ph[0::2] + 1j * ph[1::2], np.exp(-1j * phs.astype(np.float32) * np.pi / 180), atol=1e-6)
+    assert nd == npow + 2 * nbis - 1
+    for t in (0, 2):     # measured amplitudes pass through
+        assert data[npow + 2 * t] == np.float32(amp[t]) and np.isclose(err[npow + 2 * t], 1 / amperr[t], rtol=1e-6)
+    for t in (1, 3):
+        s = []
+        for k in uvpnt[t]:
+            p, e = float(p32[k]), float(e32[k])
+            st = (p + np.sqrt(p * p + 2 * e * e)) / 2
+            assert abs(st * st - p * st - e * e / 2) <= 1e-6 * st * st       # the quadratic
+            s.append(st)
+        assert np.isclose(a2[t], np.sqrt(np.prod(s)), rtol=1e-5)
+        assert data[npow + 2 * t] == a2[t] and e2[t] > 0
+    assert a2[4] == 1.0 and err[npow + 8] == 0.0
+
+
+def test_init_data_small_error_limit():
+    """small V2 errors: the extrapolated amplitude tends to |V_ab||V_bc||V_ca|"""
+    pow_ = np.array([0.25, 0.36, 0.49, 0.64])
+    uvpnt = np.array([[0, 1, 2], [1, 2, 3]])
+    data, err, ph, nd, a2, e2 = _init_data(pow_, 1e-6 * pow_, np.ones(2), -np.ones(2), np.zeros(2), np.ones(2),
+                                           uvpnt)
+    assert np.allclose(a2, [0.5 * 0.6 * 0.7, 0.6 * 0.7 * 0.8], rtol=1e-5)
+    # relative error of the product of three independent 1e-6 estimates ~ sqrt(3) * 1e-6 / 2
+    assert np.allclose(e2 / a2, np.sqrt(3) * 0.5e-6, rtol=0.05)

@@ -303,3 +303,48 @@ def _replay(orc, P, img, alpha, nit):
         img = np.maximum(img + sel * d, minv)
         g_old = g
     return np.array(Js)
+
+
+def test_parabola_step_closed_form(orc):
+    """orc_parabola_step: the vertex of the parabola through three points."""
+    a, b, c = 2.0, -3.0, 0.5          # J(s) = a (s - 0.75)^2 + c, vertex 0.75
+    f = lambda s: a * (s - 0.75) ** 2 + c
+    for s in ((0.1, 0.5, 1.3), (0.5, 0.7, 2.0), (0.0, 1.0, 4.0)):
+        v = orc.L.orc_parabola_step(s[0], s[1], s[2], f(s[0]), f(s[1]), f(s[2]))
+        assert abs(v - 0.75) <= 1e-12
+    del b
+
+
+def test_dft_raw_is_the_unnormalised_dft(orc):
+    """orc_dft_raw = F * V: point source of flux 3 at (x0, y0) gives 3 X[p,x0] Y[p,y0]."""
+    prob = _small(9)
+    W = prob.W
+    img = np.zeros(W * W)
+    x0, y0 = 5, 11
+    img[x0 + W * y0] = 3.0
+    U = orc.dft_raw(W, prob.au(), prob.av(), img)
+    ref = 3.0 * np.exp(2j * np.pi * (prob.au() * x0 - prob.av() * y0))
+    assert np.abs(U - ref).max() <= 1e-12
+
+
+def test_pcg_oracle_descends_and_tracks_the_objective(orc):
+    """orc_pcg (the accelerated rules): its FP64 objective never increases, the
+    objective it tracks by exact per-datum / per-pixel differences equals the
+    objective recomputed from the final image, and the batch counters add up."""
+    prob = _small(10)
+    P = orc.unpack(prob)
+    alpha = prob.meta["alpha"]
+    img, tr = orc.pcg(P, prob.start.astype(np.float64), alpha, precond=1, max_iter=15, chi2r_target=0.0)
+    assert len(tr) >= 10
+    J = tr[:, 0]
+    assert np.all(np.diff(J) <= 1e-12 * np.abs(J[:-1]))
+    assert np.all(np.diff(tr[:, 5]) >= 1) and np.all(np.diff(tr[:, 6]) == 1)   # DFT counters
+    # the objective tracked at the start of iteration 6 equals the objective of the
+    # image after 6 iterations, recomputed from scratch
+    img6, _ = orc.pcg(P, prob.start.astype(np.float64), alpha, precond=1, max_iter=6, chi2r_target=0.0)
+    J6 = orc.objective(P, img6, alpha)[0]
+    assert abs(tr[6, 0] - J6) <= 1e-10 * abs(J6)
+    assert abs(orc.objective(P, img, alpha)[0] - J[-1]) > 0        # the final commit moved it
+    # chi2 and entropy columns match the objective's parts at the first row
+    J0, chi20, S0, *_ = orc.objective(P, prob.start.astype(np.float64), alpha)
+    assert abs(tr[0, 1] - chi20) <= 1e-12 * chi20 and abs(tr[0, 2] - S0) <= 1e-9 * max(abs(S0), 1.0)
