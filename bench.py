@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--no-target", action="store_true", help="skip the runs to chi2_r = 1")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-start run")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch shapes")
     ap.add_argument("--cpu-kind", default="auto", choices=["auto", "reference", "port"])
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 256 MiB device copy (HBM counter calibration for rocprofv3 --pmc)")
@@ -175,6 +176,11 @@ def main():
     ctx = GpairContext.from_problem(prob)
     ctx.set_precision(prec)
     L = ctx.L
+    tuned = None if a.no_autotune else ctx.autotune()
+
+    def tune(c):
+        if not a.no_autotune:
+            c.autotune()
     if a.warmup > 0:
         ctx.pcg_run(alpha, precond=1, max_iter=a.warmup, chi2r_target=0.0)
     L.gpair_sync(ctx.h)
@@ -221,6 +227,8 @@ def main():
                      "flop_per_launch": flop},
         "adjoint": {"kernel": aname, "avg_us": adj_ms / max(adj_n, 1) * 1e3, "launches": adj_n,
                     "achieved_tflops": flop / (adj_ms / max(adj_n, 1) * 1e-3) / 1e12 if adj_n else None},
+        "autotune": None if tuned is None else dict(zip(
+            ("ls_chi2_blocks", "ls_entropy_blocks", "coef_blocks", "adj_fp32_groups", "adj_bf16x3_groups"), tuned)),
         "dfts_per_iter": {"forward": (res.forward_dfts) / max(steps_done, 1),
                           "adjoint": res.adjoint_dfts / max(steps_done, 1)},
     }
@@ -233,6 +241,7 @@ def main():
         for name, p in (("accelerated_fp32", 0), ("accelerated_bf16x3", 1)):
             c2 = GpairContext.from_problem(prob)
             c2.set_precision(p)
+            tune(c2)
             t0 = time.perf_counter()
             r2, _ = c2.pcg_run(alpha, precond=1, max_iter=a.max_iter, chi2r_target=1.0)
             c2.L.gpair_sync(c2.h)
@@ -251,6 +260,7 @@ def main():
             net = learned.UNet.load(learned.WEIGHTS)
             c2 = GpairContext.from_problem(prob)
             c2.set_precision(1)
+            tune(c2)
             t0 = time.perf_counter()
             start, pf = learned.predict(net, c2, prob)
             t_inf = time.perf_counter() - t0

@@ -56,7 +56,7 @@ struct gpair_ctx {
     int adj_b3_nsplit, adj_b3_pper;
     // accelerated loop
     gpair_buf_s z, zold, delta;
-    float2 *Ud, *vis2;
+    float2 *Ud, *Udel, *vis2;   // DFTs of the direction and of a clamped change
     double2 *Vd, *Vn;
     float *mock2;
     float *pfac;
@@ -212,7 +212,7 @@ extern "C" int gpu_cleanup(gpair_ctx *c)
     dfree(&c->data); dfree(&c->err); dfree(&c->phasor); dfree(&c->tri); dfree(&c->csr_ptr); dfree(&c->csr_ent);
     dfree(&c->X); dfree(&c->Y); dfree(&c->vis); dfree(&c->coef); dfree(&c->mock);
     dfree(&c->scal); dfree(&c->invflux); dfree(&c->part); dfree(&c->ticket); dfree(&c->adj_part);
-    dfree(&c->Ud); dfree(&c->pfac); dfree(&c->ls_out);
+    dfree(&c->Ud); dfree(&c->Udel); dfree(&c->pfac); dfree(&c->ls_out);
     dfree(&c->vis2); dfree(&c->Vd); dfree(&c->Vn); dfree(&c->mock2);
     dfree(&c->Xh); dfree(&c->Xl); dfree(&c->Ih); dfree(&c->Il); dfree(&c->Xth); dfree(&c->Xtl);
     if (c->host_scal)
@@ -325,7 +325,7 @@ extern "C" int gpu_copy_data(gpair_ctx *c, const float *data, const float *data_
         (rc = dalloc(c, &c->phasor, nbis)) || (rc = dalloc(c, &c->tri, nbis)) ||
         (rc = dalloc(c, &c->csr_ptr, nuv + 1)) || (rc = dalloc(c, &c->csr_ent, 3 * (size_t)nbis)) ||
         (rc = dalloc(c, &c->vis, nuv)) || (rc = dalloc(c, &c->coef, nuv)) || (rc = dalloc(c, &c->Ud, nuv)) ||
-        (rc = dalloc(c, &c->vis2, nuv)) || (rc = dalloc(c, &c->Vd, nuv)) || (rc = dalloc(c, &c->Vn, nuv)) ||
+        (rc = dalloc(c, &c->Udel, nuv)) || (rc = dalloc(c, &c->vis2, nuv)) || (rc = dalloc(c, &c->Vd, nuv)) || (rc = dalloc(c, &c->Vn, nuv)) ||
         (rc = dalloc(c, &c->mock2, ndof)))
         return rc;
     if ((rc = alloc_images(c, image_width)))

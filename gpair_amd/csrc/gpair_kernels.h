@@ -26,6 +26,7 @@ enum {
     SC_FDELTA = 10,  // sum of the committed image change, entropy change, chi2 change
     SC_DSENT = 11,
     SC_DCHI2 = 12,
+    SC_BETA = 13,    // beta the direction kernel used
     GPAIR_NSCAL = 16
 };
 
@@ -36,6 +37,7 @@ enum {
 // batched line search: LS_K candidate steps, s_0 = 0 and s_k = s_ref * 2^((k - LS_TOP) / 2)
 #define LS_K 32
 #define LS_TOP 26
+#define LS_G 8   // candidates per block row of the line-search kernels (grid.y = LS_K / LS_G)
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef float v16f __attribute__((ext_vector_type(16)));

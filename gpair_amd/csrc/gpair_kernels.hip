@@ -580,12 +580,23 @@ __global__ __launch_bounds__(RED_THREADS) void k_pcg_precond(const float *__rest
 }
 
 // d = -z + beta d, projected to zero where the pixel sits on the floor and d < 0;
-// out[0] = d.g, out[1] = sum d, out[2] = sum |d|
+// out[0] = d.g, out[1] = sum d, out[2] = sum |d|.  conj != 0: Polak-Ribiere+
+// beta = max(0, (z.g - z.g_old) / zg_old) from the z.g slots the preconditioner
+// kernel left in scal (kept in scal[SC_BETA]); conj == 0: beta = 0 (-z)
 __global__ __launch_bounds__(RED_THREADS) void k_pcg_dir(const float *__restrict__ img, const float *__restrict__ z,
-                                                         const float *__restrict__ g, float beta, float minval, int n,
-                                                         float *__restrict__ d, double *part, unsigned *ticket,
-                                                         double *out)
+                                                         const float *__restrict__ g, int conj, double zg_old,
+                                                         float minval, int n, float *__restrict__ d, double *scal,
+                                                         double *part, unsigned *ticket, double *out)
 {
+    double b = 0.0;
+    if (conj) {
+        b = (scal[SC_ZG] - scal[SC_ZGO]) / zg_old;
+        if (!(b > 0.0))
+            b = 0.0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        scal[SC_BETA] = b;
+    const float beta = (float)b;
     double v[3] = {0.0, 0.0, 0.0};
     for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) {
         float di = -z[i] + beta * d[i];
@@ -640,21 +651,27 @@ __global__ __launch_bounds__(RED_THREADS) void k_ls_dchi2(const double2 *__restr
                                                           const int4 *__restrict__ tri, int npow, int nbis,
                                                           double *part, unsigned *ticket, double *out)
 {
-    double v[LS_K];
-#pragma unroll
-    for (int k = 0; k < LS_K; k++)
-        v[k] = 0.0;
+    // candidates g0 .. g0 + LS_G - 1 in this block row: LS_K / LS_G rows of blocks,
+    // each with its own partials, ticket and outputs (fewer live registers per lane)
+    const int grp = blockIdx.y, g0 = grp * LS_G;
     const double FI = scal[SC_FLUX], Fd = scal[SC_FD];
+    double sk[LS_G], ivk[LS_G], v[LS_G];
+#pragma unroll
+    for (int k = 0; k < LS_G; k++) {
+        sk[k] = ls_step(s_ref, g0 + k);
+        ivk[k] = 1.0 / (FI + sk[k] * Fd);
+        v[k] = 0.0;
+    }
     const int total = npow + nbis;
     for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < total; i += gridDim.x * RED_THREADS) {
         if (i < npow) {
             const double2 a = Vd[i];
             const double bx = Ud[i].x, by = Ud[i].y, d0 = data[i], e = err[i];
             const double r0 = (a.x * a.x + a.y * a.y - d0) * e;
+            const double fx = FI * a.x, fy = FI * a.y;
 #pragma unroll
-            for (int k = 0; k < LS_K; k++) {
-                const double s = ls_step(s_ref, k), iv = 1.0 / (FI + s * Fd);
-                const double x = (FI * a.x + s * bx) * iv, y = (FI * a.y + s * by) * iv;
+            for (int k = 0; k < LS_G; k++) {
+                const double x = (fx + sk[k] * bx) * ivk[k], y = (fy + sk[k] * by) * ivk[k];
                 const double r = (x * x + y * y - d0) * e;
                 v[k] += r * r - r0 * r0;
             }
@@ -666,28 +683,33 @@ __global__ __launch_bounds__(RED_THREADS) void k_ls_dchi2(const double2 *__restr
 #pragma unroll
             for (int j = 0; j < 3; j++) {
                 a[j] = Vd[idx[j]];
+                a[j].x *= FI;
+                a[j].y *= FI;
                 b[j] = make_double2(Ud[idx[j]].x, Ud[idx[j]].y);
             }
             const double2 rho = make_double2(phasor[t].x, phasor[t].y);
             const int kk = npow + 2 * t;
             const double dr = data[kk], di = data[kk + 1], er = err[kk], ei = err[kk + 1];
+            const double iF = 1.0 / FI;
             double2 T0 = dcmul(dcmul(dcmul(dleg(a[0], tr.w & 1), dleg(a[1], tr.w & 2)), dleg(a[2], tr.w & 4)), rho);
+            T0.x *= iF * iF * iF;
+            T0.y *= iF * iF * iF;
             const double q0 = ((T0.x - dr) * er) * ((T0.x - dr) * er) + ((T0.y - di) * ei) * ((T0.y - di) * ei);
 #pragma unroll
-            for (int k = 0; k < LS_K; k++) {
-                const double s = ls_step(s_ref, k), iv = 1.0 / (FI + s * Fd);
+            for (int k = 0; k < LS_G; k++) {
                 double2 l[3];
 #pragma unroll
                 for (int j = 0; j < 3; j++)
-                    l[j] = make_double2((FI * a[j].x + s * b[j].x) * iv, (FI * a[j].y + s * b[j].y) * iv);
-                const double2 T =
+                    l[j] = make_double2(a[j].x + sk[k] * b[j].x, a[j].y + sk[k] * b[j].y);
+                double2 T =
                     dcmul(dcmul(dcmul(dleg(l[0], tr.w & 1), dleg(l[1], tr.w & 2)), dleg(l[2], tr.w & 4)), rho);
-                const double r0 = (T.x - dr) * er, r1 = (T.y - di) * ei;
+                const double iv3 = ivk[k] * ivk[k] * ivk[k];
+                const double r0 = (T.x * iv3 - dr) * er, r1 = (T.y * iv3 - di) * ei;
                 v[k] += r0 * r0 + r1 * r1 - q0;
             }
         }
     }
-    reduce_finish<LS_K>(v, part, ticket, out, nullptr);
+    reduce_finish<LS_G>(v, part + (size_t)grp * gridDim.x * LS_G, ticket + grp, out + g0, nullptr);
 }
 
 // change of the entropy along the line: out[k] = sum S(max(I + s_k d, minval)) - S(I)
@@ -697,11 +719,13 @@ __global__ __launch_bounds__(RED_THREADS) void k_ls_dentropy(const float *__rest
                                                              double minval, int n, double *part, unsigned *ticket,
                                                              double *out)
 {
-    double v[LS_K], sk[LS_K];
+    const int grp = blockIdx.y, g0 = grp * LS_G;
+    float sk[LS_G];
+    double v[LS_G];
 #pragma unroll
-    for (int k = 0; k < LS_K; k++) {
+    for (int k = 0; k < LS_G; k++) {
+        sk[k] = (float)ls_step(s_ref, g0 + k);
         v[k] = 0.0;
-        sk[k] = ls_step(s_ref, k);
     }
     for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) {
         const float I = img[i], dd = d[i], M = model[i];
@@ -711,8 +735,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_ls_dentropy(const float *__rest
         // proportional to the change, so FP32 resolves it relative to the change itself
         const float L = logf(I / M), invI = 1.0f / I;
 #pragma unroll
-        for (int k = 0; k < LS_K; k++) {
-            float t = I + (float)sk[k] * dd;
+        for (int k = 0; k < LS_G; k++) {
+            float t = I + sk[k] * dd;
             if (t < (float)minval)
                 t = (float)minval;
             const float dl = t - I, q = dl * invI;
@@ -722,7 +746,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_ls_dentropy(const float *__rest
             v[k] += (double)(dl * (1.0f - L) - t * lr);
         }
     }
-    reduce_finish<LS_K>(v, part, ticket, out, nullptr);
+    reduce_finish<LS_G>(v, part + (size_t)grp * gridDim.x * LS_G, ticket + grp, out + g0, nullptr);
 }
 
 // trial image I_t = max(I + s d, minval), the change delta = I_t - I, and
@@ -944,23 +968,32 @@ __global__ __launch_bounds__(256) void k_vis_fwd_b3(const unsigned short *__rest
         __syncthreads();   // the previous group's readers are done with sB
         b3_store<PER, CH>(h0, h1, h2, h3, l0, l1, l2, l3, sB[0][0], sB[0][1], tid);
         __syncthreads();
-        uint4 ah_n = make_uint4(0, 0, 0, 0), al_n = ah_n;
+        // A operands (this lane's X row) one whole chunk ahead: the table streams from
+        // HBM, and with one wave per SIMD only the wave's own MFMA work hides the latency
+        static_assert(B3_KC == 32, "two k-steps per chunk");
+        uint4 a0h = make_uint4(0, 0, 0, 0), a0l = a0h, a1h = a0h, a1l = a0h;
         if (aval) {
-            ah_n = *reinterpret_cast<const uint4 *>(Xh + arow + 8 * lh);
-            al_n = *reinterpret_cast<const uint4 *>(Xl + arow + 8 * lh);
+            a0h = *reinterpret_cast<const uint4 *>(Xh + arow + 8 * lh);
+            a0l = *reinterpret_cast<const uint4 *>(Xl + arow + 8 * lh);
+            a1h = *reinterpret_cast<const uint4 *>(Xh + arow + 16 + 8 * lh);
+            a1l = *reinterpret_cast<const uint4 *>(Xl + arow + 16 + 8 * lh);
         }
         for (int c = 0; c < nchunk; c++) {
             const int buf = c & 1;
-            if (c + 1 < nchunk)
+            uint4 n0h = a0h, n0l = a0l, n1h = a1h, n1l = a1l;
+            if (c + 1 < nchunk) {
+                if (aval) {
+                    const int kn = (c + 1) * B3_KC;
+                    n0h = *reinterpret_cast<const uint4 *>(Xh + arow + kn + 8 * lh);
+                    n0l = *reinterpret_cast<const uint4 *>(Xl + arow + kn + 8 * lh);
+                    n1h = *reinterpret_cast<const uint4 *>(Xh + arow + kn + 16 + 8 * lh);
+                    n1l = *reinterpret_cast<const uint4 *>(Xl + arow + kn + 16 + 8 * lh);
+                }
                 b3_fetch<PER, CH>(h0, h1, h2, h3, l0, l1, l2, l3, Ih, Il, tid, y0, W, c + 1);
+            }
 #pragma unroll
             for (int ks = 0; ks < B3_KC / 16; ks++) {
-                const uint4 ah_c = ah_n, al_c = al_n;
-                const int kn = c * B3_KC + ks * 16 + 16;   // next k-step's first column
-                if (aval && kn < W) {
-                    ah_n = *reinterpret_cast<const uint4 *>(Xh + arow + kn + 8 * lh);
-                    al_n = *reinterpret_cast<const uint4 *>(Xl + arow + kn + 8 * lh);
-                }
+                const uint4 ah_c = ks ? a1h : a0h, al_c = ks ? a1l : a0l;
                 const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(&ah_c);
                 const bf16x8 al = *reinterpret_cast<const bf16x8 *>(&al_c);
 #pragma unroll
@@ -973,6 +1006,10 @@ __global__ __launch_bounds__(256) void k_vis_fwd_b3(const unsigned short *__rest
                     acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[b], 0, 0, 0);
                 }
             }
+            a0h = n0h;
+            a0l = n0l;
+            a1h = n1h;
+            a1l = n1l;
             if (c + 1 < nchunk) {
                 // the other buffer was last read in chunk c-1, before the barrier below
                 b3_store<PER, CH>(h0, h1, h2, h3, l0, l1, l2, l3, sB[buf ^ 1][0], sB[buf ^ 1][1], tid);

@@ -657,6 +657,11 @@ int orc_pcg(int W, int nuv, int npow, int nbis, const double *au, const double *
                 w1 = dot(n, d, g);
                 if (w1 < 0.0 || beta == 0.0)
                     break;
+                /* the product scores a direction before it reads d.g back: a
+                 * conjugate direction that is not a descent one costs it one DFT
+                 * and one batch (counted here so the counters agree) */
+                nfwd++;
+                nbatch++;
                 beta = 0.0;
             }
             if (!(w1 < 0.0))
