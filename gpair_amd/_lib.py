@@ -39,7 +39,8 @@ class PcgParams(C.Structure):
 
 class PcgResult(C.Structure):
     _fields_ = [("iterations", _i), ("forward_dfts", _i), ("adjoint_dfts", _i), ("ls_batches", _i), ("J", _d),
-                ("chi2", _d), ("entropy", _d), ("chi2r", _d), ("converged", _i), ("stalled", _i), ("status", _i)]
+                ("chi2", _d), ("entropy", _d), ("chi2r", _d), ("converged", _i), ("stalled", _i), ("status", _i),
+                ("resyncs", _i)]
 
 
 # name -> (restype, argtypes)

@@ -48,7 +48,7 @@ PF_CLIP = 4.0             # |log pf| <= 4
 # (e.g. 1,699 instead of 467 iterations), so it is not applied by default
 BLEND = 0.75
 USE_PF = False
-BUDGET = 4000             # iterations granted to a learned start before the fallback
+BUDGET = 6000             # iterations granted to a learned start before the fallback
 
 
 def _p(a):

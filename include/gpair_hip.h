@@ -209,6 +209,7 @@ typedef struct {
     int converged;
     int stalled;           /* no decrease found along the direction */
     int status;
+    int resyncs;           /* exact re-evaluations of the tracked objective (their DFTs are not in forward_dfts) */
 } gpair_pcg_result;
 
 void gpair_pcg_default_params(gpair_pcg_params *p);

@@ -38,8 +38,8 @@ def test_learned_start_against_the_plain_start():
     start is rejected).  Every accepted start meets the acceptance rule; on the
     held-out problems of the training family most starts are accepted and the
     learned path costs no more than the plain one; the headline problem's outcome
-    is printed (its time to chi2/ndof = 1 from the learned start varies with the
-    trajectory, 2,760 to over 4,000 iterations against 5,337 from the prior)."""
+    is printed (its time to chi2/ndof = 1 varies with the trajectory: 2,760 to
+    4,100 iterations from the learned start, 5,200 to 6,000 from the prior)."""
     net = learned.UNet.load(learned.WEIGHTS)
     probs = [learned.family_problem(s) for s in HELD_OUT] + [synth.make_problem("cfg2", seed=1)]
     rows = learned.evaluate(net, None, problems=probs)
