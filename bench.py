@@ -13,7 +13,9 @@ timed region.  A step is one CG iteration.
   loop in FP32 and in BF16x3, from the learned start (network inference
   included, BF16x3), and for the plain loop with the reference's rules
   (gpair_cg_run, FP32 DFT), the latter bounded by --plain-budget-s.
-* roofline: the forward DFT, timed with HIP events on the context stream.
+* roofline: the forward DFT, timed with HIP events on the context stream, against
+  the roofline that binds it: MFMA for FP32 (64 flop per compulsory byte at 256^2,
+  above the FP32 balance of 20), HBM for BF16x3 (below its balance of 104).
 * cpu_baseline: the reference's own CPU model (oracle/_ref) on one core; the
   same leg re-evaluates each final image's objective with the FP64 oracle
   (err_vs_fp64).
@@ -35,6 +37,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "wall-time & CG iters/s to chi2_r=1 at 256², BF16x3/learned vs plain FP32; err vs FP64"
 PEAK = {"fp32": 157.3, "bf16x3": 2500.0 / 3.0}   # TFLOP/s: FP32 MFMA; BF16 MFMA / 3 products
+HBM_PEAK = 8.0e12                                  # B/s
 PROFILES = os.path.join(ROOT, "profiles")
 
 
@@ -201,7 +204,24 @@ def main():
     adj_n, adj_ms = tk["adjoint"]
     avg_fwd = fwd_ms / max(fwd_n, 1) * 1e-3
     achieved = flop / avg_fwd / 1e12
-    kname = "k_vis_fwd_f32<4>" if prec == 0 else "k_vis_fwd_b3<8>"
+    nbk = (tuned[3] if tuned and tuned[3] else min(W, 256) // 32) if prec == 1 else 0
+    kname = f"k_vis_fwd_f32<{W // 64}>" if prec == 0 else f"k_vis_fwd_b3<{nbk}>"
+    # compulsory bytes of one forward: X and Y tables (8 B per (uv, x) entry each: float2,
+    # or bf16 hi/lo of Re and Im), the image (4 B per pixel: float, or bf16 hi + lo), V out
+    nbytes = 16.0 * nuv * W + 4.0 * W * W + 8.0 * nuv
+    intensity = flop / nbytes
+    traffic = read_traffic(f"forward_dft_W{W}_{a.precision}_pmc.txt")
+    if intensity < PEAK[a.precision] * 1e12 / HBM_PEAK:
+        # below the machine balance: the HBM roofline binds (BF16x3)
+        roofline = {"bound": "hbm", "achieved": nbytes / avg_fwd / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                    "frac": nbytes / avg_fwd / HBM_PEAK}
+    else:
+        roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK[a.precision], "unit": "TFLOP/s",
+                    "frac": achieved / PEAK[a.precision]}
+    roofline.update({"traffic": traffic, "kernel": f"{kname} (forward DFT)", "avg_us": avg_fwd * 1e6,
+                     "launches": fwd_n, "flop_per_launch": flop, "bytes_per_launch": nbytes,
+                     "flop_per_byte": intensity, "mfma_tflops": achieved,
+                     "mfma_frac": achieved / PEAK[a.precision]})
     aname = "k_adj_f32" if prec == 0 else "k_adj_b3"
     out = {
         "metric": METRIC,
@@ -220,15 +240,11 @@ def main():
                                f"accelerated loop (preconditioned PR-CG, batched line search), {a.precision} DFT",
                    "W": W, "nuv": nuv, "npow": prob.npow, "nbis": prob.nbis, "alpha": alpha,
                    "prior": list(prob.meta["prior"]), "seed": a.seed, "precision": a.precision},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK[a.precision], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK[a.precision],
-                     "traffic": read_traffic(f"forward_dft_W{W}_{a.precision}_pmc.txt"),
-                     "kernel": f"{kname} (forward DFT)", "avg_us": avg_fwd * 1e6, "launches": fwd_n,
-                     "flop_per_launch": flop},
+        "roofline": roofline,
         "adjoint": {"kernel": aname, "avg_us": adj_ms / max(adj_n, 1) * 1e3, "launches": adj_n,
                     "achieved_tflops": flop / (adj_ms / max(adj_n, 1) * 1e-3) / 1e12 if adj_n else None},
         "autotune": None if tuned is None else dict(zip(
-            ("ls_chi2_blocks", "ls_entropy_blocks", "coef_blocks", "adj_fp32_groups", "adj_bf16x3_groups"), tuned)),
+            ("ls_chi2_blocks", "ls_entropy_blocks", "coef_blocks", "fwd_bf16x3_rows_per_group_32"), tuned)),
         "dfts_per_iter": {"forward": (res.forward_dfts) / max(steps_done, 1),
                           "adjoint": res.adjoint_dfts / max(steps_done, 1)},
     }

@@ -232,10 +232,10 @@ class GpairContext:
 
     def autotune(self, reps=5):
         """times candidate launch shapes on this context's problem; returns the chosen
-        (line-search chi2 grid, line-search entropy grid, coefficient grid, FP32 adjoint
-        workgroups, BF16x3 adjoint workgroups)."""
-        out = np.zeros(5, np.int32)
-        self._check(self.L.gpair_autotune(self.h, int(reps), _ptr(out), 5), "gpair_autotune")
+        (line-search chi2 grid, line-search entropy grid, coefficient grid, BF16x3 forward
+        rows per group / 32)."""
+        out = np.zeros(4, np.int32)
+        self._check(self.L.gpair_autotune(self.h, int(reps), _ptr(out), 4), "gpair_autotune")
         return tuple(int(v) for v in out)
 
     def pcg_run(self, alpha, precond=1, max_iter=1000, chi2r_target=1.0, verbose=False):

@@ -63,6 +63,7 @@ struct gpair_ctx {
     double *ls_out, *host_ls;   // device / pinned host, 2*LS_K
     // launch shapes chosen by gpair_autotune (0: the built-in default)
     int ls_blocks_chi2, ls_blocks_ent, coef_blocks;
+    int fwd_nbk;   // image rows per BF16x3 forward group / 32 (0: min(W, 256) / 32)
     // optional per-launch timing of the DFT kernels (HIP events on the context stream)
     int timing;
     std::vector<hipEvent_t> ev[GPAIR_TK_COUNT];
@@ -648,7 +649,7 @@ static int launch_forward_to(gpair_ctx *c, const float *img, const float *scale,
         k_split_image<<<ew_blocks(c->n), 256, 0, c->stream>>>(img, c->n, c->Ih, c->Il);
         LAUNCHCHK(c);
         const unsigned g3 = (unsigned)((c->nuv + B3_PW - 1) / B3_PW);
-        const int nbk = (c->W < B3_ROWS ? c->W : B3_ROWS) / 32;
+        const int nbk = c->fwd_nbk ? c->fwd_nbk : (c->W < B3_ROWS ? c->W : B3_ROWS) / 32;
         switch (nbk) {
 #define B3_CASE(N)                                                                                          \
     case N:                                                                                                 \

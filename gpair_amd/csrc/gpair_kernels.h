@@ -135,6 +135,63 @@ __device__ inline void reduce_finish(double (&v)[NV], double *part, unsigned *ti
     }
 }
 
+// Launch-shape independent reduction over items [0, n): the range is cut into
+// nparts = min(RED_PARTS, ceil(n / (2 RED_THREADS))) partitions; partition q is
+// summed by one block, thread t taking items q*span + t, + RED_THREADS, ... in
+// order, then block_sum, into part[q*NV + j].  Block b takes partitions b,
+// b + gridDim.x, ...; the last block to arrive sums the partials in a fixed order.
+// The result does not depend on the number of blocks, so a tuned grid changes no
+// bit of it.  item(i, v) adds item i into v[0..NV-1].
+#define RED_PARTS 512
+
+template <int NV, typename F>
+__device__ inline void partitioned_reduce(int n, F item, double *part, unsigned *ticket, double *out)
+{
+    __shared__ double sh[NV][RED_THREADS / 64];
+    __shared__ int is_last;
+    int nparts = (n + 2 * RED_THREADS - 1) / (2 * RED_THREADS);
+    nparts = nparts < 1 ? 1 : (nparts > RED_PARTS ? RED_PARTS : nparts);
+    const int span = (n + nparts - 1) / nparts;
+    for (int q = blockIdx.x; q < nparts; q += gridDim.x) {
+        double v[NV];
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+            v[j] = 0.0;
+        const int beg = q * span, end = beg + span < n ? beg + span : n;
+        for (int i = beg + threadIdx.x; i < end; i += RED_THREADS)
+            item(i, v);
+        block_sum<NV>(v, sh);
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int j = 0; j < NV; j++)
+                __hip_atomic_store(&part[q * NV + j], v[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    if (threadIdx.x == 0) {
+        const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last)
+        return;
+    double acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++)
+        acc[j] = 0.0;
+    for (int q = threadIdx.x; q < nparts; q += RED_THREADS) {
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+            acc[j] += __hip_atomic_load(&part[q * NV + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    block_sum<NV>(acc, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+            out[j] = acc[j];
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // phase helpers: exp(2 pi i t) with t range-reduced in FP64
 // ---------------------------------------------------------------------------

@@ -258,8 +258,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_coef(const float2 *__restrict__
                                                       float2 *__restrict__ coef, double *part, unsigned *ticket,
                                                       double *scal)
 {
-    double v[1] = {0.0};
-    for (int p = blockIdx.x * RED_THREADS + threadIdx.x; p < nuv; p += gridDim.x * RED_THREADS) {
+    auto item = [&](int p, double (&v)[1]) {
         const float2 Vp = vis[p];
         float2 a = make_float2(0.f, 0.f);
         if (p < npow) {
@@ -286,8 +285,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_coef(const float2 *__restrict__
         }
         coef[p] = a;
         v[0] += (double)a.x * (double)Vp.x - (double)a.y * (double)Vp.y;
-    }
-    reduce_finish<1>(v, part, ticket, scal + SC_C0, nullptr);
+    };
+    partitioned_reduce<1>(nuv, item, part, ticket, scal + SC_C0);
 }
 
 // ---------------------------------------------------------------------------
@@ -655,15 +654,13 @@ __global__ __launch_bounds__(RED_THREADS) void k_ls_dchi2(const double2 *__restr
     // each with its own partials, ticket and outputs (fewer live registers per lane)
     const int grp = blockIdx.y, g0 = grp * LS_G;
     const double FI = scal[SC_FLUX], Fd = scal[SC_FD];
-    double sk[LS_G], ivk[LS_G], v[LS_G];
+    double sk[LS_G], ivk[LS_G];
 #pragma unroll
     for (int k = 0; k < LS_G; k++) {
         sk[k] = ls_step(s_ref, g0 + k);
         ivk[k] = 1.0 / (FI + sk[k] * Fd);
-        v[k] = 0.0;
     }
-    const int total = npow + nbis;
-    for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < total; i += gridDim.x * RED_THREADS) {
+    auto item = [&](int i, double (&v)[LS_G]) {
         if (i < npow) {
             const double2 a = Vd[i];
             const double bx = Ud[i].x, by = Ud[i].y, d0 = data[i], e = err[i];
@@ -708,8 +705,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_ls_dchi2(const double2 *__restr
                 v[k] += r0 * r0 + r1 * r1 - q0;
             }
         }
-    }
-    reduce_finish<LS_G>(v, part + (size_t)grp * gridDim.x * LS_G, ticket + grp, out + g0, nullptr);
+    };
+    partitioned_reduce<LS_G>(npow + nbis, item, part + (size_t)grp * RED_PARTS * LS_G, ticket + grp, out + g0);
 }
 
 // change of the entropy along the line: out[k] = sum S(max(I + s_k d, minval)) - S(I)
@@ -721,16 +718,13 @@ __global__ __launch_bounds__(RED_THREADS) void k_ls_dentropy(const float *__rest
 {
     const int grp = blockIdx.y, g0 = grp * LS_G;
     float sk[LS_G];
-    double v[LS_G];
 #pragma unroll
-    for (int k = 0; k < LS_G; k++) {
+    for (int k = 0; k < LS_G; k++)
         sk[k] = (float)ls_step(s_ref, g0 + k);
-        v[k] = 0.0;
-    }
-    for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) {
+    auto item = [&](int i, double (&v)[LS_G]) {
         const float I = img[i], dd = d[i], M = model[i];
         if (!(I > 0.f && M > 0.f))
-            continue;   // the floor keeps I > 0; S = -M there whatever the step
+            return;   // the floor keeps I > 0; S = -M there whatever the step
         // S(t) - S(I) = dl (1 - ln(I/M)) - t log1p(dl / I), dl = t - I: every term is
         // proportional to the change, so FP32 resolves it relative to the change itself
         const float L = logf(I / M), invI = 1.0f / I;
@@ -745,8 +739,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_ls_dentropy(const float *__rest
             const float lr = q > -0.5f ? log1pf(q) : logf(t * invI);
             v[k] += (double)(dl * (1.0f - L) - t * lr);
         }
-    }
-    reduce_finish<LS_G>(v, part + (size_t)grp * gridDim.x * LS_G, ticket + grp, out + g0, nullptr);
+    };
+    partitioned_reduce<LS_G>(n, item, part + (size_t)grp * RED_PARTS * LS_G, ticket + grp, out + g0);
 }
 
 // trial image I_t = max(I + s d, minval), the change delta = I_t - I, and

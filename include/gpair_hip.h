@@ -223,13 +223,14 @@ int gpair_set_pfactor(gpair_ctx *ctx, const float *pfactor, int size);
 
 /* Autotuner (no reference counterpart: the reference fixes its work-group sizes
  * when it builds the kernels, gpair1/trunk/gpu.c:413-498).  Times candidate
- * launch shapes of the size-dependent kernels on the context's data and current
- * image -- the grids of the batched line-search reductions and of the adjoint
- * coefficients, the uv split of the FP32 and BF16x3 adjoints -- and keeps the
- * fastest for the context (reps launches per candidate, <= 0: 5).  chosen, if not
- * NULL, receives up to 5 values: line-search chi2 grid, line-search entropy grid,
- * coefficient grid, FP32 adjoint workgroups, BF16x3 adjoint workgroups.  Call it
- * after gpu_build_kernels; a new gpu_build_kernels restores the default split. */
+ * launch shapes of the size-dependent reductions on the context's data and
+ * current image -- the grids of the batched line-search kernels and of the
+ * adjoint coefficients, the row grouping of the BF16x3 forward DFT -- and keeps
+ * the fastest for the context (reps launches per candidate, <= 0: 5).  No
+ * summation order depends on these shapes, so tuning changes no result.
+ * chosen, if not NULL, receives up to 4 values: line-search chi2 grid,
+ * line-search entropy grid, coefficient grid, forward rows per group / 32.
+ * Call it after gpu_build_kernels. */
 int gpair_autotune(gpair_ctx *ctx, int reps, int *chosen, int nchosen);
 
 /* ---- host-side data preparation (no device needed) ----------------------- */

@@ -254,3 +254,20 @@ def test_autotuned_shapes_keep_parity(orc):
     img_o, tr_o = orc.pcg(P, prob.start.astype(np.float64), alpha, precond=1, max_iter=12, chi2r_target=0.0)
     assert np.abs(tr[:, 0] - tr_o[:, 0]).max() / np.abs(tr_o[:, 0]).min() <= 1e-3
     assert np.array_equal(tr[:, 5:8], tr_o[:, 5:8])
+
+
+def test_autotuned_grids_change_no_bit():
+    """the tuned reductions sum in a grid-independent order: a pcg run with tuned
+    launch shapes reproduces the default-shape run bit for bit."""
+    prob = synth.make_problem("cfg1", seed=3, W=128, ntel=7, nch=6, nsnap=5)
+    alpha = prob.meta["alpha"]
+    outs = []
+    for tune in (False, True):
+        with GpairContext.from_problem(prob) as ctx:
+            ctx.set_precision(1)
+            if tune:
+                print("autotune", ctx.autotune(reps=2), flush=True)
+            res, tr = ctx.pcg_run(alpha, precond=1, max_iter=40, chi2r_target=0.0)
+            outs.append((tr.copy(), ctx.get("ci")))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
