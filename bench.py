@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-start run")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch shapes")
+    ap.add_argument("--batch", type=int, default=64, help="problems of the batching workload (0: skip)")
+    ap.add_argument("--batch-iters", type=int, default=100)
     ap.add_argument("--cpu-kind", default="auto", choices=["auto", "reference", "port"])
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 256 MiB device copy (HBM counter calibration for rocprofv3 --pmc)")
@@ -146,6 +148,52 @@ def fp64_errors(prob, alpha, finals):
         J64 = orc.objective(P, img.astype(np.float64), alpha)[0]
         out[name] = {"J_reported": J, "J_fp64": J64, "rel_err": abs(J - J64) / abs(J64)}
     return out
+
+
+def batch_workload(a):
+    """cfg4: independent problems of uneven sizes (widths 64-256, 500-40,000 V2, T3 about
+    2.3 V2), each run for a fixed number of accelerated-loop iterations (BF16x3): one
+    gpair_pcg_run after the other, then all of them in one gpair_batch_pcg_run (one stream
+    per problem, gradient phase replayed from hipGraphs).  Problem set-up is untimed."""
+    from gpair_amd import synth
+    from gpair_amd.api import GpairContext, batch_pcg_run
+    t0 = time.perf_counter()
+    probs = synth.batch_problems(a.batch)
+    t_gen = time.perf_counter() - t0
+    ctxs = []
+    for p in probs:
+        c = GpairContext.from_problem(p)
+        c.set_precision(1)
+        if not a.no_autotune:
+            c.autotune(reps=2)
+        ctxs.append(c)
+    alphas = [p.meta["alpha"] for p in probs]
+    starts = [c.get("ci") for c in ctxs]
+    K = a.batch_iters
+    t0 = time.perf_counter()
+    its_seq = 0
+    for c, al in zip(ctxs, alphas):
+        r, _ = c.pcg_run(al, precond=1, max_iter=K, chi2r_target=0.0)
+        its_seq += r.iterations
+    for c in ctxs:
+        c.L.gpair_sync(c.h)
+    t_seq = time.perf_counter() - t0
+    for c, s0 in zip(ctxs, starts):
+        c.gpu_copy_image(s0)
+        c.set_graphs(True)
+    t0 = time.perf_counter()
+    res = batch_pcg_run(ctxs, alphas, max_iter=K, chi2r_target=0.0)
+    t_batch = time.perf_counter() - t0
+    its_b = sum(r.iterations for r in res)
+    for c in ctxs:
+        c.close()
+    print(f"# batch cfg4: {len(probs)} problems x {K} its: sequential {t_seq:.2f} s, batch {t_batch:.2f} s",
+          file=sys.stderr, flush=True)
+    return {"problems": len(probs), "iterations_each": K, "widths": sorted({p.W for p in probs}),
+            "v2_range": [min(p.npow for p in probs), max(p.npow for p in probs)],
+            "t3_range": [min(p.nbis for p in probs), max(p.nbis for p in probs)],
+            "sequential_s": t_seq, "batch_s": t_batch, "iters_per_s_sequential": its_seq / t_seq,
+            "iters_per_s_batch": its_b / t_batch, "speedup": t_seq / t_batch, "setup_generation_s": t_gen}
 
 
 def timing(L, h):
@@ -324,6 +372,8 @@ def main():
         tt["speedup_vs_plain_fp32"] = ((">=" if not conv else "") +
                                        f"{tt['plain_fp32']['wall_s'] / best:.1f}")
         out["to_target"] = tt
+    if a.batch > 0:
+        out["batch_cfg4"] = batch_workload(a)
     if not a.no_cpu:
         counts = plain_counts if plain_counts else (10.0, 2.0)
         out["cpu_baseline"] = cpu_baseline(prob, counts[0], counts[1], a.cpu_kind)

@@ -94,6 +94,8 @@ SIGNATURES = {
     "gpair_pcg_run": (_i, [_vp, C.POINTER(PcgParams), C.POINTER(PcgResult), _vp, _i]),
     "gpair_set_pfactor": (_i, [_vp, _vp, _i]),
     "gpair_autotune": (_i, [_vp, _i, _vp, _i]),
+    "gpair_set_graphs": (_i, [_vp, _i]),
+    "gpair_batch_pcg_run": (_i, [_vp, _i, _vp, _vp]),
     "gpair_set_model": (_i, [_vp, _i, C.c_float, _i, C.c_float, C.c_float, _i]),
     "gpair_init_data": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _pi]),
     # learned start (include/gpair_nn.h)

@@ -249,3 +249,25 @@ class GpairContext:
         rc = self.L.gpair_pcg_run(self.h, C.byref(p), C.byref(res), _ptr(trace), trace.shape[0])
         self._check(rc, "gpair_pcg_run")
         return res, trace[:min(res.iterations, trace.shape[0])]
+
+    def set_graphs(self, on=True):
+        self._check(self.L.gpair_set_graphs(self.h, int(bool(on))), "gpair_set_graphs")
+
+
+def batch_pcg_run(ctxs, alphas, precond=1, max_iter=1000, chi2r_target=1.0):
+    """gpair_batch_pcg_run over several contexts (one stream each); returns the
+    per-context results."""
+    L = _lib.load()
+    n = len(ctxs)
+    params = (_lib.PcgParams * n)()
+    for i, a in enumerate(alphas):
+        L.gpair_pcg_default_params(C.byref(params[i]))
+        params[i].alpha, params[i].precond, params[i].max_iter = float(a), int(precond), int(max_iter)
+        params[i].chi2r_target = float(chi2r_target)
+    results = (_lib.PcgResult * n)()
+    handles = (C.c_void_p * n)(*[c.h.value for c in ctxs])
+    rc = L.gpair_batch_pcg_run(handles, n, params, results)
+    if rc != 0:
+        bad = [i for i in range(n) if results[i].status]
+        raise GpairError(f"gpair_batch_pcg_run: {L.gpair_strerror(rc).decode()} (contexts {bad})")
+    return list(results)

@@ -64,6 +64,7 @@ struct gpair_ctx {
     // launch shapes chosen by gpair_autotune (0: the built-in default)
     int ls_blocks_chi2, ls_blocks_ent, coef_blocks;
     int fwd_nbk;   // image rows per BF16x3 forward group / 32 (0: min(W, 256) / 32)
+    int graphs;    // replay the accelerated loop's gradient phase from a hipGraph
     // optional per-launch timing of the DFT kernels (HIP events on the context stream)
     int timing;
     std::vector<hipEvent_t> ev[GPAIR_TK_COUNT];

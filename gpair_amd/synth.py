@@ -342,6 +342,10 @@ CONFIGS = {
     # closure phase by 1 / |observed amplitude|)
     "cfg5": dict(W=256, pix=0.05, ntel=9, nch=64, nsnap=16, uvmax=3e8, sky="learn_mix", alpha=0.3,
                  prior=("gauss", 5.0), t3_min=1e-3),
+    # batching: 9 telescopes (T3 = 84/36 of V2), widths and V2 counts per problem
+    # drawn by batch_problems
+    "cfg4": dict(W=256, pix=0.05, ntel=9, nch=16, nsnap=4, uvmax=3e8, sky="learn_mix", alpha=0.3,
+                 prior=("gauss", 5.0), t3_min=1e-3),
     "cfg3": dict(W=512, pix=0.05, ntel=9, nch=256, nsnap=16, uvmax=3e8, sky="ring_disk", alpha=0.3,
                  prior=("gauss", 10.0)),
 }
@@ -451,3 +455,28 @@ def make_problem(cfg="cfg1", seed=1, W=None, pix=None, ntel=None, nch=None, nsna
     return Problem(W=W_, pixellation=c["pix"], u=uv[:, 0].copy(), v=uv[:, 1].copy(), npow=npow,
                    nbis=nbis, data=data, err=err, phasor=phasor, uvpnt=uvpnt, sign=sign,
                    model=model, start=start, truth_vis=V, meta=meta, truth=(truth * prior_flux).astype(np.float32))
+
+
+BATCH_WIDTHS = (64, 128, 192, 256)
+
+
+def batch_spec(count=64, seed=4, v2_range=(500, 40000)):
+    """sizes of the batching workload: per problem a width from BATCH_WIDTHS and a
+    V2 count near a uniform draw from v2_range (36 baselines x channels x snapshots)."""
+    rng = np.random.default_rng(seed)
+    spec = []
+    for i in range(count):
+        W = int(rng.choice(BATCH_WIDTHS))
+        target = rng.uniform(*v2_range)
+        nsnap = int(rng.integers(1, 17))
+        nch = max(1, int(round(target / (36 * nsnap))))
+        spec.append(dict(seed=seed * 1000 + i, W=W, nch=nch, nsnap=nsnap))
+    return spec
+
+
+def batch_problems(count=64, seed=4, v2_range=(500, 40000), workers=8):
+    """cfg4 problems (built on host threads: the FP64 numpy DFTs release the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    spec = batch_spec(count, seed, v2_range)
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        return list(pool.map(lambda k: make_problem("cfg4", **k), spec))

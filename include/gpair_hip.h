@@ -220,6 +220,19 @@ int gpair_pcg_run(gpair_ctx *ctx, const gpair_pcg_params *params, gpair_pcg_resu
                   int trace_rows);
 /* per-pixel preconditioner factor [W*W] multiplying P (NULL clears it) */
 int gpair_set_pfactor(gpair_ctx *ctx, const float *pfactor, int size);
+/* on != 0: gpair_pcg_run replays each iteration's gradient phase (flux, adjoint
+ * coefficients, adjoint DFT, entropy and criterion gradients, preconditioner)
+ * from a hipGraph captured at the start of the run; results are bit-identical
+ * to the launched path.  Off by default; ignored while DFT timing is on. */
+int gpair_set_graphs(gpair_ctx *ctx, int on);
+/* Batch (no reference counterpart): runs the accelerated loop on `count`
+ * independent contexts (any sizes), each on its own stream, with params[i] and
+ * results[i] per context.  The host resumes whichever problem has finished its
+ * read-back, so the GPU works on the others meanwhile; every problem follows
+ * exactly the steps gpair_pcg_run would take (bit-identical results).  Returns
+ * the first non-zero status, GPAIR_OK when all succeed. */
+int gpair_batch_pcg_run(gpair_ctx *const *ctxs, int count, const gpair_pcg_params *params,
+                        gpair_pcg_result *results);
 
 /* Autotuner (no reference counterpart: the reference fixes its work-group sizes
  * when it builds the kernels, gpair1/trunk/gpu.c:413-498).  Times candidate

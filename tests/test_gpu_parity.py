@@ -271,3 +271,43 @@ def test_autotuned_grids_change_no_bit():
             outs.append((tr.copy(), ctx.get("ci")))
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_gradient_graphs_are_bit_identical():
+    """the hipGraph replay of the gradient phase, replayed every iteration with the
+    loop's own launches and read-backs in between, against the launched path."""
+    prob = synth.make_problem("cfg1", seed=5, W=128, ntel=7, nch=6, nsnap=5)
+    alpha = prob.meta["alpha"]
+    outs = []
+    for graphs in (False, True):
+        with GpairContext.from_problem(prob) as ctx:
+            ctx.set_precision(1)
+            ctx.set_graphs(graphs)
+            res, tr = ctx.pcg_run(alpha, precond=1, max_iter=40, chi2r_target=0.0)
+            outs.append((tr.copy(), ctx.get("ci"), res.J))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1]) and outs[0][2] == outs[1][2]
+
+
+def test_batch_matches_single_runs():
+    """gpair_batch_pcg_run over problems of different widths and uv counts takes, for
+    each problem, exactly the steps of its own gpair_pcg_run."""
+    from gpair_amd.api import batch_pcg_run
+    probs = [synth.make_problem("cfg1", seed=1), synth.make_problem("cfg1", seed=2, W=128, ntel=7, nch=6, nsnap=5)]
+    probs += synth.batch_problems(3, seed=7, v2_range=(500, 4000))
+    print("batch sizes", [(p.W, p.npow, p.nbis) for p in probs], flush=True)
+    single = []
+    for p in probs:
+        with GpairContext.from_problem(p) as ctx:
+            ctx.set_precision(1)
+            r, _ = ctx.pcg_run(p.meta["alpha"], precond=1, max_iter=150, chi2r_target=1.0)
+            single.append((r.iterations, r.J, r.converged, ctx.get("ci")))
+    ctxs = [GpairContext.from_problem(p) for p in probs]
+    for c in ctxs:
+        c.set_precision(1)
+        c.set_graphs(True)
+    res = batch_pcg_run(ctxs, [p.meta["alpha"] for p in probs], max_iter=150, chi2r_target=1.0)
+    for (its, J, conv, img), r, c in zip(single, res, ctxs):
+        assert r.iterations == its and r.J == J and bool(r.converged) == bool(conv)
+        assert np.array_equal(c.get("ci"), img)
+        c.close()

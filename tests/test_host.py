@@ -169,3 +169,14 @@ def test_init_data_small_error_limit():
     assert np.allclose(a2, [0.5 * 0.6 * 0.7, 0.6 * 0.7 * 0.8], rtol=1e-5)
     # relative error of the product of three independent 1e-6 estimates ~ sqrt(3) * 1e-6 / 2
     assert np.allclose(e2 / a2, np.sqrt(3) * 0.5e-6, rtol=0.05)
+
+
+def test_batch_rejects_bad_arguments():
+    L = _lib.load()
+    p = (_lib.PcgParams * 2)()
+    r = (_lib.PcgResult * 2)()
+    assert L.gpair_batch_pcg_run(None, 2, p, r) == 1
+    h = (C.c_void_p * 2)(None, None)
+    assert L.gpair_batch_pcg_run(h, 2, p, r) == 1
+    assert L.gpair_batch_pcg_run(h, 0, p, r) == 1
+    assert L.gpair_set_graphs(None, 1) == 1
