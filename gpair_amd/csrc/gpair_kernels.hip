@@ -488,15 +488,23 @@ __global__ __launch_bounds__(256) void k_adj_b3(const float2 *__restrict__ coef,
             const size_t bo = ((size_t)xw * nuvp + pr + 8 * ks + 4 * lh) * 2;
             const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(Xth + bo);
             const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(Xtl + bo);
+            bf16x8 ah[2], al[2];
 #pragma unroll
             for (int m = 0; m < 2; m++) {
                 const int o = (32 * m + lr) * AB3_STRIDE + 16 * ks + 8 * lh;
-                const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(&sAh[o]);
-                const bf16x8 al = *reinterpret_cast<const bf16x8 *>(&sAl[o]);
-                acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[m], 0, 0, 0);
-                acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[m], 0, 0, 0);
-                acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[m], 0, 0, 0);
+                ah[m] = *reinterpret_cast<const bf16x8 *>(&sAh[o]);
+                al[m] = *reinterpret_cast<const bf16x8 *>(&sAl[o]);
             }
+            // product-major over the two accumulators (same per-tile order hh, hl, lh)
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+                acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m], bh, acc[m], 0, 0, 0);
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+                acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m], bl, acc[m], 0, 0, 0);
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+                acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[m], bh, acc[m], 0, 0, 0);
         }
         __syncthreads();
     }
@@ -972,43 +980,62 @@ __global__ __launch_bounds__(256) void k_vis_fwd_b3(const unsigned short *__rest
             a1h = *reinterpret_cast<const uint4 *>(Xh + arow + 16 + 8 * lh);
             a1l = *reinterpret_cast<const uint4 *>(Xl + arow + 16 + 8 * lh);
         }
-        for (int c = 0; c < nchunk; c++) {
+        // One chunk: fetch the next chunk's image rows (L2-resident) and then its X
+        // rows (the HBM stream) into the idle register set, run the MFMAs on the
+        // current set, store the image rows into the idle LDS buffer.  The last chunk
+        // re-fetches itself so that every chunk has the same body; issuing the image
+        // loads before the table loads lets the table loads stay in flight across
+        // the barrier (in-order load counters).  Chunks go in pairs, so the two
+        // register sets swap roles without copies.
+        uint4 b0h = a0h, b0l = a0l, b1h = a1h, b1l = a1l;
+        auto chunk = [&](int c, const uint4 &c0h, const uint4 &c0l, const uint4 &c1h, const uint4 &c1l, uint4 &d0h,
+                         uint4 &d0l, uint4 &d1h, uint4 &d1l) {
             const int buf = c & 1;
-            uint4 n0h = a0h, n0l = a0l, n1h = a1h, n1l = a1l;
-            if (c + 1 < nchunk) {
-                if (aval) {
-                    const int kn = (c + 1) * B3_KC;
-                    n0h = *reinterpret_cast<const uint4 *>(Xh + arow + kn + 8 * lh);
-                    n0l = *reinterpret_cast<const uint4 *>(Xl + arow + kn + 8 * lh);
-                    n1h = *reinterpret_cast<const uint4 *>(Xh + arow + kn + 16 + 8 * lh);
-                    n1l = *reinterpret_cast<const uint4 *>(Xl + arow + kn + 16 + 8 * lh);
-                }
-                b3_fetch<PER, CH>(h0, h1, h2, h3, l0, l1, l2, l3, Ih, Il, tid, y0, W, c + 1);
+            const int cn = c + 1 < nchunk ? c + 1 : c;
+            b3_fetch<PER, CH>(h0, h1, h2, h3, l0, l1, l2, l3, Ih, Il, tid, y0, W, cn);
+            __builtin_amdgcn_sched_barrier(0);   // keep the image loads ahead of the table loads
+            {   // unconditional: lanes past nuv read row 0 (arow = 0) and are never stored;
+                // a branch here would make the load counters wait for these loads too
+                const int kn = cn * B3_KC;
+                d0h = *reinterpret_cast<const uint4 *>(Xh + arow + kn + 8 * lh);
+                d0l = *reinterpret_cast<const uint4 *>(Xl + arow + kn + 8 * lh);
+                d1h = *reinterpret_cast<const uint4 *>(Xh + arow + kn + 16 + 8 * lh);
+                d1l = *reinterpret_cast<const uint4 *>(Xl + arow + kn + 16 + 8 * lh);
             }
+            __builtin_amdgcn_sched_barrier(0);   // and both ahead of this chunk's MFMAs
 #pragma unroll
             for (int ks = 0; ks < B3_KC / 16; ks++) {
-                const uint4 ah_c = ks ? a1h : a0h, al_c = ks ? a1l : a0l;
-                const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(&ah_c);
-                const bf16x8 al = *reinterpret_cast<const bf16x8 *>(&al_c);
+                const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(ks ? &c1h : &c0h);
+                const bf16x8 al = *reinterpret_cast<const bf16x8 *>(ks ? &c1l : &c0l);
+                // every B operand of the k-step read from LDS before the first MFMA: the
+                // reads overlap each other instead of one LDS latency per tile
+                bf16x8 bh[NBK], bl[NBK];
 #pragma unroll
                 for (int b = 0; b < NBK; b++) {
                     const int o = (b * 32 + lr) * B3_STRIDE + ks * 16 + 8 * lh;
-                    const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&sB[buf][0][o]);
-                    const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&sB[buf][1][o]);
-                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[b], 0, 0, 0);
-                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[b], 0, 0, 0);
-                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[b], 0, 0, 0);
+                    bh[b] = *reinterpret_cast<const bf16x8 *>(&sB[buf][0][o]);
+                    bl[b] = *reinterpret_cast<const bf16x8 *>(&sB[buf][1][o]);
                 }
+                // product-major: consecutive MFMAs update different accumulators, so
+                // none waits for the previous one's result (same per-tile order hh, hl, lh)
+#pragma unroll
+                for (int b = 0; b < NBK; b++)
+                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[b], acc[b], 0, 0, 0);
+#pragma unroll
+                for (int b = 0; b < NBK; b++)
+                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[b], acc[b], 0, 0, 0);
+#pragma unroll
+                for (int b = 0; b < NBK; b++)
+                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[b], acc[b], 0, 0, 0);
             }
-            a0h = n0h;
-            a0l = n0l;
-            a1h = n1h;
-            a1l = n1l;
-            if (c + 1 < nchunk) {
-                // the other buffer was last read in chunk c-1, before the barrier below
-                b3_store<PER, CH>(h0, h1, h2, h3, l0, l1, l2, l3, sB[buf ^ 1][0], sB[buf ^ 1][1], tid);
-                __syncthreads();
-            }
+            // the other buffer was last read in chunk c-1, before the previous barrier
+            b3_store<PER, CH>(h0, h1, h2, h3, l0, l1, l2, l3, sB[buf ^ 1][0], sB[buf ^ 1][1], tid);
+            __syncthreads();
+        };
+        static_assert(B3_KC == 32, "W / 32 chunks, an even count for W a multiple of 64");
+        for (int c = 0; c < nchunk; c += 2) {
+            chunk(c, a0h, a0l, a1h, a1l, b0h, b0l, b1h, b1l);
+            chunk(c + 1, b0h, b0l, b1h, b1l, a0h, a0l, a1h, a1l);
         }
         // epilogue of this row group: point 8(r/4) + 4 lh + r%4 (r < 8) has Re T in
         // register r and Im T in register r+8, column y = y0 + 32 b + lr
