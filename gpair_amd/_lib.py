@@ -96,6 +96,14 @@ SIGNATURES = {
     "gpair_autotune": (_i, [_vp, _i, _vp, _i]),
     "gpair_set_graphs": (_i, [_vp, _i]),
     "gpair_batch_pcg_run": (_i, [_vp, _i, _vp, _vp]),
+    "gpair_pixel_begin": (_i, [_vp]),
+    "gpu_update_vis_fluxchange": (_i, [_vp, _i, _i, _f, _i, _i, _i]),
+    "gpair_update_vis_positionchange": (_i, [_vp, _i, _i, _i, _i, _f, _i, _i]),
+    "gpu_new_chi2": (_i, [_vp, _i, _i, _i, _i, _pd]),
+    "gpair_get_trial_vis": (_i, [_vp, _vp, _i]),
+    "gpair_pixel_propose": (_i, [_vp, _vp, _i, _vp, _vp]),
+    "gpair_pixel_apply": (_i, [_vp, _vp, _i]),
+    "gpair_pixel_state": (_i, [_vp, _pd, _pd, _pd]),
     "gpair_set_model": (_i, [_vp, _i, C.c_float, _i, C.c_float, C.c_float, _i]),
     "gpair_init_data": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _pi]),
     # learned start (include/gpair_nn.h)

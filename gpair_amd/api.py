@@ -21,6 +21,18 @@ class GpairError(RuntimeError):
     pass
 
 
+# gpair_pixel_move (include/gpair_hip.h): pixels x + W*y, pix_b = -1 for one pixel
+PIXEL_MOVE = np.dtype([("pix_a", np.int32), ("pix_b", np.int32), ("flux_a", np.float32), ("flux_b", np.float32)])
+
+
+def pixel_moves(moves):
+    """moves as a contiguous PIXEL_MOVE array; accepts such an array or a
+    sequence of (pix_a, pix_b, flux_a, flux_b) tuples"""
+    if isinstance(moves, np.ndarray) and moves.dtype == PIXEL_MOVE:
+        return np.ascontiguousarray(moves)
+    return np.array([tuple(m) for m in moves], dtype=PIXEL_MOVE)
+
+
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
@@ -135,6 +147,54 @@ class GpairContext:
         out = np.empty(self.ndof, np.float32)
         self._check(self.L.gpair_get_mock(self.h, _ptr(out), self.ndof), "gpair_get_mock")
         return out
+
+    # -- single-pixel moves (gpair1/trunk/gpu.c:2067-2182, gpair1/trunk/cpu.c:477-497)
+    def pixel_begin(self):
+        """(re)build the pixel state from the current image (one forward DFT)"""
+        self._check(self.L.gpair_pixel_begin(self.h), "gpair_pixel_begin")
+
+    def gpu_update_vis_fluxchange(self, x, y, new_pixel_flux):
+        """trial visibilities of the pixel state with pixel (x, y) set to new_pixel_flux"""
+        self._check(self.L.gpu_update_vis_fluxchange(self.h, int(x), int(y), float(new_pixel_flux), self.W,
+                                                     self.nuv, self.nuv), "gpu_update_vis_fluxchange")
+
+    def update_vis_positionchange(self, x_old, y_old, x_new, y_new, flux):
+        """trial visibilities with `flux` moved from (x_old, y_old) to (x_new, y_new)"""
+        self._check(self.L.gpair_update_vis_positionchange(self.h, int(x_old), int(y_old), int(x_new), int(y_new),
+                                                           float(flux), self.W, self.nuv),
+                    "gpair_update_vis_positionchange")
+
+    def gpu_new_chi2(self):
+        """chi2 of the trial visibilities"""
+        v = C.c_double()
+        self._check(self.L.gpu_new_chi2(self.h, self.nuv, self.npow, self.nbis, self.ndof, C.byref(v)),
+                    "gpu_new_chi2")
+        return v.value
+
+    def trial_vis(self):
+        out = np.empty(self.nuv, np.complex64)
+        self._check(self.L.gpair_get_trial_vis(self.h, _ptr(out), self.nuv), "gpair_get_trial_vis")
+        return out
+
+    def pixel_propose(self, moves):
+        """dchi2, dentropy (FP64 arrays) of each move against the pixel state;
+        moves: a PIXEL_MOVE structured array (or anything pixel_moves accepts)"""
+        m = pixel_moves(moves)
+        dchi2 = np.empty(m.size, np.float64)
+        dS = np.empty(m.size, np.float64)
+        self._check(self.L.gpair_pixel_propose(self.h, _ptr(m), m.size, _ptr(dchi2), _ptr(dS)),
+                    "gpair_pixel_propose")
+        return dchi2, dS
+
+    def pixel_apply(self, moves):
+        m = pixel_moves(moves)
+        self._check(self.L.gpair_pixel_apply(self.h, _ptr(m), m.size), "gpair_pixel_apply")
+
+    def pixel_state(self):
+        """(chi2, entropy, flux) of the pixel state"""
+        c2, S, F = C.c_double(), C.c_double(), C.c_double()
+        self._check(self.L.gpair_pixel_state(self.h, C.byref(c2), C.byref(S), C.byref(F)), "gpair_pixel_state")
+        return c2.value, S.value, F.value
 
     # -- objective / gradients (gpair1/trunk/gpu.c:596-857, 1598-1802) ---------
     def gpu_get_chi2_curr(self):

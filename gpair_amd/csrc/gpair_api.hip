@@ -65,6 +65,19 @@ struct gpair_ctx {
     int ls_blocks_chi2, ls_blocks_ent, coef_blocks;
     int fwd_nbk;   // image rows per BF16x3 forward group / 32 (0: min(W, 256) / 32)
     int graphs;    // replay the accelerated loop's gradient phase from a hipGraph
+    // single-pixel moves: state image, FP64 unnormalised visibilities, x-major
+    // tables, trial visibilities and mock data, per-move buffers
+    float *px_img;
+    double2 *px_U;
+    float2 *px_XT, *px_YT, *px_vis, *px_Ud;
+    float *px_mock;
+    double *px_dF, *px_out, *px_part, *px_dpx;
+    unsigned *px_ticket;
+    gpair_pixel_move *px_moves;
+    double *px_host;                // pinned, 2 * px_cap
+    gpair_pixel_move *px_staged;    // pinned, px_cap
+    hipEvent_t px_ev;               // last upload from px_staged
+    int px_cap, px_valid, px_trial, px_tables, px_n, px_nuv, px_ndof;
     // optional per-launch timing of the DFT kernels (HIP events on the context stream)
     int timing;
     std::vector<hipEvent_t> ev[GPAIR_TK_COUNT];
@@ -193,6 +206,9 @@ extern "C" int gpu_init(gpair_ctx **out, int device)
     return GPAIR_OK;
 }
 
+static void px_free(gpair_ctx *c);
+static inline void px_invalidate(gpair_ctx *c);
+
 static void free_images(gpair_ctx *c)
 {
     gpair_buf_s *bufs[] = {&c->img,  &c->tmp,  &c->model, &c->gdata, &c->gent, &c->gnew,
@@ -211,6 +227,7 @@ extern "C" int gpu_cleanup(gpair_ctx *c)
     if (c->stream)
         (void)hipStreamSynchronize(c->stream);
     free_images(c);
+    px_free(c);
     dfree(&c->data); dfree(&c->err); dfree(&c->phasor); dfree(&c->tri); dfree(&c->csr_ptr); dfree(&c->csr_ent);
     dfree(&c->X); dfree(&c->Y); dfree(&c->vis); dfree(&c->coef); dfree(&c->mock);
     dfree(&c->scal); dfree(&c->invflux); dfree(&c->part); dfree(&c->ticket); dfree(&c->adj_part);
@@ -322,6 +339,7 @@ extern "C" int gpu_copy_data(gpair_ctx *c, const float *data, const float *data_
         for (int j = 0; j < 3; j++)
             ent[fill[uvpnt[4 * t + j]]++] = 3 * t + j;
     HIPCHK(c, hipSetDevice(c->device));
+    px_invalidate(c);   // sizes may change: the pixel state is rebuilt on its next use
     const int ndof = npow + 2 * nbis;
     if ((rc = dalloc(c, &c->data, ndof)) || (rc = dalloc(c, &c->err, ndof)) || (rc = dalloc(c, &c->mock, ndof)) ||
         (rc = dalloc(c, &c->phasor, nbis)) || (rc = dalloc(c, &c->tri, nbis)) ||
@@ -365,6 +383,7 @@ extern "C" int gpu_copy_image(gpair_ctx *c, const float *image, int x_size, int 
         return fail(c, GPAIR_E_STATE, "gpu_copy_image before gpu_copy_data");
     if (x_size != c->W || y_size != c->W)
         return fail(c, GPAIR_E_ARG, "gpu_copy_image: %dx%d, expected %dx%d", x_size, y_size, c->W, c->W);
+    px_invalidate(c);
     HIPCHK(c, hipMemcpyAsync(c->img.p, image, sizeof(float) * c->n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GPAIR_OK;
@@ -404,6 +423,8 @@ extern "C" int gpair_set_uv(gpair_ctx *c, const float *u, const float *v, int nu
     if (!err)
         c->have_tables = 1;
     c->split_valid = 0;
+    px_invalidate(c);
+    c->px_tables = 0;
     return err;
 }
 
@@ -438,6 +459,8 @@ extern "C" int gpu_copy_dft(gpair_ctx *c, const float *dft_x, const float *dft_y
     if (!err)
         c->have_tables = 1;
     c->split_valid = 0;
+    px_invalidate(c);
+    c->px_tables = 0;
     return err;
 }
 
@@ -892,6 +915,7 @@ extern "C" int gpu_update_tempimage(gpair_ctx *c, int W, double s, double minval
 
 extern "C" int gpu_update_image(gpair_ctx *c, int W, double s, double minval, gpair_buf d)
 {
+    px_invalidate(c);
     return step(c, c ? c->img.p : nullptr, W, s, minval, d);
 }
 
@@ -901,6 +925,8 @@ extern "C" int gpu_backup_gradient(gpair_ctx *c, int n, gpair_buf in, gpair_buf 
         return GPAIR_E_ARG;
     if (!own_buf(c, in) || !own_buf(c, out) || n < 0 || n > in->n || n > out->n)
         return fail(c, GPAIR_E_ARG, "gpu_backup_gradient: bad buffers or size %d", n);
+    if (out == &c->img)
+        px_invalidate(c);
     HIPCHK(c, hipMemcpyAsync(out->p, in->p, sizeof(float) * n, hipMemcpyDeviceToDevice, c->stream));
     return GPAIR_OK;
 }
@@ -922,6 +948,8 @@ extern "C" int gpair_put_image(gpair_ctx *c, gpair_buf which, const float *host,
         return fail(c, GPAIR_E_ARG, "gpair_put_image: NULL pointer");
     if (!own_buf(c, which) || size < 0 || size > which->n)
         return fail(c, GPAIR_E_ARG, "gpair_put_image: bad buffer or size %d", size);
+    if (which == &c->img)
+        px_invalidate(c);
     HIPCHK(c, hipMemcpyAsync(which->p, host, sizeof(float) * size, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GPAIR_OK;
@@ -1090,3 +1118,4 @@ static inline void swap_ptr(T *&a, T *&b)
 }
 #include "gpair_pcg.inc"
 #include "gpair_tune.inc"
+#include "gpair_pixel.inc"

@@ -27,6 +27,8 @@ enum {
     SC_DSENT = 11,
     SC_DCHI2 = 12,
     SC_BETA = 13,    // beta the direction kernel used
+    SC_PXF = 14,     // flux of the pixel-move state
+    SC_PXC = 15,     // chi2 of the pixel-move state
     GPAIR_NSCAL = 16
 };
 

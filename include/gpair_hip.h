@@ -246,6 +246,61 @@ int gpair_batch_pcg_run(gpair_ctx *const *ctxs, int count, const gpair_pcg_param
  * Call it after gpu_build_kernels. */
 int gpair_autotune(gpair_ctx *ctx, int reps, int *chosen, int nchosen);
 
+/* ---- single-pixel moves: incremental visibility updates ------------------
+ * For MCMC / annealing-style samplers.  With U = F V the unnormalised
+ * visibilities of an image of total flux F, changing pixel (x, y) by d gives
+ *   V'_p = (U_p + d X[p][x] Y[p][y]) / (F + d),
+ * the identity behind update_vis_fluxchange (gpair1/trunk/cpu.c:477-486), whose
+ * V' = r V + (1 - r) X Y holds with r = F / (F + d), the ratio of the TOTAL
+ * fluxes before and after.  The pixel state is an image (a copy of the current
+ * image at gpair_pixel_begin, plus the moves applied since), its U in FP64 and
+ * its flux in FP64; proposals are scored against it exactly, in FP64
+ * differences, without a DFT.  Pixels are indexed pix = x + W*y. */
+typedef struct {
+    int pix_a;             /* first pixel, x + W*y */
+    int pix_b;             /* second pixel, or -1 for a one-pixel move (must differ from pix_a) */
+    float flux_a, flux_b;  /* their new fluxes */
+} gpair_pixel_move;
+
+/* no reference counterpart: (re)builds the pixel state from the current image
+ * (one forward DFT); any later change of the current image by other calls is not
+ * seen until the next gpair_pixel_begin.  The calls below build it when absent;
+ * gpu_copy_image, gpu_update_image, the CG loops and new tables drop it. */
+int gpair_pixel_begin(gpair_ctx *ctx);
+/* gpu_update_vis_fluxchange, gpair1/trunk/gpu.c:2137-2182 (kernel
+ * gpair1/branches/cpu_dev/kernel_vis_update_fluxchange.cl:1-40): the trial
+ * visibilities (the reference's visi1) of the pixel state with pixel (x, y) set
+ * to new_pixel_flux; the state is not changed.  Differs from the reference as
+ * the identity above requires: r is the ratio of total fluxes (the reference
+ * divides pixel fluxes), the y table is indexed by y (the reference kernel uses
+ * x_pos for both), and the pixel is read on the device (the reference reads it
+ * back with a byte offset equal to the pixel index).  data_alloc_uv is unused. */
+int gpu_update_vis_fluxchange(gpair_ctx *ctx, int x, int y, float new_pixel_flux, int image_width, int nuv,
+                              int data_alloc_uv);
+/* update_vis_positionchange, gpair1/trunk/cpu.c:488-497: trial visibilities with
+ * flux moved from (x_old, y_old) to (x_new, y_new); the total flux is unchanged,
+ * so V' = V + (flux / F) (X_new Y_new - X_old Y_old) */
+int gpair_update_vis_positionchange(gpair_ctx *ctx, int x_old, int y_old, int x_new, int y_new, float flux,
+                                    int image_width, int nuv);
+/* gpu_new_chi2, gpair1/trunk/gpu.c:2067-2071: chi2 of the trial visibilities
+ * (vis2data + data2chi2 on visi1); its mock data stay apart from the current
+ * image's.  data_alloc is unused. */
+int gpu_new_chi2(gpair_ctx *ctx, int nuv, int npow, int nbis, int data_alloc, double *chi2);
+/* copy of the trial visibilities, float2 [nuv] */
+int gpair_get_trial_vis(gpair_ctx *ctx, float *vis, int nuv);
+/* no reference counterpart: scores `count` (<= 4096) independent moves against
+ * the pixel state in a few launches: dchi2[k] = chi2(state after move k) -
+ * chi2(state), dentropy[k] = S(after) - S(before), both FP64 (either may be NULL).
+ * A sampler's dJ is dchi2 - alpha dentropy. */
+int gpair_pixel_propose(gpair_ctx *ctx, const gpair_pixel_move *moves, int count, double *dchi2,
+                        double *dentropy);
+/* no reference counterpart: applies `count` (<= 4096) moves in order to the
+ * pixel state and writes their pixels into the current image too */
+int gpair_pixel_apply(gpair_ctx *ctx, const gpair_pixel_move *moves, int count);
+/* no reference counterpart: chi2 (from the FP64 state visibilities), entropy and
+ * flux of the pixel state */
+int gpair_pixel_state(gpair_ctx *ctx, double *chi2, double *entropy, double *flux);
+
 /* ---- host-side data preparation (no device needed) ----------------------- */
 /* set_model, gpair1/trunk/cpu.c:645-749: prior image [W*W] of total flux `flux`.
  * type 0 flat, 1 centred Dirac, 2 uniform disk (radius width mas), 3 Gaussian

@@ -733,3 +733,90 @@ int orc_pcg(int W, int nuv, int npow, int nbis, const double *au, const double *
     free(g); free(g_old); free(z); free(z_old); free(d); free(tmp); free(UI); free(Ud); free(vt); free(mt);
     return rows;
 }
+
+/* ---------------------------------------------------------------------------
+ * Single-pixel moves: update_vis_fluxchange (gpair1/trunk/cpu.c:477-486) and
+ * update_vis_positionchange (gpair1/trunk/cpu.c:488-497), restated through the
+ * identity they rest on.  With U = F V the unnormalised visibilities of an image
+ * of total flux F, changing pixel (x, y) by d gives
+ *   V'_p = (F V_p + d X[p][x] Y[p][y]) / (F + d),
+ * the reference's V' = r V + (1 - r) X Y with r = F / (F + d): the ratio of the
+ * TOTAL fluxes before and after (cpu.c:480-481 leave the total to the caller).
+ * A position change moves flux f between two pixels: F is unchanged and
+ * V' = V + (f / F) (X_new Y_new - X_old Y_old).  Table entries are evaluated
+ * from au, av here (turn_exp), not read from tables.
+ * ------------------------------------------------------------------------- */
+/* npx pixel changes d[j] at pix[j] = x + W*y; vis, vis_new: 2*nuv doubles;
+ * returns the new flux */
+double orc_px_update(int W, int nuv, const double *au, const double *av, const double *vis, double F, int npx,
+                     const int *pix, const double *d, double *vis_new)
+{
+    double Fn = F;
+    for (int j = 0; j < npx; j++)
+        Fn += d[j];
+#pragma omp parallel for schedule(static)
+    for (int p = 0; p < nuv; p++) {
+        double complex acc = F * (vis[2 * p] + I * vis[2 * p + 1]);
+        for (int j = 0; j < npx; j++) {
+            const int x = pix[j] % W, y = pix[j] / W;
+            acc += d[j] * turn_exp(au[p] * (double)x) * turn_exp(-av[p] * (double)y);
+        }
+        acc /= Fn;
+        vis_new[2 * p] = creal(acc);
+        vis_new[2 * p + 1] = cimag(acc);
+    }
+    return Fn;
+}
+
+static double px_pix_entropy(double t, double M)
+{
+    return (t > 0.0 && M > 0.0) ? (t - M - t * log(t / M)) : -M;
+}
+
+/* Scores `count` independent moves against an image (img, its vis and flux F):
+ * move k sets pixel pix_a[k] to flux_a[k] and, when pix_b[k] >= 0, pixel
+ * pix_b[k] to flux_b[k].  dchi2[k] = chi2 after - chi2 before (vis2data +
+ * data2chi2 of the updated visibilities), dS[k] = entropy after - before. */
+void orc_px_propose(int W, int nuv, int npow, int nbis, const double *au, const double *av, const double *data,
+                    const double *err, const int *uvpnt, const int *sign, const double *phasor,
+                    const double *model, const double *img, const double *vis, double F, int count,
+                    const int *pix_a, const int *pix_b, const double *flux_a, const double *flux_b, double *dchi2,
+                    double *dS)
+{
+    const int nd = npow + 2 * nbis;
+    double *mock0 = malloc(sizeof(double) * (size_t)nd);
+    orc_vis2data(npow, nbis, vis, uvpnt, sign, phasor, mock0);
+    const double chi0 = orc_data2chi2(nd, data, err, mock0);
+    free(mock0);
+#pragma omp parallel
+    {
+        double *vn = malloc(sizeof(double) * 2 * (size_t)nuv);
+        double *mk = malloc(sizeof(double) * (size_t)nd);
+        double complex *ca = malloc(sizeof(double complex) * (size_t)nuv);
+#pragma omp for schedule(dynamic)
+        for (int k = 0; k < count; k++) {
+            int pix[2] = {pix_a[k], pix_b[k]};
+            double d[2] = {flux_a[k] - img[pix_a[k]], pix_b[k] >= 0 ? flux_b[k] - img[pix_b[k]] : 0.0};
+            const int npx = pix_b[k] >= 0 ? 2 : 1;
+            double Fn = F + d[0] + (npx > 1 ? d[1] : 0.0);
+            for (int p = 0; p < nuv; p++) {
+                double complex acc = F * (vis[2 * p] + I * vis[2 * p + 1]);
+                for (int j = 0; j < npx; j++)
+                    acc += d[j] * turn_exp(au[p] * (double)(pix[j] % W)) * turn_exp(-av[p] * (double)(pix[j] / W));
+                ca[p] = acc / Fn;
+                vn[2 * p] = creal(ca[p]);
+                vn[2 * p + 1] = cimag(ca[p]);
+            }
+            orc_vis2data(npow, nbis, vn, uvpnt, sign, phasor, mk);
+            dchi2[k] = orc_data2chi2(nd, data, err, mk) - chi0;
+            double s = 0.0;
+            for (int j = 0; j < npx; j++)
+                s += px_pix_entropy(j == 0 ? flux_a[k] : flux_b[k], model[pix[j]]) -
+                     px_pix_entropy(img[pix[j]], model[pix[j]]);
+            dS[k] = s;
+        }
+        free(vn);
+        free(mk);
+        free(ca);
+    }
+}

@@ -73,6 +73,10 @@ class Oracle:
         L.orc_pcg.restype = C.c_int
         L.orc_dft_raw.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp]
         L.orc_parabola_step.argtypes = [C.c_double] * 6
+        L.orc_px_update.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int, _ip, _dp, _dp]
+        L.orc_px_update.restype = C.c_double
+        L.orc_px_propose.argtypes = [C.c_int] * 4 + [_dp, _dp, _dp, _dp, _ip, _ip, _dp, _dp, _dp, _dp,
+                                                     C.c_double, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp]
         L.orc_parabola_step.restype = C.c_double
         L.orc_num_threads.restype = C.c_int
 
@@ -175,6 +179,27 @@ class Oracle:
                               None if pf is None else pf.ctypes.data, int(max_iter), float(chi2r_target), img,
                               trace.ctypes.data)
         return img, trace[:rows]
+
+    def px_update(self, P, vis, F, pix, d):
+        """visibilities and flux after pixel changes d[j] at pix[j] (x + W*y)"""
+        pix = _i32(np.atleast_1d(pix))
+        d = _c64(np.atleast_1d(d))
+        out = np.empty(2 * P["nuv"])
+        Fn = self.L.orc_px_update(P["W"], P["nuv"], P["au"], P["av"],
+                                  _c64(np.asarray(vis, np.complex128).view(np.float64)), float(F), pix.size, pix, d,
+                                  out)
+        return out.view(np.complex128), Fn
+
+    def px_propose(self, P, img, vis, F, pix_a, pix_b, flux_a, flux_b):
+        """dchi2, dS of each move (pixels pix_a / pix_b set to flux_a / flux_b; pix_b = -1: none)"""
+        pix_a, pix_b = _i32(pix_a), _i32(pix_b)
+        k = pix_a.size
+        dchi2, dS = np.empty(k), np.empty(k)
+        self.L.orc_px_propose(P["W"], P["nuv"], P["npow"], P["nbis"], P["au"], P["av"], P["data"], P["err"],
+                              P["uvpnt"], P["sign"], P["phasor"], P["model"], _c64(img),
+                              _c64(np.asarray(vis, np.complex128).view(np.float64)), float(F), k, pix_a, pix_b,
+                              _c64(flux_a), _c64(flux_b), dchi2, dS)
+        return dchi2, dS
 
     def dft_raw(self, W, au, av, img):
         out = np.empty(2 * au.size)

@@ -348,3 +348,75 @@ def test_pcg_oracle_descends_and_tracks_the_objective(orc):
     # chi2 and entropy columns match the objective's parts at the first row
     J0, chi20, S0, *_ = orc.objective(P, prob.start.astype(np.float64), alpha)
     assert abs(tr[0, 1] - chi20) <= 1e-12 * chi20 and abs(tr[0, 2] - S0) <= 1e-9 * max(abs(S0), 1.0)
+
+
+def test_pixel_update_is_the_dft_of_the_changed_image(orc):
+    """orc_px_update (update_vis_fluxchange / update_vis_positionchange,
+    gpair1/trunk/cpu.c:477-497): the visibilities after a one- or two-pixel change
+    equal the DFT of the changed image, the new flux is the changed image's flux,
+    and a one-pixel change has the reference's form r V + (1 - r) X Y with r the
+    ratio of TOTAL fluxes.  The reference kernel's pixel-flux ratio and its y
+    index taken from x_pos (kernel_vis_update_fluxchange.cl:27-28) both fail."""
+    prob = _small(11)
+    P = orc.unpack(prob)
+    W = prob.W
+    rng = np.random.default_rng(11)
+    img = rng.random(W * W) + 0.05
+    vis, F = orc.image2vis(W, P["au"], P["av"], img)
+    x, y, new = 13, 29, 2.5
+    pix = x + W * y
+    d = new - img[pix]
+    changed = img.copy()
+    changed[pix] = new
+    ref, Fref = orc.image2vis(W, P["au"], P["av"], changed)
+    got, Fn = orc.px_update(P, vis, F, pix, d)
+    assert np.abs(got - ref).max() < 1e-12 and abs(Fn - Fref) < 1e-12 * Fref
+    XY = np.exp(2j * math.pi * (P["au"] * x - P["av"] * y))
+    r = F / (F + d)
+    assert np.abs(r * vis + (1 - r) * XY - ref).max() < 1e-12
+    # planted mistakes: the ratio of pixel fluxes, the y table indexed by x
+    r_pix = img[pix] / new
+    assert np.abs(r_pix * vis + (1 - r_pix) * XY - ref).max() > 1e-3
+    XX = np.exp(2j * math.pi * (P["au"] * x - P["av"] * x))
+    assert np.abs(r * vis + (1 - r) * XX - ref).max() > 1e-3
+    # position change: flux f from (3, 4) to (50, 7), total flux unchanged
+    f, po, pn = 0.3, 3 + W * 4, 50 + W * 7
+    moved = img.copy()
+    moved[po] -= f
+    moved[pn] += f
+    ref2, _ = orc.image2vis(W, P["au"], P["av"], moved)
+    got2, F2 = orc.px_update(P, vis, F, [po, pn], [-f, f])
+    assert np.abs(got2 - ref2).max() < 1e-12 and abs(F2 - F) < 1e-12 * F
+    e_old = np.exp(2j * math.pi * (P["au"] * 3 - P["av"] * 4))
+    e_new = np.exp(2j * math.pi * (P["au"] * 50 - P["av"] * 7))
+    assert np.abs(vis + f / F * (e_new - e_old) - ref2).max() < 1e-12
+
+
+def test_pixel_propose_scores_moves_by_chi2_and_entropy_differences(orc):
+    """orc_px_propose: per move, chi2 and entropy of the changed image minus those
+    of the image, each recomputed from scratch (full DFT); a move that keeps the
+    pixel value scores exactly zero."""
+    prob = _small(12)
+    P = orc.unpack(prob)
+    W = prob.W
+    rng = np.random.default_rng(12)
+    img = prob.start.astype(np.float64) * (0.5 + rng.random(W * W))
+    vis, F = orc.image2vis(W, P["au"], P["av"], img)
+    K = 10
+    pa = rng.choice(W * W, K, replace=False).astype(np.int32)
+    pb = np.where(np.arange(K) % 3 == 0, (pa + 1 + rng.integers(0, W * W - 1, K)) % (W * W), -1).astype(np.int32)
+    fa = img[pa] * (1 + rng.normal(0, 2, K)) ** 2
+    fb = np.where(pb >= 0, img[np.maximum(pb, 0)] * 3.0, 0.0)
+    fa[1] = img[pa[1]]
+    pb[1] = -1
+    dchi2, dS = orc.px_propose(P, img, vis, F, pa, pb, fa, fb)
+    _, chi0, S0, *_ = orc.objective(P, img, 0.0)
+    for k in range(K):
+        new = img.copy()
+        new[pa[k]] = fa[k]
+        if pb[k] >= 0:
+            new[pb[k]] = fb[k]
+        _, chi, S, *_ = orc.objective(P, new, 0.0)
+        assert abs(dchi2[k] - (chi - chi0)) <= 1e-9 * chi0
+        assert abs(dS[k] - (S - S0)) <= 1e-9 * max(abs(S0), 1.0)
+    assert dchi2[1] == 0.0 and dS[1] == 0.0
