@@ -19,6 +19,11 @@ timed region.  A step is one CG iteration.
 * cpu_baseline: the reference's own CPU model (oracle/_ref) on one core; the
   same leg re-evaluates each final image's objective with the FP64 oracle
   (err_vs_fp64).
+* batch_cfg4: many independent reconstructions of uneven sizes, sequential
+  against gpair_batch_pcg_run.
+* pixel_moves: single-pixel moves scored per second (gpair_pixel_propose, --moves
+  per call) against a full forward DFT + chi2 per move, with the kernels' own
+  rooflines and the FP64 port on the host cores.
 Prints one JSON line on stdout.
 """
 from __future__ import annotations
@@ -61,6 +66,8 @@ def parse():
                     help="further seeds of the headline configuration run to the target with BF16x3 "
                          "(the iteration count to chi2_r = 1 varies with the trajectory)")
     ap.add_argument("--batch-iters", type=int, default=100)
+    ap.add_argument("--moves", type=int, default=4096, help="single-pixel moves scored per call (0: skip)")
+    ap.add_argument("--move-calls", type=int, default=20, help="timed gpair_pixel_propose calls")
     ap.add_argument("--cpu-kind", default="auto", choices=["auto", "reference", "port"])
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 256 MiB device copy (HBM counter calibration for rocprofv3 --pmc)")
@@ -197,6 +204,87 @@ def batch_workload(a):
             "t3_range": [min(p.nbis for p in probs), max(p.nbis for p in probs)],
             "sequential_s": t_seq, "batch_s": t_batch, "iters_per_s_sequential": its_seq / t_seq,
             "iters_per_s_batch": its_b / t_batch, "speedup": t_seq / t_batch, "setup_generation_s": t_gen}
+
+
+def pixel_workload(a, prob, prec, cpu=True):
+    """Single-pixel moves on the headline problem: moves scored per second by
+    gpair_pixel_propose (a.moves moves per call, FP64 chi2 and entropy changes
+    against the pixel state), against scoring each move by a full forward DFT and
+    chi2 of the changed image on the GPU, and the FP64 port of the reference's
+    incremental update (update_vis_fluxchange + vis2data + data2chi2) on the host."""
+    from gpair_amd.api import PIXEL_MOVE, GpairContext
+    W, nuv, npow, nbis = prob.W, prob.nuv, prob.npow, prob.nbis
+    K = a.moves
+    rng = np.random.default_rng(17)
+    ctx = GpairContext.from_problem(prob)
+    ctx.set_precision(prec)
+    L = ctx.L
+    img = ctx.get("ci")
+    m = np.zeros(K, PIXEL_MOVE)
+    m["pix_a"] = rng.integers(0, W * W, K)
+    m["pix_b"] = -1
+    m["flux_a"] = img[m["pix_a"]] * np.exp(rng.normal(0.0, 1.0, K)).astype(np.float32)
+    two = np.arange(K) % 4 == 0   # every fourth move: two pixels
+    m["pix_b"][two] = (m["pix_a"][two] + 1 + rng.integers(0, W * W - 1, two.sum())) % (W * W)
+    m["flux_b"][two] = img[m["pix_b"][two]] * 1.5
+    t0 = time.perf_counter()
+    ctx.pixel_begin()
+    L.gpair_sync(ctx.h)
+    t_begin = time.perf_counter() - t0
+    ctx.pixel_propose(m)   # warm-up: buffers sized for K moves
+    L.gpair_timing_enable(ctx.h, 1)
+    t0 = time.perf_counter()
+    for _ in range(a.move_calls):
+        d, s = ctx.pixel_propose(m)
+    dt = time.perf_counter() - t0
+    tk = {}
+    for k, name in ((2, "k_px_delta"), (3, "k_px_dchi2")):
+        cnt, tot = C.c_int(), C.c_double()
+        L.gpair_timing_read(ctx.h, k, C.byref(cnt), C.byref(tot))
+        tk[name] = tot.value / max(cnt.value, 1) * 1e3   # us per launch
+    L.gpair_timing_enable(ctx.h, 0)
+    # the alternative on the GPU: one forward DFT + chi2 (+ read-back) per move
+    R = 50
+    ctx.gpu_get_chi2_curr()
+    t0 = time.perf_counter()
+    for _ in range(R):
+        ctx.gpu_get_chi2_curr()
+    t_full = (time.perf_counter() - t0) / R
+    vis_f = ctx.vis().astype(np.complex128)
+    ctx.close()
+    kpad = (K + 7) // 8 * 8
+    # k_px_delta: reads two table columns (2 x 8 B) and writes one float2 per uv point and move
+    bytes_delta = 24.0 * nuv * kpad
+    # k_px_dchi2: FP64 operations per move, about 11 per V2 and 36 per T3 datum
+    flop_dchi2 = (11.0 * npow + 36.0 * nbis) * kpad
+    out = {"workload": f"{K} moves per call (a quarter of them two-pixel) on {W}x{W} px, {npow} V2 + {nbis} T3",
+           "moves_per_s": K * a.move_calls / dt, "us_per_call": dt / a.move_calls * 1e6,
+           "state_build_ms": t_begin * 1e3,
+           "full_dft_moves_per_s": 1.0 / t_full, "speedup_vs_full_dft": K * a.move_calls / dt * t_full,
+           "k_px_delta": {"avg_us": tk["k_px_delta"], "bytes_per_launch": bytes_delta, "bound": "hbm",
+                          "achieved_GBps": bytes_delta / (tk["k_px_delta"] * 1e-6) / 1e9,
+                          "frac": bytes_delta / (tk["k_px_delta"] * 1e-6) / HBM_PEAK},
+           "k_px_dchi2": {"avg_us": tk["k_px_dchi2"], "fp64_flop_per_launch": flop_dchi2, "bound": "fp64 valu",
+                          "achieved_tflops": flop_dchi2 / (tk["k_px_dchi2"] * 1e-6) / 1e12,
+                          "frac": flop_dchi2 / (tk["k_px_dchi2"] * 1e-6) / 78.6e12}}
+    print(f"# pixel moves: {out['moves_per_s']:.3g} moves/s ({out['us_per_call']:.0f} us per {K}), "
+          f"k_px_delta {tk['k_px_delta']:.1f} us, k_px_dchi2 {tk['k_px_dchi2']:.1f} us; full DFT "
+          f"{1.0 / t_full:.0f} moves/s", file=sys.stderr, flush=True)
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        orc = O.Oracle()
+        P = orc.unpack(prob)
+        ks = min(K, 2048)
+        img64 = img.astype(np.float64)
+        t0 = time.perf_counter()
+        orc.px_propose(P, img64, vis_f, img64.sum(), m["pix_a"][:ks], m["pix_b"][:ks],
+                       m["flux_a"][:ks].astype(np.float64), m["flux_b"][:ks].astype(np.float64))
+        tc = time.perf_counter() - t0
+        out["cpu"] = {"value": ks / tc, "unit": "moves/s", "cores": orc.threads(), "kind": "port",
+                      "sample": f"{ks} moves of the same set, FP64 oracle (incremental update from the table "
+                                f"identity, vis2data, data2chi2 per move), {orc.threads()} OpenMP threads"}
+    return out
 
 
 def timing(L, h):
@@ -395,6 +483,8 @@ def main():
         tt["speedup_vs_plain_fp32"] = ((">=" if not conv else "") +
                                        f"{tt['plain_fp32']['wall_s'] / best:.1f}")
         out["to_target"] = tt
+    if a.moves > 0 and a.move_calls > 0:
+        out["pixel_moves"] = pixel_workload(a, prob, prec, cpu=not a.no_cpu)
     if a.batch > 0:
         out["batch_cfg4"] = batch_workload(a)
     if not a.no_cpu:

@@ -11,7 +11,8 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgpair_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "gpair_hip.h")
-HEADERS = [HEADER, os.path.join(os.path.dirname(HERE), "include", "gpair_nn.h")]
+HEADERS = [HEADER, os.path.join(os.path.dirname(HERE), "include", "gpair_nn.h"),
+           os.path.join(os.path.dirname(HERE), "include", "gpair_io.h")]
 
 _vp = C.c_void_p
 _i = C.c_int
@@ -44,6 +45,19 @@ class PcgResult(C.Structure):
 
 
 # name -> (restype, argtypes)
+class OiData(C.Structure):
+    """gpair_oidata (include/gpair_io.h)"""
+    _fields_ = [("npow", C.c_int), ("nbis", C.c_int), ("nuv", C.c_int),
+                ("pow", C.POINTER(C.c_float)), ("powerr", C.POINTER(C.c_float)),
+                ("bisamp", C.POINTER(C.c_float)), ("bisamperr", C.POINTER(C.c_float)),
+                ("bisphs", C.POINTER(C.c_float)), ("bisphserr", C.POINTER(C.c_float)),
+                ("u", C.POINTER(C.c_float)), ("v", C.POINTER(C.c_float)),
+                ("wavelength", C.POINTER(C.c_float)), ("bandwidth", C.POINTER(C.c_float)),
+                ("uvpnt", C.POINTER(C.c_int64)), ("sign", C.POINTER(C.c_int16)),
+                ("target_id", C.c_int), ("minband", C.c_float), ("maxband", C.c_float),
+                ("uv_compares", C.c_longlong)]
+
+
 SIGNATURES = {
     "gpair_strerror": (C.c_char_p, [_i]),
     "gpair_last_error": (C.c_char_p, [_vp]),
@@ -104,6 +118,15 @@ SIGNATURES = {
     "gpair_pixel_propose": (_i, [_vp, _vp, _i, _vp, _vp]),
     "gpair_pixel_apply": (_i, [_vp, _vp, _i]),
     "gpair_pixel_state": (_i, [_vp, _pd, _pd, _pd]),
+    # data I/O (include/gpair_io.h)
+    "gpair_io_last_error": (C.c_char_p, []),
+    "gpair_oifits_targets": (_i, [C.c_char_p, _vp, _vp, _i, _pi]),
+    "gpair_oifits_read": (_i, [C.c_char_p, _i, _f, _f, C.POINTER(OiData)]),
+    "gpair_oidata_free": (None, [C.POINTER(OiData)]),
+    "gpair_oifits_write": (_i, [C.c_char_p, C.c_char_p, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _i,
+                                _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gpair_fits_read_image": (_i, [C.c_char_p, _vp, C.c_long, _pi, _pi]),
+    "gpair_fits_write_image": (_i, [C.c_char_p, _vp, _i, _i, _i, _f, C.c_char_p, C.c_char_p]),
     "gpair_set_model": (_i, [_vp, _i, C.c_float, _i, C.c_float, C.c_float, _i]),
     "gpair_init_data": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _pi]),
     # learned start (include/gpair_nn.h)

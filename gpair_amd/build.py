@@ -39,7 +39,7 @@ def build(verbose: bool = True) -> str:
         obj = os.path.join(BUILD, src + ".o")
         _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    for src in ("gpair_cg.c", "gpair_host.c"):
+    for src in ("gpair_cg.c", "gpair_host.c", "gpair_io.c"):
         obj = os.path.join(BUILD, src + ".o")
         _run(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

@@ -146,15 +146,24 @@ __device__ inline void reduce_finish(double (&v)[NV], double *part, unsigned *ti
 // bit of it.  item(i, v) adds item i into v[0..NV-1].
 #define RED_PARTS 512
 
+__host__ __device__ inline int red_nparts(int n, int span)
+{
+    int nparts = (n + span - 1) / span;
+    return nparts < 1 ? 1 : (nparts > RED_PARTS ? RED_PARTS : nparts);
+}
+
+// partitioned_reduce_at: the same with partitions of at least `span` items, and
+// this block's index bx among the nbx blocks that share the ticket given
+// explicitly (for remapped launches).
 template <int NV, typename F>
-__device__ inline void partitioned_reduce(int n, F item, double *part, unsigned *ticket, double *out)
+__device__ inline void partitioned_reduce_at(int n, int span_min, F item, double *part, unsigned *ticket,
+                                             double *out, int bx, int nbx)
 {
     __shared__ double sh[NV][RED_THREADS / 64];
     __shared__ int is_last;
-    int nparts = (n + 2 * RED_THREADS - 1) / (2 * RED_THREADS);
-    nparts = nparts < 1 ? 1 : (nparts > RED_PARTS ? RED_PARTS : nparts);
+    const int nparts = red_nparts(n, span_min);
     const int span = (n + nparts - 1) / nparts;
-    for (int q = blockIdx.x; q < nparts; q += gridDim.x) {
+    for (int q = bx; q < nparts; q += nbx) {
         double v[NV];
 #pragma unroll
         for (int j = 0; j < NV; j++)
@@ -171,7 +180,7 @@ __device__ inline void partitioned_reduce(int n, F item, double *part, unsigned 
     }
     if (threadIdx.x == 0) {
         const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (t == gridDim.x - 1);
+        is_last = (t == (unsigned)nbx - 1);
     }
     __syncthreads();
     if (!is_last)
@@ -192,6 +201,12 @@ __device__ inline void partitioned_reduce(int n, F item, double *part, unsigned 
             out[j] = acc[j];
         __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+template <int NV, typename F>
+__device__ inline void partitioned_reduce(int n, F item, double *part, unsigned *ticket, double *out)
+{
+    partitioned_reduce_at<NV>(n, 2 * RED_THREADS, item, part, ticket, out, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // ---------------------------------------------------------------------------

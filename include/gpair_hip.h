@@ -176,9 +176,15 @@ int gpair_cg_run(gpair_ctx *ctx, const gpair_cg_params *params, gpair_cg_result 
 
 /* ---- measurement (no reference counterpart; the reference queue has no
  * profiling, gpair1/trunk/gpu.c:1724) ------------------------------------ */
-enum { GPAIR_TK_FORWARD = 0, GPAIR_TK_ADJOINT = 1, GPAIR_TK_COUNT = 2 };
-/* start (on != 0) or stop recording HIP events around every forward / adjoint
- * DFT launch on the context stream; enabling clears what was recorded */
+enum {
+    GPAIR_TK_FORWARD = 0,
+    GPAIR_TK_ADJOINT = 1,
+    GPAIR_TK_PX_DELTA = 2,   /* per-move changes of U (gpair_pixel_propose) */
+    GPAIR_TK_PX_DCHI2 = 3,   /* per-move chi2 changes (gpair_pixel_propose) */
+    GPAIR_TK_COUNT = 4
+};
+/* start (on != 0) or stop recording HIP events around every launch of the
+ * kernels above on the context stream; enabling clears what was recorded */
 int gpair_timing_enable(gpair_ctx *ctx, int on);
 /* number of timed launches of a kernel and their summed duration (ms) */
 int gpair_timing_read(gpair_ctx *ctx, int kernel, int *count, double *total_ms);
