@@ -109,6 +109,9 @@ SIGNATURES = {
     "gpair_set_pfactor": (_i, [_vp, _vp, _i]),
     "gpair_autotune": (_i, [_vp, _i, _vp, _i]),
     "gpair_set_graphs": (_i, [_vp, _i]),
+    "gpair_pcg_run_checkpointed": (_i, [_vp, C.POINTER(PcgParams), C.POINTER(PcgResult), _vp, _i, C.c_char_p,
+                                        C.c_char_p]),
+    "gpair_set_snapshot": (_i, [_vp, C.c_char_p, _i, _f]),
     "gpair_batch_pcg_run": (_i, [_vp, _i, _vp, _vp]),
     "gpair_pixel_begin": (_i, [_vp]),
     "gpu_update_vis_fluxchange": (_i, [_vp, _i, _i, _f, _i, _i, _i]),

@@ -310,6 +310,26 @@ class GpairContext:
         self._check(rc, "gpair_pcg_run")
         return res, trace[:min(res.iterations, trace.shape[0])]
 
+    def pcg_run_checkpointed(self, alpha, precond=1, max_iter=1000, chi2r_target=1.0, resume_from=None,
+                             save_to=None, verbose=False):
+        """gpair_pcg_run_checkpointed: resume from / save to a checkpoint file"""
+        p = _lib.PcgParams()
+        self.L.gpair_pcg_default_params(C.byref(p))
+        p.alpha, p.precond, p.max_iter = float(alpha), int(precond), int(max_iter)
+        p.chi2r_target, p.verbose = float(chi2r_target), int(verbose)
+        res = _lib.PcgResult()
+        trace = np.zeros((max(max_iter, 1), 8))
+        rc = self.L.gpair_pcg_run_checkpointed(self.h, C.byref(p), C.byref(res), _ptr(trace), trace.shape[0],
+                                               None if resume_from is None else str(resume_from).encode(),
+                                               None if save_to is None else str(save_to).encode())
+        self._check(rc, "gpair_pcg_run_checkpointed")
+        return res, trace[:min(res.iterations, trace.shape[0])]
+
+    def set_snapshot(self, path, every=5, pixsize=0.0):
+        """the loops write the current image to `path` every `every` iterations (None: off)"""
+        self._check(self.L.gpair_set_snapshot(self.h, None if path is None else str(path).encode(), int(every),
+                                              float(pixsize)), "gpair_set_snapshot")
+
     def set_graphs(self, on=True):
         self._check(self.L.gpair_set_graphs(self.h, int(bool(on))), "gpair_set_graphs")
 

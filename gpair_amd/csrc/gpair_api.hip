@@ -12,6 +12,8 @@
 #include <vector>
 
 #include "../../include/gpair_hip.h"
+#include "../../include/gpair_io.h"
+#include "gpair_io_internal.h"
 #include "gpair_kernels.hip"
 
 // gpair1/trunk/getoifits.h:87-88
@@ -78,6 +80,10 @@ struct gpair_ctx {
     gpair_pixel_move *px_staged;    // pinned, px_cap
     hipEvent_t px_ev;               // last upload from px_staged
     int px_cap, px_valid, px_trial, px_tables, px_n, px_nuv, px_ndof;
+    // snapshots of the current image written by the loops
+    char snap_path[1024];
+    int snap_every;
+    float snap_pix;
     // optional per-launch timing of the DFT kernels (HIP events on the context stream)
     int timing;
     std::vector<hipEvent_t> ev[GPAIR_TK_COUNT];
@@ -1116,6 +1122,39 @@ static inline void swap_ptr(T *&a, T *&b)
     a = b;
     b = t;
 }
+// ---------------------------------------------------------------------------
+// snapshots of the current image (the reference's periodic reconst.fits)
+// ---------------------------------------------------------------------------
+extern "C" int gpair_set_snapshot(gpair_ctx *c, const char *path, int every, float pixsize)
+{
+    if (!c)
+        return GPAIR_E_ARG;
+    if (!path || every <= 0) {
+        c->snap_every = 0;
+        c->snap_path[0] = '\0';
+        return GPAIR_OK;
+    }
+    if (strlen(path) >= sizeof(c->snap_path))
+        return fail(c, GPAIR_E_ARG, "gpair_set_snapshot: path longer than %zu", sizeof(c->snap_path) - 1);
+    strcpy(c->snap_path, path);
+    c->snap_every = every;
+    c->snap_pix = pixsize;
+    return GPAIR_OK;
+}
+
+extern "C" int gpair_snapshot_due(const gpair_ctx *c, int it) { return c && c->snap_every > 0 && it % c->snap_every == 0; }
+
+extern "C" int gpair_snapshot_write(gpair_ctx *c)
+{
+    if (!c || !c->img.p)
+        return GPAIR_E_ARG;
+    std::vector<float> host(c->n);
+    HIPCHK(c, hipMemcpyAsync(host.data(), c->img.p, sizeof(float) * c->n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int rc = gpair_fits_write_image(c->snap_path, host.data(), c->W, c->W, -64, c->snap_pix, nullptr, nullptr);
+    return rc ? fail(c, rc, "snapshot: %s", gpair_io_last_error()) : GPAIR_OK;
+}
+
 #include "gpair_pcg.inc"
 #include "gpair_tune.inc"
 #include "gpair_pixel.inc"

@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include "../../include/gpair_hip.h"
+#include "gpair_io_internal.h"
 
 void gpair_cg_default_params(gpair_cg_params *p)
 {
@@ -75,6 +76,8 @@ int gpair_cg_run(gpair_ctx *ctx, const gpair_cg_params *prm, gpair_cg_result *re
         if (prm->verbose)
             printf("it %d J %.10e chi2r %.6f S %.6e step %.3e beta %.3e evals %d/%d\n", it, J, chi2 / ndof, S, sel,
                    beta, res.criterion_evals, res.grad_evals);
+        if (gpair_snapshot_due(ctx, it))   /* gpair1/trunk/gpair.c:698-702 */
+            TRY(gpair_snapshot_write(ctx));
         if (prm->chi2r_target > 0.0 && chi2 / ndof <= prm->chi2r_target) {
             res.converged = 1;
             break;

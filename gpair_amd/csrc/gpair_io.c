@@ -11,6 +11,7 @@
  */
 #include "../../include/gpair_io.h"
 #include "../../include/gpair_hip.h"
+#include "gpair_io_internal.h"
 
 #include <math.h>
 #include <stdarg.h>
@@ -1281,5 +1282,142 @@ int gpair_fits_write_image(const char *path, const float *image, int width, int 
     free(b.p);
     if (fclose(fp) != 0 || !ok)
         return io_fail(GPAIR_E_ARG, "short write of %s", path);
+    return GPAIR_OK;
+}
+
+/* ------------------------------------------------------------------------
+ * multi-HDU image files (gpair_io_internal.h)
+ * ---------------------------------------------------------------------- */
+
+struct gpair_fitsw {
+    obuf b;
+    int nhdu;
+};
+
+gpair_fitsw *gpair_fitsw_new(void) { return calloc(1, sizeof(gpair_fitsw)); }
+
+void gpair_fitsw_free(gpair_fitsw *w)
+{
+    if (w) {
+        free(w->b.p);
+        free(w);
+    }
+}
+
+int gpair_fitsw_add(gpair_fitsw *w, const char *extname, int bitpix, const void *data, long n1, long n2)
+{
+    if (!w || !extname || !data || n1 < 1 || n2 < 1 || (bitpix != -32 && bitpix != -64 && bitpix != 32))
+        return io_fail(GPAIR_E_ARG, "gpair_fitsw_add: bad arguments");
+    obuf *b = &w->b;
+    int rc = w->nhdu == 0 ? card_log(b, "SIMPLE", 1, NULL) : card_str(b, "XTENSION", "IMAGE", NULL);
+    rc = rc || card_int(b, "BITPIX", bitpix, NULL) || card_int(b, "NAXIS", 2, NULL) ||
+         card_int(b, "NAXIS1", n1, NULL) || card_int(b, "NAXIS2", n2, NULL);
+    if (w->nhdu == 0)
+        rc = rc || card_log(b, "EXTEND", 1, NULL);
+    else
+        rc = rc || card_int(b, "PCOUNT", 0, NULL) || card_int(b, "GCOUNT", 1, NULL);
+    rc = rc || card_str(b, "EXTNAME", extname, NULL) || ob_card(b, "END") || ob_pad(b, ' ');
+    const int nb = bitpix == -64 ? 8 : 4;
+    const long n = n1 * n2;
+    if (!rc && grow((void **)&b->p, 1, &b->cap, b->n + n * nb))
+        rc = 1;
+    if (!rc) {
+        unsigned char *p = b->p + b->n;
+        for (long i = 0; i < n; i++) {
+            uint64_t bits = 0;
+            if (nb == 8)
+                memcpy(&bits, (const double *)data + i, 8);
+            else {
+                uint32_t b32;
+                memcpy(&b32, (const uint32_t *)data + i, 4);
+                bits = b32;
+            }
+            put_be(p, bits, nb);
+            p += nb;
+        }
+        b->n += n * nb;
+        rc = ob_pad(b, 0);
+    }
+    if (rc)
+        return io_fail(GPAIR_E_NOMEM, "gpair_fitsw_add: out of host memory");
+    w->nhdu++;
+    return GPAIR_OK;
+}
+
+int gpair_fitsw_save(gpair_fitsw *w, const char *path)
+{
+    if (!w || !path || w->nhdu == 0)
+        return io_fail(GPAIR_E_ARG, "gpair_fitsw_save: nothing to write");
+    FILE *fp = fopen(path, "wb");
+    if (!fp)
+        return io_fail(GPAIR_E_ARG, "cannot create %s", path);
+    const int ok = fwrite(w->b.p, 1, (size_t)w->b.n, fp) == (size_t)w->b.n;
+    if (fclose(fp) != 0 || !ok)
+        return io_fail(GPAIR_E_ARG, "short write of %s", path);
+    return GPAIR_OK;
+}
+
+struct gpair_fitsr {
+    ffile f;
+};
+
+int gpair_fitsr_open(const char *path, gpair_fitsr **r)
+{
+    if (!path || !r)
+        return io_fail(GPAIR_E_ARG, "gpair_fitsr_open: NULL argument");
+    *r = calloc(1, sizeof(gpair_fitsr));
+    if (!*r)
+        return io_fail(GPAIR_E_NOMEM, "out of host memory");
+    const int rc = ff_open(path, &(*r)->f);
+    if (rc) {
+        free(*r);
+        *r = NULL;
+    }
+    return rc;
+}
+
+void gpair_fitsr_close(gpair_fitsr *r)
+{
+    if (r) {
+        ff_close(&r->f);
+        free(r);
+    }
+}
+
+static const fhdu *named_hdu(const ffile *f, const char *extname)
+{
+    for (int i = 0; i < f->nhdu; i++) {
+        char name[72];
+        if (hdu_string(&f->h[i], "EXTNAME", name, sizeof(name)) == 0 && strcmp(name, extname) == 0)
+            return &f->h[i];
+    }
+    return NULL;
+}
+
+int gpair_fitsr_has(gpair_fitsr *r, const char *extname) { return r && named_hdu(&r->f, extname) != NULL; }
+
+int gpair_fitsr_get(gpair_fitsr *r, const char *extname, int bitpix, void *out, long n1, long n2)
+{
+    if (!r || !extname || !out)
+        return io_fail(GPAIR_E_ARG, "gpair_fitsr_get: NULL argument");
+    const fhdu *h = named_hdu(&r->f, extname);
+    long bp = 0;
+    if (!h)
+        return io_fail(GPAIR_E_ARG, "no %s image in the file", extname);
+    hdu_long(h, "BITPIX", &bp);
+    if (bp != bitpix || h->naxis1 != n1 || h->naxis2 != n2)
+        return io_fail(GPAIR_E_ARG, "%s: BITPIX %ld, %ld x %ld (expected %d, %ld x %ld)", extname, bp, h->naxis1,
+                       h->naxis2, bitpix, n1, n2);
+    const int nb = bitpix == -64 ? 8 : 4;
+    const unsigned char *p = r->f.buf + h->data;
+    for (long i = 0; i < n1 * n2; i++, p += nb) {
+        const uint64_t bits = be_bits(p, nb);
+        if (nb == 8)
+            memcpy((double *)out + i, &bits, 8);
+        else {
+            const uint32_t b32 = (uint32_t)bits;
+            memcpy((uint32_t *)out + i, &b32, 4);
+        }
+    }
     return GPAIR_OK;
 }

@@ -240,6 +240,27 @@ int gpair_set_graphs(gpair_ctx *ctx, int on);
 int gpair_batch_pcg_run(gpair_ctx *const *ctxs, int count, const gpair_pcg_params *params,
                         gpair_pcg_result *results);
 
+/* No reference counterpart (the reference restarts from scratch): the
+ * accelerated loop with a checkpoint.  resume_from (NULL: start from the current
+ * image as gpair_pcg_run) is a file written by save_to of an earlier run on a
+ * context with the same data, sizes, precision, alpha and preconditioner; the
+ * run then continues at the saved iteration, up to params->max_iter iterations
+ * in total, exactly as the uninterrupted run would (bit for bit).  save_to (NULL:
+ * none) receives the state when the run ends at an iteration boundary (iteration
+ * cap or target reached): a FITS file whose primary image is the current image,
+ * with the direction, previous gradient, FP64 and float visibilities, mock
+ * data, preconditioner factor and loop scalars as IMAGE extensions.  Trace rows
+ * are indexed by iteration number. */
+int gpair_pcg_run_checkpointed(gpair_ctx *ctx, const gpair_pcg_params *params, gpair_pcg_result *result,
+                               double *trace, int trace_rows, const char *resume_from, const char *save_to);
+/* The reference writes the current image to reconst.fits every 5 iterations
+ * (gpair1/trunk/gpair.c:698-702, writefits at gpair.c:954-990).  With path set
+ * and every > 0, gpair_cg_run and the accelerated loops write the current image
+ * (FITS, BITPIX -64, PIXSIZE = pixsize mas) at every objective evaluation of an
+ * accepted image whose index is a multiple of `every`; NULL or every <= 0
+ * turns it off.  Each snapshot reads the image back and writes the file. */
+int gpair_set_snapshot(gpair_ctx *ctx, const char *path, int every, float pixsize);
+
 /* Autotuner (no reference counterpart: the reference fixes its work-group sizes
  * when it builds the kernels, gpair1/trunk/gpu.c:413-498).  Times candidate
  * launch shapes of the size-dependent reductions on the context's data and
