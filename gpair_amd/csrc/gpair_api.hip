@@ -53,6 +53,7 @@ struct gpair_ctx {
     // split-precision (BF16x3) operands
     unsigned short *Xh, *Xl, *Ih, *Il;
     unsigned short *Xth, *Xtl;   // x-major split X for the adjoint, points padded to nuvp
+    float2 *fwdp;                // partial sums of the forward's blocks split between two workgroups
     int nuvp;
     int split_valid;
     int adj_b3_nsplit, adj_b3_pper;
@@ -239,7 +240,7 @@ extern "C" int gpu_cleanup(gpair_ctx *c)
     dfree(&c->scal); dfree(&c->invflux); dfree(&c->part); dfree(&c->ticket); dfree(&c->adj_part);
     dfree(&c->Ud); dfree(&c->Udel); dfree(&c->pfac); dfree(&c->ls_out);
     dfree(&c->vis2); dfree(&c->Vd); dfree(&c->Vn); dfree(&c->mock2);
-    dfree(&c->Xh); dfree(&c->Xl); dfree(&c->Ih); dfree(&c->Il); dfree(&c->Xth); dfree(&c->Xtl);
+    dfree(&c->Xh); dfree(&c->Xl); dfree(&c->Ih); dfree(&c->Il); dfree(&c->Xth); dfree(&c->Xtl); dfree(&c->fwdp);
     if (c->host_scal)
         (void)hipHostFree(c->host_scal);
     if (c->host_ls)
@@ -538,7 +539,8 @@ static int ensure_split_tables(gpair_ctx *c)
     const size_t tsz = (size_t)c->nuv * c->W;
     int rc;
     if ((rc = dalloc(c, &c->Xh, 2 * tsz)) || (rc = dalloc(c, &c->Xl, 2 * tsz)) ||
-        (rc = dalloc(c, &c->Ih, (size_t)c->n)) || (rc = dalloc(c, &c->Il, (size_t)c->n)))
+        (rc = dalloc(c, &c->Ih, (size_t)c->n)) || (rc = dalloc(c, &c->Il, (size_t)c->n)) ||
+        (rc = dalloc(c, &c->fwdp, 2 * (size_t)c->nuv)))
         return rc;
     k_split_tables<<<(unsigned)((tsz + 255) / 256), 256, 0, c->stream>>>(c->X, c->nuv, c->W, c->Xh, c->Xl);
     LAUNCHCHK(c);
@@ -678,14 +680,20 @@ static int launch_forward_to(gpair_ctx *c, const float *img, const float *scale,
             return rc;
         k_split_image<<<ew_blocks(c->n), 256, 0, c->stream>>>(img, c->n, c->Ih, c->Il);
         LAUNCHCHK(c);
-        const unsigned g3 = (unsigned)((c->nuv + B3_PW - 1) / B3_PW);
+        // stream-K balance: at most B3_WG workgroups (one per CU at one wave per SIMD),
+        // each taking an equal run of (64-point block, 32-column chunk) units
+        const int nblk = (c->nuv + B3_PW - 1) / B3_PW, nchunk = c->W / B3_KC;
+        const int nwg = nblk < B3_WG ? nblk : B3_WG;
+        const int upw = (nblk * nchunk + nwg - 1) / nwg;
         const int nbk = c->fwd_nbk ? c->fwd_nbk : (c->W < B3_ROWS ? c->W : B3_ROWS) / 32;
         switch (nbk) {
-#define B3_CASE(N)                                                                                          \
-    case N:                                                                                                 \
-        TIMED_LAUNCH(c, GPAIR_TK_FORWARD,                                                                   \
-                     (k_vis_fwd_b3<N><<<g3, 256, 0, c->stream>>>(c->Ih, c->Il, c->Xh, c->Xl, c->Y, c->nuv, c->W, \
-                                                                 scale, vis_out)));                         \
+#define B3_CASE(N)                                                                                             \
+    case N:                                                                                                    \
+        TIMED_LAUNCH(c, GPAIR_TK_FORWARD,                                                                      \
+                     (k_vis_fwd_b3<N><<<nwg, 256, 0, c->stream>>>(c->Ih, c->Il, c->Xh, c->Xl, c->Y, c->nuv, c->W,  \
+                                                                  upw, scale, vis_out, c->fwdp),               \
+                      k_vis_fwd_b3_combine<<<nwg > 1 ? nwg - 1 : 1, B3_PW, 0, c->stream>>>(c->fwdp, c->nuv, c->W, \
+                                                                                         upw, scale, vis_out))); \
         break;
             B3_CASE(2)
             B3_CASE(4)

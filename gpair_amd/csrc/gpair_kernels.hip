@@ -881,6 +881,7 @@ __global__ void k_split_image(const float *__restrict__ img, int n, unsigned sho
 #define B3_KC 32
 #define B3_STRIDE 40   // bf16 per LDS row: 32 + 8 padding, conflict-free 16-byte reads
 #define B3_ROWS 256
+#define B3_WG 256      // stream-K workgroups of the forward (at most)
 
 
 // cooperative copy of one 32-column chunk of the split image rows into registers,
@@ -932,13 +933,21 @@ __device__ __forceinline__ void b3_store(const uint4 &h0, const uint4 &h1, const
     if constexpr (PER > 3) b3_store1<3, CH>(h3, l3, sh, sl, tid);
 }
 
+// Work is balanced over the workgroups stream-K style: the units are (block of 64
+// points, 32-column chunk of x), numbered block-major; workgroup w takes the units
+// [w upw, (w+1) upw).  A block whose chunks are split between two workgroups (upw is
+// at least the chunks per block, so never more than two) gets two partial sums,
+// written to vpart[0] (the part starting at chunk 0) and vpart[1] and added by
+// k_vis_fwd_b3_combine; whole blocks are written to vis directly.  The split points
+// depend on the sizes only, not on NBK, so the rows-per-group choice changes no bit.
 template <int NBK>
 __global__ __launch_bounds__(256) void k_vis_fwd_b3(const unsigned short *__restrict__ Ih,
                                                        const unsigned short *__restrict__ Il,
                                                        const unsigned short *__restrict__ Xh,
                                                        const unsigned short *__restrict__ Xl,
-                                                       const float2 *__restrict__ Y, int nuv, int W,
-                                                       const float *__restrict__ scale, float2 *__restrict__ vis)
+                                                       const float2 *__restrict__ Y, int nuv, int W, int upw,
+                                                       const float *__restrict__ scale, float2 *__restrict__ vis,
+                                                       float2 *__restrict__ vpart)
 {
     constexpr int ROWS = NBK * 32;              // image rows per group
     constexpr int CH = ROWS * (B3_KC / 8);      // 16-byte pieces per array per chunk
@@ -946,141 +955,176 @@ __global__ __launch_bounds__(256) void k_vis_fwd_b3(const unsigned short *__rest
     __shared__ __attribute__((aligned(16))) unsigned short sB[2][2][ROWS * B3_STRIDE];   // [buf][hi/lo]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lr = lane & 31, lh = lane >> 5;
-    const int p0 = blockIdx.x * B3_PW + wave * 16;
-    const int pa = p0 + (lr & 15);               // A row of this lane: point pa, component lr >> 4
-    const bool aval = pa < nuv;
-    const size_t arow = aval ? (size_t)(2 * pa + (lr >> 4)) * W : 0;
-    float vr[8], vi[8];
-#pragma unroll
-    for (int r = 0; r < 8; r++) {
-        vr[r] = 0.f;
-        vi[r] = 0.f;
-    }
     const int nchunk = W / B3_KC;
-    for (int y0 = 0; y0 < W; y0 += ROWS) {
-        v16f acc[NBK];
+    const int nblk = (nuv + B3_PW - 1) / B3_PW;
+    const int total = nblk * nchunk;
+    const int u_end = min(total, (int)(blockIdx.x + 1) * upw);
+    for (int u = (int)blockIdx.x * upw; u < u_end;) {
+        const int blk = u / nchunk, c0 = u % nchunk;
+        const int c1 = min(nchunk, c0 + (u_end - u));
+        u += c1 - c0;
+        const int p0 = blk * B3_PW + wave * 16;
+        const int pa = p0 + (lr & 15);           // A row of this lane: point pa, component lr >> 4
+        const bool aval = pa < nuv;
+        const size_t arow = aval ? (size_t)(2 * pa + (lr >> 4)) * W : 0;
+        float vr[8], vi[8];
 #pragma unroll
-        for (int b = 0; b < NBK; b++)
-#pragma unroll
-            for (int r = 0; r < 16; r++)
-                acc[b][r] = 0.f;
-        static_assert(PER <= 4, "staging holds up to 4 pieces per thread");
-        uint4 h0 = make_uint4(0, 0, 0, 0), h1 = h0, h2 = h0, h3 = h0, l0 = h0, l1 = h0, l2 = h0, l3 = h0;
-        b3_fetch<PER, CH>(h0, h1, h2, h3, l0, l1, l2, l3, Ih, Il, tid, y0, W, 0);
-        __syncthreads();   // the previous group's readers are done with sB
-        b3_store<PER, CH>(h0, h1, h2, h3, l0, l1, l2, l3, sB[0][0], sB[0][1], tid);
-        __syncthreads();
-        // A operands (this lane's X row) one whole chunk ahead: the table streams from
-        // HBM, and with one wave per SIMD only the wave's own MFMA work hides the latency
-        static_assert(B3_KC == 32, "two k-steps per chunk");
-        uint4 a0h = make_uint4(0, 0, 0, 0), a0l = a0h, a1h = a0h, a1l = a0h;
-        if (aval) {
-            a0h = *reinterpret_cast<const uint4 *>(Xh + arow + 8 * lh);
-            a0l = *reinterpret_cast<const uint4 *>(Xl + arow + 8 * lh);
-            a1h = *reinterpret_cast<const uint4 *>(Xh + arow + 16 + 8 * lh);
-            a1l = *reinterpret_cast<const uint4 *>(Xl + arow + 16 + 8 * lh);
+        for (int r = 0; r < 8; r++) {
+            vr[r] = 0.f;
+            vi[r] = 0.f;
         }
-        // One chunk: fetch the next chunk's image rows (L2-resident) and then its X
-        // rows (the HBM stream) into the idle register set, run the MFMAs on the
-        // current set, store the image rows into the idle LDS buffer.  The last chunk
-        // re-fetches itself so that every chunk has the same body; issuing the image
-        // loads before the table loads lets the table loads stay in flight across
-        // the barrier (in-order load counters).  Chunks go in pairs, so the two
-        // register sets swap roles without copies.
-        uint4 b0h = a0h, b0l = a0l, b1h = a1h, b1l = a1l;
-        auto chunk = [&](int c, const uint4 &c0h, const uint4 &c0l, const uint4 &c1h, const uint4 &c1l, uint4 &d0h,
-                         uint4 &d0l, uint4 &d1h, uint4 &d1l) {
-            const int buf = c & 1;
-            const int cn = c + 1 < nchunk ? c + 1 : c;
-            b3_fetch<PER, CH>(h0, h1, h2, h3, l0, l1, l2, l3, Ih, Il, tid, y0, W, cn);
-            __builtin_amdgcn_sched_barrier(0);   // keep the image loads ahead of the table loads
-            {   // unconditional: lanes past nuv read row 0 (arow = 0) and are never stored;
-                // a branch here would make the load counters wait for these loads too
-                const int kn = cn * B3_KC;
-                d0h = *reinterpret_cast<const uint4 *>(Xh + arow + kn + 8 * lh);
-                d0l = *reinterpret_cast<const uint4 *>(Xl + arow + kn + 8 * lh);
-                d1h = *reinterpret_cast<const uint4 *>(Xh + arow + kn + 16 + 8 * lh);
-                d1l = *reinterpret_cast<const uint4 *>(Xl + arow + kn + 16 + 8 * lh);
-            }
-            __builtin_amdgcn_sched_barrier(0);   // and both ahead of this chunk's MFMAs
+        for (int y0 = 0; y0 < W; y0 += ROWS) {
+            v16f acc[NBK];
 #pragma unroll
-            for (int ks = 0; ks < B3_KC / 16; ks++) {
-                const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(ks ? &c1h : &c0h);
-                const bf16x8 al = *reinterpret_cast<const bf16x8 *>(ks ? &c1l : &c0l);
-                // every B operand of the k-step read from LDS before the first MFMA: the
-                // reads overlap each other instead of one LDS latency per tile
-                bf16x8 bh[NBK], bl[NBK];
+            for (int b = 0; b < NBK; b++)
 #pragma unroll
-                for (int b = 0; b < NBK; b++) {
-                    const int o = (b * 32 + lr) * B3_STRIDE + ks * 16 + 8 * lh;
-                    bh[b] = *reinterpret_cast<const bf16x8 *>(&sB[buf][0][o]);
-                    bl[b] = *reinterpret_cast<const bf16x8 *>(&sB[buf][1][o]);
-                }
-                // product-major: consecutive MFMAs update different accumulators, so
-                // none waits for the previous one's result (same per-tile order hh, hl, lh)
-#pragma unroll
-                for (int b = 0; b < NBK; b++)
-                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[b], acc[b], 0, 0, 0);
-#pragma unroll
-                for (int b = 0; b < NBK; b++)
-                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[b], acc[b], 0, 0, 0);
-#pragma unroll
-                for (int b = 0; b < NBK; b++)
-                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[b], acc[b], 0, 0, 0);
-            }
-            // the other buffer was last read in chunk c-1, before the previous barrier
-            b3_store<PER, CH>(h0, h1, h2, h3, l0, l1, l2, l3, sB[buf ^ 1][0], sB[buf ^ 1][1], tid);
+                for (int r = 0; r < 16; r++)
+                    acc[b][r] = 0.f;
+            static_assert(PER <= 4, "staging holds up to 4 pieces per thread");
+            uint4 h0 = make_uint4(0, 0, 0, 0), h1 = h0, h2 = h0, h3 = h0, l0 = h0, l1 = h0, l2 = h0, l3 = h0;
+            b3_fetch<PER, CH>(h0, h1, h2, h3, l0, l1, l2, l3, Ih, Il, tid, y0, W, c0);
+            __syncthreads();   // the previous readers are done with sB
+            b3_store<PER, CH>(h0, h1, h2, h3, l0, l1, l2, l3, sB[c0 & 1][0], sB[c0 & 1][1], tid);
             __syncthreads();
-        };
-        static_assert(B3_KC == 32, "W / 32 chunks, an even count for W a multiple of 64");
-        for (int c = 0; c < nchunk; c += 2) {
-            chunk(c, a0h, a0l, a1h, a1l, b0h, b0l, b1h, b1l);
-            chunk(c + 1, b0h, b0l, b1h, b1l, a0h, a0l, a1h, a1l);
-        }
-        // epilogue of this row group: point 8(r/4) + 4 lh + r%4 (r < 8) has Re T in
-        // register r and Im T in register r+8, column y = y0 + 32 b + lr
+            // A operands (this lane's X row) one whole chunk ahead: the table streams from
+            // HBM, and with one wave per SIMD only the wave's own MFMA work hides the latency
+            static_assert(B3_KC == 32, "two k-steps per chunk");
+            uint4 a0h = make_uint4(0, 0, 0, 0), a0l = a0h, a1h = a0h, a1l = a0h;
+            if (aval) {
+                const int k0 = c0 * B3_KC;
+                a0h = *reinterpret_cast<const uint4 *>(Xh + arow + k0 + 8 * lh);
+                a0l = *reinterpret_cast<const uint4 *>(Xl + arow + k0 + 8 * lh);
+                a1h = *reinterpret_cast<const uint4 *>(Xh + arow + k0 + 16 + 8 * lh);
+                a1l = *reinterpret_cast<const uint4 *>(Xl + arow + k0 + 16 + 8 * lh);
+            }
+            // One chunk: fetch the next chunk's image rows (L2-resident) and then its X
+            // rows (the HBM stream) into the idle register set, run the MFMAs on the
+            // current set, store the image rows into the idle LDS buffer.  The last chunk
+            // of the range re-fetches itself so that every chunk has the same body; issuing
+            // the image loads before the table loads lets the table loads stay in flight
+            // across the barrier (in-order load counters).  Chunks go in pairs, so the two
+            // register sets swap roles without copies.
+            uint4 b0h = a0h, b0l = a0l, b1h = a1h, b1l = a1l;
+            auto chunk = [&](int c, const uint4 &c0h, const uint4 &c0l, const uint4 &c1h, const uint4 &c1l,
+                             uint4 &d0h, uint4 &d0l, uint4 &d1h, uint4 &d1l) {
+                const int buf = c & 1;
+                const int cn = c + 1 < c1 ? c + 1 : c;
+                b3_fetch<PER, CH>(h0, h1, h2, h3, l0, l1, l2, l3, Ih, Il, tid, y0, W, cn);
+                __builtin_amdgcn_sched_barrier(0);   // keep the image loads ahead of the table loads
+                {   // unconditional: lanes past nuv read row 0 (arow = 0) and are never stored;
+                    // a branch here would make the load counters wait for these loads too
+                    const int kn = cn * B3_KC;
+                    d0h = *reinterpret_cast<const uint4 *>(Xh + arow + kn + 8 * lh);
+                    d0l = *reinterpret_cast<const uint4 *>(Xl + arow + kn + 8 * lh);
+                    d1h = *reinterpret_cast<const uint4 *>(Xh + arow + kn + 16 + 8 * lh);
+                    d1l = *reinterpret_cast<const uint4 *>(Xl + arow + kn + 16 + 8 * lh);
+                }
+                __builtin_amdgcn_sched_barrier(0);   // and both ahead of this chunk's MFMAs
 #pragma unroll
-        for (int r = 0; r < 8; r++) {
-            const int p = p0 + 8 * (r >> 2) + 4 * lh + (r & 3);
-            if (p < nuv) {
-                const float2 *yrow = Y + (size_t)p * W + y0 + lr;
+                for (int ks = 0; ks < B3_KC / 16; ks++) {
+                    const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(ks ? &c1h : &c0h);
+                    const bf16x8 al = *reinterpret_cast<const bf16x8 *>(ks ? &c1l : &c0l);
+                    // every B operand of the k-step read from LDS before the first MFMA: the
+                    // reads overlap each other instead of one LDS latency per tile
+                    bf16x8 bh[NBK], bl[NBK];
 #pragma unroll
-                for (int b = 0; b < NBK; b++) {
-                    const float2 yv = yrow[b * 32];
-                    const float tr = acc[b][r], ti = acc[b][r + 8];
-                    vr[r] += tr * yv.x - ti * yv.y;
-                    vi[r] += tr * yv.y + ti * yv.x;
+                    for (int b = 0; b < NBK; b++) {
+                        const int o = (b * 32 + lr) * B3_STRIDE + ks * 16 + 8 * lh;
+                        bh[b] = *reinterpret_cast<const bf16x8 *>(&sB[buf][0][o]);
+                        bl[b] = *reinterpret_cast<const bf16x8 *>(&sB[buf][1][o]);
+                    }
+                    // product-major: consecutive MFMAs update different accumulators, so
+                    // none waits for the previous one's result (same per-tile order hh, hl, lh)
+#pragma unroll
+                    for (int b = 0; b < NBK; b++)
+                        acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[b], acc[b], 0, 0, 0);
+#pragma unroll
+                    for (int b = 0; b < NBK; b++)
+                        acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[b], acc[b], 0, 0, 0);
+#pragma unroll
+                    for (int b = 0; b < NBK; b++)
+                        acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[b], acc[b], 0, 0, 0);
+                }
+                // the other buffer was last read in chunk c-1, before the previous barrier
+                b3_store<PER, CH>(h0, h1, h2, h3, l0, l1, l2, l3, sB[buf ^ 1][0], sB[buf ^ 1][1], tid);
+                __syncthreads();
+            };
+            int c = c0;
+            for (; c + 1 < c1; c += 2) {
+                chunk(c, a0h, a0l, a1h, a1l, b0h, b0l, b1h, b1l);
+                chunk(c + 1, b0h, b0l, b1h, b1l, a0h, a0l, a1h, a1l);
+            }
+            if (c < c1)
+                chunk(c, a0h, a0l, a1h, a1l, b0h, b0l, b1h, b1l);
+            // epilogue of this row group: point 8(r/4) + 4 lh + r%4 (r < 8) has Re T in
+            // register r and Im T in register r+8, column y = y0 + 32 b + lr
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const int p = p0 + 8 * (r >> 2) + 4 * lh + (r & 3);
+                if (p < nuv) {
+                    const float2 *yrow = Y + (size_t)p * W + y0 + lr;
+#pragma unroll
+                    for (int b = 0; b < NBK; b++) {
+                        const float2 yv = yrow[b * 32];
+                        const float tr = acc[b][r], ti = acc[b][r + 8];
+                        vr[r] += tr * yv.x - ti * yv.y;
+                        vi[r] += tr * yv.y + ti * yv.x;
+                    }
                 }
             }
         }
-    }
-#pragma unroll
-    for (int r = 0; r < 8; r++) {
-#pragma unroll
-        for (int o = 1; o < 32; o <<= 1) {
-            vr[r] += __shfl_xor(vr[r], o, 64);
-            vi[r] += __shfl_xor(vi[r], o, 64);
-        }
-    }
-    if (lr == 0) {
-        const float f = scale ? *scale : 1.0f;
 #pragma unroll
         for (int r = 0; r < 8; r++) {
-            const int p = p0 + 8 * (r >> 2) + 4 * lh + (r & 3);
-            if (p < nuv)
-                vis[p] = make_float2(vr[r] * f, vi[r] * f);
+#pragma unroll
+            for (int o = 1; o < 32; o <<= 1) {
+                vr[r] += __shfl_xor(vr[r], o, 64);
+                vi[r] += __shfl_xor(vi[r], o, 64);
+            }
+        }
+        if (lr == 0) {
+            const bool whole = c0 == 0 && c1 == nchunk;
+            const float f = whole && scale ? *scale : 1.0f;
+            float2 *dst = whole ? vis : vpart + (c0 == 0 ? 0 : nuv);
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const int p = p0 + 8 * (r >> 2) + 4 * lh + (r & 3);
+                if (p < nuv)
+                    dst[p] = make_float2(vr[r] * f, vi[r] * f);
+            }
         }
     }
 }
 
+// the blocks split between two workgroups of k_vis_fwd_b3: one 64-thread block per
+// workgroup boundary, vis = (part 0 + part 1) * scale
+__global__ void k_vis_fwd_b3_combine(const float2 *__restrict__ vpart, int nuv, int W, int upw,
+                                     const float *__restrict__ scale, float2 *__restrict__ vis)
+{
+    const int nchunk = W / B3_KC;
+    const int total = (nuv + B3_PW - 1) / B3_PW * nchunk;
+    const int ub = (int)(blockIdx.x + 1) * upw;
+    if (ub >= total || ub % nchunk == 0)
+        return;
+    const int p = ub / nchunk * B3_PW + (int)threadIdx.x;
+    if (p >= nuv)
+        return;
+    const float f = scale ? *scale : 1.0f;
+    const float2 a = vpart[p], b = vpart[nuv + p];
+    vis[p] = make_float2((a.x + b.x) * f, (a.y + b.y) * f);
+}
+
 template __global__ void k_vis_fwd_b3<2>(const unsigned short *, const unsigned short *, const unsigned short *,
-                                         const unsigned short *, const float2 *, int, int, const float *, float2 *);
+                                         const unsigned short *, const float2 *, int, int, int, const float *, float2 *,
+                                         float2 *);
 template __global__ void k_vis_fwd_b3<4>(const unsigned short *, const unsigned short *, const unsigned short *,
-                                         const unsigned short *, const float2 *, int, int, const float *, float2 *);
+                                         const unsigned short *, const float2 *, int, int, int, const float *, float2 *,
+                                         float2 *);
 template __global__ void k_vis_fwd_b3<6>(const unsigned short *, const unsigned short *, const unsigned short *,
-                                         const unsigned short *, const float2 *, int, int, const float *, float2 *);
+                                         const unsigned short *, const float2 *, int, int, int, const float *, float2 *,
+                                         float2 *);
 template __global__ void k_vis_fwd_b3<8>(const unsigned short *, const unsigned short *, const unsigned short *,
-                                         const unsigned short *, const float2 *, int, int, const float *, float2 *);
+                                         const unsigned short *, const float2 *, int, int, int, const float *, float2 *,
+                                         float2 *);
 
 // explicit instantiations for the supported widths (W = 64 * NB)
 template __global__ void k_vis_fwd_f32<1>(const float *, const float2 *, const float2 *, int, const float *, float2 *);
