@@ -474,9 +474,13 @@ extern "C" int gpu_copy_dft(gpair_ctx *c, const float *dft_x, const float *dft_y
 // split of the adjoint over uv for about `groups` workgroups in total (b3: the
 // split-precision adjoint's 64 x 128 tiles, else the FP32 adjoint's 64 x 64 tiles);
 // grows the shared partial buffer when needed
+// the split-precision adjoint takes 256 x 256 quadrants (k_adj_b3w) when the width allows
+static inline int adj_wide(const gpair_ctx *c) { return ADJ_WIDE && c->W % AW_T == 0; }
+
 static int set_adj_split(gpair_ctx *c, int b3, int groups)
 {
-    const int tiles = b3 ? (c->W / AB3_YT) * (c->W / AB3_XT) : (c->W / 64) * (c->W / 64);
+    const int tiles = b3 ? (adj_wide(c) ? (c->W / AW_T) * (c->W / AW_T) : (c->W / AB3_YT) * (c->W / AB3_XT))
+                         : (c->W / 64) * (c->W / 64);
     const int gran = b3 ? AB3_PR : 8;
     int nsplit = groups / tiles;
     if (nsplit < 1) nsplit = 1;
@@ -552,7 +556,7 @@ static int ensure_split_tables(gpair_ctx *c)
         k_split_tables_t<<<(unsigned)((tt + 255) / 256), 256, 0, c->stream>>>(c->X, c->nuv, c->nuvp, c->W, c->Xth,
                                                                              c->Xtl);
         LAUNCHCHK(c);
-        if (!c->adj_b3_nsplit && (rc = set_adj_split(c, 1, 512)))
+        if (!c->adj_b3_nsplit && (rc = set_adj_split(c, 1, adj_wide(c) ? ADJ_WIDE_GROUPS : 512)))
             return rc;
     }
     c->split_valid = 1;
@@ -762,10 +766,17 @@ static int launch_adjoint(gpair_ctx *c)
         if ((rc = ensure_split_tables(c)))
             return rc;
         nsplit = c->adj_b3_nsplit;
-        dim3 grid((c->W / AB3_YT) * (c->W / AB3_XT), nsplit);
-        TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,
-                     (k_adj_b3<<<grid, 256, 0, c->stream>>>(c->coef, c->Y, c->Xth, c->Xtl, c->nuv, c->nuvp, c->W,
-                                                            c->adj_b3_pper, c->adj_part)));
+        if (adj_wide(c)) {
+            dim3 grid((c->W / AW_T) * (c->W / AW_T), nsplit);
+            TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,
+                         (k_adj_b3w<<<grid, 512, 0, c->stream>>>(c->coef, c->Y, c->Xth, c->Xtl, c->nuv, c->nuvp,
+                                                                 c->W, c->adj_b3_pper, c->adj_part)));
+        } else {
+            dim3 grid((c->W / AB3_YT) * (c->W / AB3_XT), nsplit);
+            TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,
+                         (k_adj_b3<<<grid, 256, 0, c->stream>>>(c->coef, c->Y, c->Xth, c->Xtl, c->nuv, c->nuvp,
+                                                                c->W, c->adj_b3_pper, c->adj_part)));
+        }
     } else {
         dim3 grid((c->W / 64) * (c->W / 64), nsplit);
         TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,
@@ -773,7 +784,8 @@ static int launch_adjoint(gpair_ctx *c)
                                                              c->adj_part)));
     }
     LAUNCHCHK(c);
-    k_grad_finish<<<ew_blocks(c->n), 256, 0, c->stream>>>(c->adj_part, nsplit, c->n, c->scal, c->gdata.p);
+    k_grad_finish<<<(c->n + GF_PIX - 1) / GF_PIX, 256, 0, c->stream>>>(c->adj_part, nsplit, c->n, c->scal,
+                                                                          c->gdata.p);
     LAUNCHCHK(c);
     return GPAIR_OK;
 }

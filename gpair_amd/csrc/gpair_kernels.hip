@@ -371,30 +371,39 @@ __global__ __launch_bounds__(256) void k_adj_f32(const float2 *__restrict__ coef
     }
 }
 
-// data gradient = (sum over splits - c0) / F, splits summed in order in FP64
-__global__ void k_grad_finish(const float *__restrict__ part, int nsplit, int n, const double *__restrict__ scal,
-                              float *__restrict__ gdata)
+// data gradient = (sum over splits - c0) / F in FP64.  A block takes 64 pixels; its
+// four waves sum a quarter of the splits each, in order, and the quarters are added
+// in order: a fixed grouping, so the result is the same on every run.
+#define GF_PIX 64
+__global__ __launch_bounds__(256) void k_grad_finish(const float *__restrict__ part, int nsplit, int n,
+                                                     const double *__restrict__ scal, float *__restrict__ gdata)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    // loads issued 8 at a time, added in split order (the result does not depend on
-    // the grouping because the FP64 sum of 64 FP32 values is exact to ~2^-29 relative
-    // either way, and the order is fixed)
+    __shared__ double sh[4][GF_PIX];
+    const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * GF_PIX + lane;
+    const int per = (nsplit + 3) / 4;
+    const int k0 = q * per, k1 = min(nsplit, k0 + per);
     double s = 0.0;
-    int k = 0;
-    for (; k + 8 <= nsplit; k += 8) {
-        float v[8];
+    if (i < n) {
+        int k = k0;
+        for (; k + 8 <= k1; k += 8) {
+            float v[8];
 #pragma unroll
-        for (int j = 0; j < 8; j++)
-            v[j] = part[(size_t)(k + j) * n + i];
+            for (int j = 0; j < 8; j++)
+                v[j] = part[(size_t)(k + j) * n + i];
 #pragma unroll
-        for (int j = 0; j < 8; j++)
-            s += (double)v[j];
+            for (int j = 0; j < 8; j++)
+                s += (double)v[j];
+        }
+        for (; k < k1; k++)
+            s += (double)part[(size_t)k * n + i];
     }
-    for (; k < nsplit; k++)
-        s += (double)part[(size_t)k * n + i];
-    gdata[i] = (float)((s - scal[SC_C0]) / scal[SC_FLUX]);
+    sh[q][lane] = s;
+    __syncthreads();
+    if (q == 0 && i < n) {
+        const double t = ((sh[0][lane] + sh[1][lane]) + sh[2][lane]) + sh[3][lane];
+        gdata[i] = (float)((t - scal[SC_C0]) / scal[SC_FLUX]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -516,6 +525,154 @@ __global__ __launch_bounds__(256) void k_adj_b3(const float2 *__restrict__ coef,
             const int y = y0 + 32 * m + 8 * (r >> 2) + 4 * lh + (r & 3);
             out[(size_t)y * W + xw] = acc[m][r];
         }
+}
+
+// Wide variant for W a multiple of 256: one workgroup (8 waves) = a 256 x 256
+// quadrant of the image for one uv split, so every table entry of the split is read
+// once per quadrant (the 64 x 128 tiles above read Y twice and X four times at
+// 256^2).  Per round of 16 uv points the A tile (a Y, 256 rows) and the B tile
+// (split X, 256 columns) are staged in LDS, double-buffered, from registers loaded
+// one round ahead; wave w owns rows 64 (w / 2) .. +63 and columns 128 (w % 2) ..
+// +127 of the quadrant (2 x 4 accumulator tiles).
+#define AW_T 256
+#define AW_STRIDE (2 * AB3_PR + 8)   // bf16 per LDS row: 32 k + 8 padding
+#ifndef ADJ_WIDE
+#define ADJ_WIDE 1                   // use k_adj_b3w where the width allows
+#endif
+#ifndef ADJ_WIDE_GROUPS
+#define ADJ_WIDE_GROUPS 256          // workgroups of k_adj_b3w (quadrants x uv splits)
+#endif
+
+__global__ __launch_bounds__(512) void k_adj_b3w(const float2 *__restrict__ coef, const float2 *__restrict__ Y,
+                                                 const unsigned short *__restrict__ Xth,
+                                                 const unsigned short *__restrict__ Xtl, int nuv, int nuvp, int W,
+                                                 int p_per_split, float *__restrict__ part)
+{
+    __shared__ __attribute__((aligned(16))) unsigned short sA[2][2][AW_T * AW_STRIDE];   // [buf][hi/lo]
+    __shared__ __attribute__((aligned(16))) unsigned short sB[2][2][AW_T * AW_STRIDE];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 31, lh = lane >> 5;
+    const int qx = W / AW_T;
+    const int y0 = (blockIdx.x / qx) * AW_T, x0 = (blockIdx.x % qx) * AW_T;
+    const int pstart = blockIdx.y * p_per_split;
+    const int pend = min(pstart + p_per_split, nuv);
+    // fill roles: A: point ap = tid / 32 of the round, rows ay .. ay + 7;
+    //             B: column bx, points bk .. bk + 7 of the round (k = 2 bk .. 2 bk + 15)
+    const int ap = tid >> 5, ay = (tid & 31) * 8;
+    const int bx = tid >> 1, bk = (tid & 1) * 8;
+    float4 ya0, ya1, ya2, ya3;
+    float2 ca;
+    uint4 xh0, xh1, xl0, xl1;
+    auto load = [&](int pr) {
+        // points past the split get a = 0 (their Y row, read from the split's last point,
+        // and their X contribute nothing)
+        const int p = pr + ap;
+        const int pv = p < pend ? p : pend - 1;
+        const float2 cv = coef[pv];
+        ca = p < pend ? cv : make_float2(0.f, 0.f);
+        const float4 *yr = reinterpret_cast<const float4 *>(Y + (size_t)pv * W + y0 + ay);
+        ya0 = yr[0];
+        ya1 = yr[1];
+        ya2 = yr[2];
+        ya3 = yr[3];
+        // rounds start at multiples of 16 below nuv, so pr + 16 <= nuvp: in the padded table
+        const size_t bo = ((size_t)(x0 + bx) * nuvp + pr + bk) * 2;
+        xh0 = *reinterpret_cast<const uint4 *>(Xth + bo);
+        xh1 = *reinterpret_cast<const uint4 *>(Xth + bo + 8);
+        xl0 = *reinterpret_cast<const uint4 *>(Xtl + bo);
+        xl1 = *reinterpret_cast<const uint4 *>(Xtl + bo + 8);
+    };
+    auto store = [&](int buf) {
+        auto put = [&](int j, float yr, float yi) {
+            const float2 v = cmul(ca, make_float2(yr, yi));
+            unsigned short h0, l0, h1, l1;
+            split_bf16(v.x, h0, l0);
+            split_bf16(-v.y, h1, l1);
+            const int o = (ay + j) * AW_STRIDE + 2 * ap;
+            *reinterpret_cast<unsigned *>(&sA[buf][0][o]) = (unsigned)h0 | ((unsigned)h1 << 16);
+            *reinterpret_cast<unsigned *>(&sA[buf][1][o]) = (unsigned)l0 | ((unsigned)l1 << 16);
+        };
+        put(0, ya0.x, ya0.y);
+        put(1, ya0.z, ya0.w);
+        put(2, ya1.x, ya1.y);
+        put(3, ya1.z, ya1.w);
+        put(4, ya2.x, ya2.y);
+        put(5, ya2.z, ya2.w);
+        put(6, ya3.x, ya3.y);
+        put(7, ya3.z, ya3.w);
+        const int o = bx * AW_STRIDE + 2 * bk;
+        *reinterpret_cast<uint4 *>(&sB[buf][0][o]) = xh0;
+        *reinterpret_cast<uint4 *>(&sB[buf][0][o + 8]) = xh1;
+        *reinterpret_cast<uint4 *>(&sB[buf][1][o]) = xl0;
+        *reinterpret_cast<uint4 *>(&sB[buf][1][o + 8]) = xl1;
+    };
+    const int wy = (wave >> 1) * 64, wx = (wave & 1) * 128;
+    v16f acc[2][4];
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+                acc[m][n][r] = 0.f;
+    int buf = 0;
+    if (pstart < pend) {
+        load(pstart);
+        store(0);
+    }
+    __syncthreads();
+    for (int pr = pstart; pr < pend; pr += AB3_PR) {
+        const bool more = pr + AB3_PR < pend;
+        if (more)
+            load(pr + AB3_PR);
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            bf16x8 ah[2], al[2], bh[4], bl[4];
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+                const int o = (wy + 32 * m + lr) * AW_STRIDE + 16 * ks + 8 * lh;
+                ah[m] = *reinterpret_cast<const bf16x8 *>(&sA[buf][0][o]);
+                al[m] = *reinterpret_cast<const bf16x8 *>(&sA[buf][1][o]);
+            }
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                const int o = (wx + 32 * n + lr) * AW_STRIDE + 16 * ks + 8 * lh;
+                bh[n] = *reinterpret_cast<const bf16x8 *>(&sB[buf][0][o]);
+                bl[n] = *reinterpret_cast<const bf16x8 *>(&sB[buf][1][o]);
+            }
+            // product-major over the eight accumulators (same per-tile order hh, hl, lh)
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+                for (int n = 0; n < 4; n++)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m], bh[n], acc[m][n], 0, 0, 0);
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+                for (int n = 0; n < 4; n++)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m], bl[n], acc[m][n], 0, 0, 0);
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+                for (int n = 0; n < 4; n++)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[m], bh[n], acc[m][n], 0, 0, 0);
+        }
+        // the other buffer was last read in the previous round, before its barrier
+        if (more)
+            store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    float *out = part + (size_t)blockIdx.y * W * W;
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int y = y0 + wy + 32 * m + 8 * (r >> 2) + 4 * lh + (r & 3);
+                out[(size_t)y * W + x0 + wx + 32 * n + lr] = acc[m][n][r];
+            }
 }
 
 // ---------------------------------------------------------------------------
