@@ -676,14 +676,17 @@ static int launch_flux(gpair_ctx *c, const float *img)
 }
 
 // vis_out = scale ? (*scale) A img : A img  (A = unnormalised DFT)
-static int launch_forward_to(gpair_ctx *c, const float *img, const float *scale, float2 *vis_out)
+// presplit: the BF16 split of img is already in Ih / Il (written by its producer)
+static int launch_forward_to(gpair_ctx *c, const float *img, const float *scale, float2 *vis_out, int presplit = 0)
 {
     if (c->precision == GPAIR_PREC_BF16X3) {
         int rc;
         if ((rc = ensure_split_tables(c)))
             return rc;
-        k_split_image<<<ew_blocks(c->n), 256, 0, c->stream>>>(img, c->n, c->Ih, c->Il);
-        LAUNCHCHK(c);
+        if (!presplit) {
+            k_split_image<<<ew_blocks(c->n), 256, 0, c->stream>>>(img, c->n, c->Ih, c->Il);
+            LAUNCHCHK(c);
+        }
         // stream-K balance: at most B3_WG workgroups (one per CU at one wave per SIMD),
         // each taking an equal run of (64-point block, 32-column chunk) units
         const int nblk = (c->nuv + B3_PW - 1) / B3_PW, nchunk = c->W / B3_KC;

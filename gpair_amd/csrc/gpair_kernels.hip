@@ -746,11 +746,13 @@ __global__ __launch_bounds__(RED_THREADS) void k_pcg_precond(const float *__rest
 // d = -z + beta d, projected to zero where the pixel sits on the floor and d < 0;
 // out[0] = d.g, out[1] = sum d, out[2] = sum |d|.  conj != 0: Polak-Ribiere+
 // beta = max(0, (z.g - z.g_old) / zg_old) from the z.g slots the preconditioner
-// kernel left in scal (kept in scal[SC_BETA]); conj == 0: beta = 0 (-z)
+// kernel left in scal (kept in scal[SC_BETA]); conj == 0: beta = 0 (-z).  With dh
+// set, also the BF16 hi / lo split of d for the forward DFT that follows.
 __global__ __launch_bounds__(RED_THREADS) void k_pcg_dir(const float *__restrict__ img, const float *__restrict__ z,
                                                          const float *__restrict__ g, int conj, double zg_old,
                                                          float minval, int n, float *__restrict__ d, double *scal,
-                                                         double *part, unsigned *ticket, double *out)
+                                                         double *part, unsigned *ticket, double *out,
+                                                         unsigned short *__restrict__ dh, unsigned short *__restrict__ dl)
 {
     double b = 0.0;
     if (conj) {
@@ -767,6 +769,12 @@ __global__ __launch_bounds__(RED_THREADS) void k_pcg_dir(const float *__restrict
         if (img[i] <= minval && di < 0.f)
             di = 0.f;
         d[i] = di;
+        if (dh) {
+            unsigned short h, l;
+            split_bf16(di, h, l);
+            dh[i] = h;
+            dl[i] = l;
+        }
         v[0] += (double)di * (double)g[i];
         v[1] += (double)di;
         v[2] += fabs((double)di);
@@ -909,13 +917,16 @@ __global__ __launch_bounds__(RED_THREADS) void k_ls_dentropy(const float *__rest
 }
 
 // trial image I_t = max(I + s d, minval), the change delta = I_t - I, and
-// out[0] = sum delta, out[1] = S(I_t) - S(I)
+// out[0] = sum delta, out[1] = S(I_t) - S(I); with th set, also the BF16 hi / lo
+// split of delta for the forward DFT that follows
 __global__ __launch_bounds__(RED_THREADS) void k_commit_step(const float *__restrict__ img,
                                                              const float *__restrict__ d,
                                                              const float *__restrict__ model, float s, float minval,
                                                              int n, float *__restrict__ trial,
                                                              float *__restrict__ delta, double *part,
-                                                             unsigned *ticket, double *out)
+                                                             unsigned *ticket, double *out,
+                                                             unsigned short *__restrict__ th,
+                                                             unsigned short *__restrict__ tl)
 {
     double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) {
@@ -926,6 +937,12 @@ __global__ __launch_bounds__(RED_THREADS) void k_commit_step(const float *__rest
         trial[i] = t;
         const float dl = t - I;
         delta[i] = dl;
+        if (th) {
+            unsigned short h, l;
+            split_bf16(dl, h, l);
+            th[i] = h;
+            tl[i] = l;
+        }
         const double M = model[i];
         v[0] += (double)dl;
         v[1] += pix_entropy((double)t, M) - pix_entropy((double)I, M);
