@@ -898,6 +898,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_ls_dentropy(const float *__rest
         const float I = img[i], dd = d[i], M = model[i];
         if (!(I > 0.f && M > 0.f))
             return;   // the floor keeps I > 0; S = -M there whatever the step
+        if (dd == 0.f && I >= (float)minval)
+            return;   // t = I for every step: each term below is an exact zero
         // S(t) - S(I) = dl (1 - ln(I/M)) - t log1p(dl / I), dl = t - I: every term is
         // proportional to the change, so FP32 resolves it relative to the change itself
         const float L = logf(I / M), invI = 1.0f / I;
