@@ -14,8 +14,11 @@ timed region.  A step is one CG iteration.
   included, BF16x3), and for the plain loop with the reference's rules
   (gpair_cg_run, FP32 DFT), the latter bounded by --plain-budget-s.
 * roofline: the forward DFT, timed with HIP events on the context stream, against
-  the roofline that binds it: MFMA for FP32 (64 flop per compulsory byte at 256^2,
-  above the FP32 balance of 20), HBM for BF16x3 (below its balance of 104).
+  the roofline that binds it.  FP32 reads the full X and Y tables (64 flop per
+  compulsory byte at 256^2, above the FP32 balance of 20): MFMA.  BF16x3 rebuilds
+  table entries from the factored tables (8 (W/16 + 16) bytes per uv point and
+  axis: 502 flop per compulsory byte at 256^2, above its balance of 104): MFMA,
+  against the BF16 peak over the 3 products per multiply-add.
 * cpu_baseline: the reference's own CPU model (oracle/_ref) on one core; the
   same leg re-evaluates each final image's objective with the FP64 oracle
   (err_vs_fp64).
@@ -344,10 +347,12 @@ def main():
     avg_fwd = fwd_ms / max(fwd_n, 1) * 1e-3
     achieved = flop / avg_fwd / 1e12
     nbk = (tuned[3] if tuned and tuned[3] else min(W, 256) // 32) if prec == 1 else 0
-    kname = f"k_vis_fwd_f32<{W // 64}>" if prec == 0 else f"k_vis_fwd_b3<{nbk}>"
-    # compulsory bytes of one forward: X and Y tables (8 B per (uv, x) entry each: float2,
-    # or bf16 hi/lo of Re and Im), the image (4 B per pixel: float, or bf16 hi + lo), V out
-    nbytes = 16.0 * nuv * W + 4.0 * W * W + 8.0 * nuv
+    kname = f"k_vis_fwd_f32<{W // 64}>" if prec == 0 else f"k_vis_fwd_b3f<{nbk}>"
+    # compulsory bytes of one forward: the tables (FP32: X and Y, 8 B per (uv, x) entry
+    # each; BF16x3: the factored X and Y tables, 8 B per factor, W/16 + 16 factors per
+    # uv point each), the image (4 B per pixel: float, or bf16 hi + lo), V out
+    table_bytes = 16.0 * nuv * W if prec == 0 else 16.0 * nuv * (W // 16 + 16)
+    nbytes = table_bytes + 4.0 * W * W + 8.0 * nuv
     intensity = flop / nbytes
     traffic = read_traffic(f"forward_dft_W{W}_{a.precision}_pmc.txt")
     if intensity < PEAK[a.precision] * 1e12 / HBM_PEAK:

@@ -37,6 +37,8 @@ struct gpair_ctx {
     int4 *tri;
     int *csr_ptr, *csr_ent;
     float2 *X, *Y;
+    float2 *FX, *FY;   // factored tables (coarse then fine factors per point), from gpair_set_uv
+    int have_ft;
     // images
     gpair_buf_s img, tmp, model, gdata, gent, gnew, gold, gtmp, dir;
     float2 *vis, *coef;
@@ -236,7 +238,7 @@ extern "C" int gpu_cleanup(gpair_ctx *c)
     free_images(c);
     px_free(c);
     dfree(&c->data); dfree(&c->err); dfree(&c->phasor); dfree(&c->tri); dfree(&c->csr_ptr); dfree(&c->csr_ent);
-    dfree(&c->X); dfree(&c->Y); dfree(&c->vis); dfree(&c->coef); dfree(&c->mock);
+    dfree(&c->X); dfree(&c->Y); dfree(&c->FX); dfree(&c->FY); dfree(&c->vis); dfree(&c->coef); dfree(&c->mock);
     dfree(&c->scal); dfree(&c->invflux); dfree(&c->part); dfree(&c->ticket); dfree(&c->adj_part);
     dfree(&c->Ud); dfree(&c->Udel); dfree(&c->pfac); dfree(&c->ls_out);
     dfree(&c->vis2); dfree(&c->Vd); dfree(&c->Vn); dfree(&c->mock2);
@@ -371,8 +373,10 @@ extern "C" int gpu_copy_data(gpair_ctx *c, const float *data, const float *data_
     HIPCHK(c, hipMemsetAsync(c->mock, 0, sizeof(float) * ndof, c->stream));
     HIPCHK(c, hipMemsetAsync(c->vis, 0, sizeof(float2) * nuv, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));   // host vectors go out of scope
-    if (c->nuv != nuv)
+    if (c->nuv != nuv) {
         c->have_tables = 0;
+        c->have_ft = 0;
+    }
     c->nuv = nuv;
     c->npow = npow;
     c->nbis = nbis;
@@ -407,8 +411,10 @@ extern "C" int gpair_set_uv(gpair_ctx *c, const float *u, const float *v, int nu
     if (!(pixellation > 0.f) || !isfinite(pixellation))
         return fail(c, GPAIR_E_ARG, "gpair_set_uv: pixellation %g", (double)pixellation);
     int rc;
-    const size_t tsz = (size_t)nuv * c->W;
-    if ((rc = dalloc(c, &c->X, tsz)) || (rc = dalloc(c, &c->Y, tsz)))
+    const size_t tsz = (size_t)nuv * c->W, fsz = (size_t)nuv * ft_len(c->W);
+    c->have_ft = 0;
+    if ((rc = dalloc(c, &c->X, tsz)) || (rc = dalloc(c, &c->Y, tsz)) || (rc = dalloc(c, &c->FX, fsz)) ||
+        (rc = dalloc(c, &c->FY, fsz)))
         return rc;
     float *du = nullptr, *dv = nullptr;
     if ((rc = dalloc(c, &du, nuv)) || (rc = dalloc(c, &dv, nuv))) {
@@ -422,13 +428,16 @@ extern "C" int gpair_set_uv(gpair_ctx *c, const float *u, const float *v, int nu
     if (!err) {
         const double a = REF_RPMAS * (double)pixellation;
         k_tables<<<(unsigned)((tsz + 255) / 256), 256, 0, c->stream>>>(du, dv, a, nuv, c->W, c->X, c->Y);
+        k_ftables<<<(unsigned)((fsz + 255) / 256), 256, 0, c->stream>>>(du, dv, a, nuv, c->W, c->FX, c->FY);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
             err = fail(c, GPAIR_E_HIP, "gpair_set_uv: table kernel failed");
     }
     dfree(&du);
     dfree(&dv);
-    if (!err)
+    if (!err) {
         c->have_tables = 1;
+        c->have_ft = 1;
+    }
     c->split_valid = 0;
     px_invalidate(c);
     c->px_tables = 0;
@@ -445,6 +454,7 @@ extern "C" int gpu_copy_dft(gpair_ctx *c, const float *dft_x, const float *dft_y
     if ((size_t)dft_size < tsz)
         return fail(c, GPAIR_E_ARG, "gpu_copy_dft: %d entries, need nuv*W = %zu", dft_size, tsz);
     int rc;
+    c->have_ft = 0;   // arbitrary tables: the BF16x3 DFTs read their full split copies
     if ((rc = dalloc(c, &c->X, tsz)) || (rc = dalloc(c, &c->Y, tsz)))
         return rc;
     float2 *tmp = nullptr;
@@ -535,27 +545,42 @@ extern "C" int gpair_set_precision(gpair_ctx *c, int precision)
     return GPAIR_OK;
 }
 
-// hi/lo BF16 copies of the X table for the split-precision forward DFT
+// BF16x3 operands: the split image buffers always; the split copies of the X table
+// only where a kernel reads them (tables copied in by gpu_copy_dft, or an adjoint
+// width without the factored quadrant kernel)
+static inline int adj_factored(const gpair_ctx *c) { return c->have_ft && adj_wide(c); }
+
 static int ensure_split_tables(gpair_ctx *c)
 {
     if (c->split_valid)
         return GPAIR_OK;
     const size_t tsz = (size_t)c->nuv * c->W;
     int rc;
-    if ((rc = dalloc(c, &c->Xh, 2 * tsz)) || (rc = dalloc(c, &c->Xl, 2 * tsz)) ||
-        (rc = dalloc(c, &c->Ih, (size_t)c->n)) || (rc = dalloc(c, &c->Il, (size_t)c->n)) ||
+    if ((rc = dalloc(c, &c->Ih, (size_t)c->n)) || (rc = dalloc(c, &c->Il, (size_t)c->n)) ||
         (rc = dalloc(c, &c->fwdp, 2 * (size_t)c->nuv)))
         return rc;
-    k_split_tables<<<(unsigned)((tsz + 255) / 256), 256, 0, c->stream>>>(c->X, c->nuv, c->W, c->Xh, c->Xl);
-    LAUNCHCHK(c);
-    if (c->W % AB3_XT == 0) {   // the split-precision adjoint tiles the image in 64 x 128
-        c->nuvp = (c->nuv + AB3_PR - 1) / AB3_PR * AB3_PR;
-        const size_t tt = (size_t)c->nuvp * c->W;
-        if ((rc = dalloc(c, &c->Xth, 2 * tt)) || (rc = dalloc(c, &c->Xtl, 2 * tt)))
+    if (!c->have_ft) {
+        if ((rc = dalloc(c, &c->Xh, 2 * tsz)) || (rc = dalloc(c, &c->Xl, 2 * tsz)))
             return rc;
-        k_split_tables_t<<<(unsigned)((tt + 255) / 256), 256, 0, c->stream>>>(c->X, c->nuv, c->nuvp, c->W, c->Xth,
-                                                                             c->Xtl);
+        k_split_tables<<<(unsigned)((tsz + 255) / 256), 256, 0, c->stream>>>(c->X, c->nuv, c->W, c->Xh, c->Xl);
         LAUNCHCHK(c);
+    } else {
+        dfree(&c->Xh);
+        dfree(&c->Xl);
+    }
+    if (c->W % AB3_XT == 0) {   // the split-precision adjoint tiles the image in 64 x 128 or 256 x 256
+        if (!adj_factored(c)) {
+            c->nuvp = (c->nuv + AB3_PR - 1) / AB3_PR * AB3_PR;
+            const size_t tt = (size_t)c->nuvp * c->W;
+            if ((rc = dalloc(c, &c->Xth, 2 * tt)) || (rc = dalloc(c, &c->Xtl, 2 * tt)))
+                return rc;
+            k_split_tables_t<<<(unsigned)((tt + 255) / 256), 256, 0, c->stream>>>(c->X, c->nuv, c->nuvp, c->W,
+                                                                                 c->Xth, c->Xtl);
+            LAUNCHCHK(c);
+        } else {
+            dfree(&c->Xth);
+            dfree(&c->Xtl);
+        }
         if (!c->adj_b3_nsplit && (rc = set_adj_split(c, 1, adj_wide(c) ? ADJ_WIDE_GROUPS : 512)))
             return rc;
     }
@@ -693,14 +718,23 @@ static int launch_forward_to(gpair_ctx *c, const float *img, const float *scale,
         const int nwg = nblk < B3_WG ? nblk : B3_WG;
         const int upw = (nblk * nchunk + nwg - 1) / nwg;
         const int nbk = c->fwd_nbk ? c->fwd_nbk : (c->W < B3_ROWS ? c->W : B3_ROWS) / 32;
+        if (c->W % (32 * nbk) != 0)
+            return fail(c, GPAIR_E_ARG, "BF16x3 forward: width %d is not a multiple of %d rows", c->W, 32 * nbk);
         switch (nbk) {
 #define B3_CASE(N)                                                                                             \
     case N:                                                                                                    \
-        TIMED_LAUNCH(c, GPAIR_TK_FORWARD,                                                                      \
-                     (k_vis_fwd_b3<N><<<nwg, 256, 0, c->stream>>>(c->Ih, c->Il, c->Xh, c->Xl, c->Y, c->nuv, c->W,  \
-                                                                  upw, scale, vis_out, c->fwdp),               \
-                      k_vis_fwd_b3_combine<<<nwg > 1 ? nwg - 1 : 1, B3_PW, 0, c->stream>>>(c->fwdp, c->nuv, c->W, \
-                                                                                         upw, scale, vis_out))); \
+        if (c->have_ft)                                                                                        \
+            TIMED_LAUNCH(c, GPAIR_TK_FORWARD,                                                                  \
+                         (k_vis_fwd_b3f<N><<<nwg, 256, 0, c->stream>>>(c->Ih, c->Il, c->FX, c->FY, c->nuv, c->W, \
+                                                                       upw, scale, vis_out, c->fwdp),          \
+                          k_vis_fwd_b3_combine<<<nwg > 1 ? nwg - 1 : 1, B3_PW, 0, c->stream>>>(             \
+                              c->fwdp, c->nuv, c->W, upw, scale, vis_out)));                                   \
+        else                                                                                                   \
+            TIMED_LAUNCH(c, GPAIR_TK_FORWARD,                                                                  \
+                         (k_vis_fwd_b3<N><<<nwg, 256, 0, c->stream>>>(c->Ih, c->Il, c->Xh, c->Xl, c->Y, c->nuv,  \
+                                                                      c->W, upw, scale, vis_out, c->fwdp),     \
+                          k_vis_fwd_b3_combine<<<nwg > 1 ? nwg - 1 : 1, B3_PW, 0, c->stream>>>(             \
+                              c->fwdp, c->nuv, c->W, upw, scale, vis_out)));                                   \
         break;
             B3_CASE(2)
             B3_CASE(4)
@@ -769,7 +803,12 @@ static int launch_adjoint(gpair_ctx *c)
         if ((rc = ensure_split_tables(c)))
             return rc;
         nsplit = c->adj_b3_nsplit;
-        if (adj_wide(c)) {
+        if (adj_factored(c)) {
+            dim3 grid((c->W / AW_T) * (c->W / AW_T), nsplit);
+            TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,
+                         (k_adj_b3wf<<<grid, 512, 0, c->stream>>>(c->coef, c->FX, c->FY, c->nuv, c->W,
+                                                                  c->adj_b3_pper, c->adj_part)));
+        } else if (adj_wide(c)) {
             dim3 grid((c->W / AW_T) * (c->W / AW_T), nsplit);
             TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,
                          (k_adj_b3w<<<grid, 512, 0, c->stream>>>(c->coef, c->Y, c->Xth, c->Xtl, c->nuv, c->nuvp,

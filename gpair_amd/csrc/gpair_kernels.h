@@ -61,6 +61,20 @@ __device__ inline void split_bf16(float x, unsigned short &hi, unsigned short &l
     lo = bf16_bits_rne(x - __uint_as_float((unsigned)hi << 16));
 }
 
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// the same split for two values at once on gfx950's packed converter
+// (v_cvt_pk_bf16_f32, round to nearest even like bf16_bits_rne for finite
+// values): h and l hold (x, y) as bf16 pairs, x in the low half
+__device__ inline void split2_bf16(float x, float y, unsigned &h, unsigned &l)
+{
+    const f32x2 v = {x, y};
+    h = __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
+    const f32x2 r = {x - __uint_as_float(h << 16), y - __uint_as_float(h & 0xffff0000u)};
+    l = __builtin_bit_cast(unsigned, __builtin_convertvector(r, bf16x2));
+}
+
 // ---------------------------------------------------------------------------
 // deterministic block reduction + last-block finish (fixed summation order)
 // ---------------------------------------------------------------------------

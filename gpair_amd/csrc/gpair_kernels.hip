@@ -1012,6 +1012,11 @@ __global__ __launch_bounds__(RED_THREADS) void k_commit_dchi2(const double2 *__r
 // ---------------------------------------------------------------------------
 // split precision BF16x3 forward DFT (split helpers in gpair_kernels.h)
 // ---------------------------------------------------------------------------
+// factored tables (below): W/16 coarse then FT_FINE fine factors per uv point
+#define FT_FINE 16
+
+__host__ __device__ inline int ft_len(int W) { return W / 16 + FT_FINE; }
+
 // X float2 [nuv][W] -> Xh, Xl bf16 [nuv][2][W] (re plane, im plane per uv point)
 __global__ void k_split_tables(const float2 *__restrict__ X, int nuv, int W, unsigned short *__restrict__ Xh,
                                unsigned short *__restrict__ Xl)
@@ -1308,3 +1313,393 @@ template __global__ void k_vis_fwd_f32<2>(const float *, const float2 *, const f
 template __global__ void k_vis_fwd_f32<3>(const float *, const float2 *, const float2 *, int, const float *, float2 *);
 template __global__ void k_vis_fwd_f32<4>(const float *, const float2 *, const float2 *, int, const float *, float2 *);
 template __global__ void k_vis_fwd_f32<8>(const float *, const float2 *, const float2 *, int, const float *, float2 *);
+
+// ---------------------------------------------------------------------------
+// BF16x3 DFTs on factored tables
+// ---------------------------------------------------------------------------
+// The table entry at column x = 16 i + j factors as exp(2 pi i s a w x) =
+// exp(2 pi i s a w 16 i) exp(2 pi i s a w j).  Per uv point the factored table
+// holds the W/16 coarse factors followed by the 16 fine ones (both from FP64
+// range-reduced phases, like k_tables): 8 (W/16 + 16) bytes per point instead of
+// 8 W.  The kernels below rebuild the entries they need as products of two
+// factors (about 2 ulp against the directly rounded entry, far below the 2^-17
+// of the BF16 split) and split them in registers, so the DFTs read a few MB of
+// tables instead of streaming the full ones.
+
+// FX[p][k] for X (w = u, s = +1), FY[p][k] for Y (w = v, s = -1)
+__global__ void k_ftables(const float *__restrict__ u, const float *__restrict__ v, double a, int nuv, int W,
+                          float2 *__restrict__ FX, float2 *__restrict__ FY)
+{
+    const int ft = ft_len(W), nco = W / 16;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nuv * ft)
+        return;
+    const int p = (int)(i / ft), k = (int)(i % ft);
+    const double m = k < nco ? 16.0 * k : (double)(k - nco);
+    const double au = a * (double)u[p], av = a * (double)v[p];
+    FX[i] = turn_exp(au * m);
+    FY[i] = turn_exp(-av * m);
+}
+
+// 8 consecutive entries of one component of a table row, split: entry e is
+// Re(cc g[e]) where g is the fine factor, pre-rotated for the imaginary component
+__device__ __forceinline__ void ft_gen8(float cr, float ci, const float2 (&g)[8], uint4 &h, uint4 &l)
+{
+    float v[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++)
+        v[e] = cr * g[e].x - ci * g[e].y;
+    split2_bf16(v[0], v[1], h.x, l.x);
+    split2_bf16(v[2], v[3], h.y, l.y);
+    split2_bf16(v[4], v[5], h.z, l.z);
+    split2_bf16(v[6], v[7], h.w, l.w);
+}
+
+// Forward DFT as k_vis_fwd_b3 (same MFMA tiling, stream-K units, LDS staging of
+// the split image and summation order), with the A operand generated: lane
+// (lr, lh) owns row lr of its wave's block, i.e. point p0 + lr % 16 and component
+// lr / 16, and columns 8 lh .. 8 lh + 7 of each 16-column half of a chunk, so its
+// 8 fine factors are fixed for the whole block and each chunk needs the two
+// coarse factors of its halves (one 16-byte load).  The image rows and coarse
+// factors of a chunk are loaded two chunks ahead into two alternating register
+// sets, so each load has two chunks of MFMA work to land before its LDS store.
+// The y-reduction rebuilds Y[p][y] from FY the same way.
+struct B3Stage {
+    uint4 h0, h1, h2, h3, l0, l1, l2, l3;
+};
+
+template <int NBK>
+__global__ __launch_bounds__(256) void k_vis_fwd_b3f(const unsigned short *__restrict__ Ih,
+                                                        const unsigned short *__restrict__ Il,
+                                                        const float2 *__restrict__ FX,
+                                                        const float2 *__restrict__ FY, int nuv, int W, int upw,
+                                                        const float *__restrict__ scale, float2 *__restrict__ vis,
+                                                        float2 *__restrict__ vpart)
+{
+    constexpr int ROWS = NBK * 32;
+    constexpr int CH = ROWS * (B3_KC / 8);
+    constexpr int PER = (CH + 255) / 256;
+    static_assert(PER <= 4, "staging holds up to 4 pieces per thread");
+    static_assert(B3_KC == 32, "two k-steps per chunk");
+    __shared__ __attribute__((aligned(16))) unsigned short sB[2][2][ROWS * B3_STRIDE];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 31, lh = lane >> 5;
+    const int ft = ft_len(W), nco = W / 16;
+    const int nchunk = W / B3_KC;
+    const int nblk = (nuv + B3_PW - 1) / B3_PW;
+    const int total = nblk * nchunk;
+    const int u_end = min(total, (int)(blockIdx.x + 1) * upw);
+    for (int u = (int)blockIdx.x * upw; u < u_end;) {
+        const int blk = u / nchunk, c0 = u % nchunk;
+        const int c1 = min(nchunk, c0 + (u_end - u));
+        u += c1 - c0;
+        const int p0 = blk * B3_PW + wave * 16;
+        const int pa = p0 + (lr & 15);
+        // lanes past nuv build row 0's entries: their output rows are never stored
+        const float2 *fx = FX + (size_t)(pa < nuv ? pa : 0) * ft;
+        float2 g[8];
+        {
+            const float4 *fp = reinterpret_cast<const float4 *>(fx + nco + 8 * lh);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const float4 f = fp[q];
+                g[2 * q] = make_float2(f.x, f.y);
+                g[2 * q + 1] = make_float2(f.z, f.w);
+            }
+            if (lr >= 16) {   // Im(c f) = Re(c (Im f, -Re f))
+#pragma unroll
+                for (int e = 0; e < 8; e++)
+                    g[e] = make_float2(g[e].y, -g[e].x);
+            }
+        }
+        float vr[8], vi[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            vr[r] = 0.f;
+            vi[r] = 0.f;
+        }
+        const int n = c1 - c0;
+        // chunk of position i in the range; positions past the end reload the last chunk
+        auto cidx = [&](int i) { return c0 + (i < n ? i : n - 1); };
+        for (int y0 = 0; y0 < W; y0 += ROWS) {
+            v16f acc[NBK];
+#pragma unroll
+            for (int b = 0; b < NBK; b++)
+#pragma unroll
+                for (int r = 0; r < 16; r++)
+                    acc[b][r] = 0.f;
+            auto fetch = [&](B3Stage &S, int c) {
+                b3_fetch<PER, CH>(S.h0, S.h1, S.h2, S.h3, S.l0, S.l1, S.l2, S.l3, Ih, Il, tid, y0, W, c);
+            };
+            auto stage = [&](const B3Stage &S, int buf) {
+                b3_store<PER, CH>(S.h0, S.h1, S.h2, S.h3, S.l0, S.l1, S.l2, S.l3, sB[buf][0], sB[buf][1], tid);
+            };
+            B3Stage P, Q;
+            P.h0 = P.h1 = P.h2 = P.h3 = P.l0 = P.l1 = P.l2 = P.l3 = make_uint4(0, 0, 0, 0);
+            Q = P;
+            fetch(P, cidx(0));
+            fetch(Q, cidx(1));
+            float4 cxa = *reinterpret_cast<const float4 *>(fx + 2 * cidx(0));
+            float4 cxb = *reinterpret_cast<const float4 *>(fx + 2 * cidx(1));
+            uint4 a0h, a0l, a1h, a1l, b0h, b0l, b1h, b1l;
+            ft_gen8(cxa.x, cxa.y, g, a0h, a0l);
+            ft_gen8(cxa.z, cxa.w, g, a1h, a1l);
+            __syncthreads();   // the previous readers are done with sB
+            stage(P, 0);
+            __syncthreads();
+            // position i: MFMAs on LDS buffer i % 2 with A set `cur`; the A set of i + 1
+            // from its coarse factors `cxn`; chunk i + 1's rows (`Sn`) into the other
+            // buffer; chunk i + 2's rows and factors loaded into `Sf` and `cxf`
+            auto step = [&](int i, int buf, const uint4 &c0h, const uint4 &c0l, const uint4 &c1h, const uint4 &c1l,
+                            uint4 &d0h, uint4 &d0l, uint4 &d1h, uint4 &d1l, const B3Stage &Sn, B3Stage &Sf,
+                            const float4 &cxn, float4 &cxf) {
+                cxf = *reinterpret_cast<const float4 *>(fx + 2 * cidx(i + 2));
+                fetch(Sf, cidx(i + 2));
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int ks = 0; ks < B3_KC / 16; ks++) {
+                    const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(ks ? &c1h : &c0h);
+                    const bf16x8 al = *reinterpret_cast<const bf16x8 *>(ks ? &c1l : &c0l);
+                    bf16x8 bh[NBK], bl[NBK];
+#pragma unroll
+                    for (int b = 0; b < NBK; b++) {
+                        const int o = (b * 32 + lr) * B3_STRIDE + ks * 16 + 8 * lh;
+                        bh[b] = *reinterpret_cast<const bf16x8 *>(&sB[buf][0][o]);
+                        bl[b] = *reinterpret_cast<const bf16x8 *>(&sB[buf][1][o]);
+                    }
+#pragma unroll
+                    for (int b = 0; b < NBK; b++)
+                        acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[b], acc[b], 0, 0, 0);
+#pragma unroll
+                    for (int b = 0; b < NBK; b++)
+                        acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[b], acc[b], 0, 0, 0);
+#pragma unroll
+                    for (int b = 0; b < NBK; b++)
+                        acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[b], acc[b], 0, 0, 0);
+                }
+                ft_gen8(cxn.x, cxn.y, g, d0h, d0l);
+                ft_gen8(cxn.z, cxn.w, g, d1h, d1l);
+                // the other buffer was last read at position i - 1, before the previous barrier
+                stage(Sn, buf ^ 1);
+                __syncthreads();
+            };
+            int i = 0;
+            for (; i + 1 < n; i += 2) {
+                step(i, 0, a0h, a0l, a1h, a1l, b0h, b0l, b1h, b1l, Q, P, cxb, cxa);
+                step(i + 1, 1, b0h, b0l, b1h, b1l, a0h, a0l, a1h, a1l, P, Q, cxa, cxb);
+            }
+            if (i < n)
+                step(i, 0, a0h, a0l, a1h, a1l, b0h, b0l, b1h, b1l, Q, P, cxb, cxa);
+            // y-reduction: point 8(r/4) + 4 lh + r%4 has Re T in register r and Im T in
+            // r+8 at column y = y0 + 32 b + lr = 16 (y0/16 + 2 b + lr/16) + lr % 16.
+            // No branch on p < nuv (points past nuv read point 0's factors and are never
+            // stored), so all factor loads of the group can be in flight together.
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const int p = p0 + 8 * (r >> 2) + 4 * lh + (r & 3);
+                const float2 *fy = FY + (size_t)(p < nuv ? p : 0) * ft;
+                const float2 fj = fy[nco + (lr & 15)];
+                const float2 *cy = fy + (y0 >> 4) + (lr >> 4);
+#pragma unroll
+                for (int b = 0; b < NBK; b++) {
+                    const float2 yv = cmul(cy[2 * b], fj);
+                    const float tr = acc[b][r], ti = acc[b][r + 8];
+                    vr[r] += tr * yv.x - ti * yv.y;
+                    vi[r] += tr * yv.y + ti * yv.x;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+#pragma unroll
+            for (int o = 1; o < 32; o <<= 1) {
+                vr[r] += __shfl_xor(vr[r], o, 64);
+                vi[r] += __shfl_xor(vi[r], o, 64);
+            }
+        }
+        if (lr == 0) {
+            const bool whole = c0 == 0 && c1 == nchunk;
+            const float f = whole && scale ? *scale : 1.0f;
+            float2 *dst = whole ? vis : vpart + (c0 == 0 ? 0 : nuv);
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const int p = p0 + 8 * (r >> 2) + 4 * lh + (r & 3);
+                if (p < nuv)
+                    dst[p] = make_float2(vr[r] * f, vi[r] * f);
+            }
+        }
+    }
+}
+
+template __global__ void k_vis_fwd_b3f<2>(const unsigned short *, const unsigned short *, const float2 *,
+                                          const float2 *, int, int, int, const float *, float2 *, float2 *);
+template __global__ void k_vis_fwd_b3f<4>(const unsigned short *, const unsigned short *, const float2 *,
+                                          const float2 *, int, int, int, const float *, float2 *, float2 *);
+template __global__ void k_vis_fwd_b3f<6>(const unsigned short *, const unsigned short *, const float2 *,
+                                          const float2 *, int, int, int, const float *, float2 *, float2 *);
+template __global__ void k_vis_fwd_b3f<8>(const unsigned short *, const unsigned short *, const float2 *,
+                                          const float2 *, int, int, int, const float *, float2 *, float2 *);
+
+// Adjoint as k_adj_b3w (256 x 256 quadrant per workgroup and uv split, 8 waves of
+// 2 x 4 accumulator tiles, LDS double-buffered per round of 16 points), with both
+// tiles built from the factored tables.  Builder role of thread t: point q = t % 16
+// of the round and the row (A) and column (B) runs 4 (t / 16) + 128 h .. + 3,
+// h = 0, 1: the four entries of a run share one coarse factor and the two runs
+// the same four fine factors.  In a wave the 16 points are consecutive 32-bit
+// words and the four runs start 80 words apart (bank offsets 0, 16, 32, 48), so
+// the builders' LDS writes are free of bank conflicts.
+// Three stages per round k: the MFMAs of round k read their tile, the tile of
+// round k + 1 is built from factor loads issued at the end of the previous round
+// (in the same basic block as the MFMAs, so that the builder's VALU work can fill
+// the MFMA pipeline's shadow), then the loads of round k + 2 are issued into the
+// same registers.  Rounds past the last reload the last round (valid addresses,
+// tiles never read).
+// Adjoint as k_adj_b3w (256 x 256 quadrant per workgroup and uv split, 8 waves of
+// 2 x 4 accumulator tiles, LDS double-buffered per round of 16 points), with both
+// tiles built from the factored tables.  Builder role of thread t: point q = t % 16
+// of the round and the row (A) and column (B) runs 4 (t / 16) + 128 h .. + 3,
+// h = 0, 1: the four entries of a run share one coarse factor and the two runs
+// the same four fine factors.  In a wave the 16 points are consecutive 32-bit
+// words and the four runs start 80 words apart (bank offsets 0, 16, 32, 48), so
+// the builders' LDS writes are free of bank conflicts.
+__global__ __launch_bounds__(512) void k_adj_b3wf(const float2 *__restrict__ coef, const float2 *__restrict__ FX,
+                                                  const float2 *__restrict__ FY, int nuv, int W, int p_per_split,
+                                                  float *__restrict__ part)
+{
+    __shared__ __attribute__((aligned(16))) unsigned short sA[2][2][AW_T * AW_STRIDE];   // [buf][hi/lo]
+    __shared__ __attribute__((aligned(16))) unsigned short sB[2][2][AW_T * AW_STRIDE];
+    constexpr int WROW = AW_STRIDE / 2;   // 32-bit words per LDS row
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 31, lh = lane >> 5;
+    const int ft = ft_len(W), nco = W / 16;
+    const int qx = W / AW_T;
+    const int y0 = (blockIdx.x / qx) * AW_T, x0 = (blockIdx.x % qx) * AW_T;
+    const int pstart = blockIdx.y * p_per_split;
+    const int pend = min(pstart + p_per_split, nuv);
+    const int q = tid & 15, o4 = 4 * (tid >> 4);
+    float2 ca, cy0, cy1, cx0, cx1;
+    float4 fy01, fy23, fx01, fx23;
+    auto load = [&](int pr) {
+        // points past the split get a = 0 (their entries, built from the split's last
+        // point, then contribute nothing)
+        const int p = pr + q;
+        const int pv = min(p, pend - 1);
+        const float2 cv = coef[pv];
+        const float keep = p < pend ? 1.f : 0.f;   // a multiply, not a branch around the load
+        ca = make_float2(cv.x * keep, cv.y * keep);
+        const float2 *fy = FY + (size_t)pv * ft, *fx = FX + (size_t)pv * ft;
+        const float4 *fyf = reinterpret_cast<const float4 *>(fy + nco + (o4 & 15));
+        const float4 *fxf = reinterpret_cast<const float4 *>(fx + nco + (o4 & 15));
+        fy01 = fyf[0];
+        fy23 = fyf[1];
+        fx01 = fxf[0];
+        fx23 = fxf[1];
+        cy0 = fy[(y0 + o4) >> 4];
+        cy1 = fy[(y0 + o4 + 128) >> 4];
+        cx0 = fx[(x0 + o4) >> 4];
+        cx1 = fx[(x0 + o4 + 128) >> 4];
+    };
+    auto store = [&](int buf) {
+        unsigned *ah = reinterpret_cast<unsigned *>(sA[buf][0]), *al = reinterpret_cast<unsigned *>(sA[buf][1]);
+        unsigned *bh = reinterpret_cast<unsigned *>(sB[buf][0]), *bl = reinterpret_cast<unsigned *>(sB[buf][1]);
+        const float2 fyv[4] = {make_float2(fy01.x, fy01.y), make_float2(fy01.z, fy01.w), make_float2(fy23.x, fy23.y),
+                               make_float2(fy23.z, fy23.w)};
+        const float2 fxv[4] = {make_float2(fx01.x, fx01.y), make_float2(fx01.z, fx01.w), make_float2(fx23.x, fx23.y),
+                               make_float2(fx23.z, fx23.w)};
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            // A[y][2q] = Re(a Y[p][y]), A[y][2q + 1] = -Im(a Y[p][y])
+            const float2 acy = cmul(ca, h ? cy1 : cy0);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float2 w = cmul(acy, fyv[j]);
+                unsigned hh, ll;
+                split2_bf16(w.x, -w.y, hh, ll);
+                const int o = (o4 + 128 * h + j) * WROW + q;
+                ah[o] = hh;
+                al[o] = ll;
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            // B[2q][x] = Re X[p][x], B[2q + 1][x] = Im X[p][x], stored by column x
+            const float2 cxh = h ? cx1 : cx0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float2 w = cmul(cxh, fxv[j]);
+                unsigned hh, ll;
+                split2_bf16(w.x, w.y, hh, ll);
+                const int o = (o4 + 128 * h + j) * WROW + q;
+                bh[o] = hh;
+                bl[o] = ll;
+            }
+        }
+    };
+    const int wy = (wave >> 1) * 64, wx = (wave & 1) * 128;
+    v16f acc[2][4];
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+                acc[m][n][r] = 0.f;
+    int buf = 0;
+    if (pstart < pend) {
+        load(pstart);
+        store(0);
+    }
+    __syncthreads();
+    for (int pr = pstart; pr < pend; pr += AB3_PR) {
+        const bool more = pr + AB3_PR < pend;
+        if (more)
+            load(pr + AB3_PR);
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            bf16x8 ah[2], al[2], bh[4], bl[4];
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+                const int o = (wy + 32 * m + lr) * AW_STRIDE + 16 * ks + 8 * lh;
+                ah[m] = *reinterpret_cast<const bf16x8 *>(&sA[buf][0][o]);
+                al[m] = *reinterpret_cast<const bf16x8 *>(&sA[buf][1][o]);
+            }
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                const int o = (wx + 32 * n + lr) * AW_STRIDE + 16 * ks + 8 * lh;
+                bh[n] = *reinterpret_cast<const bf16x8 *>(&sB[buf][0][o]);
+                bl[n] = *reinterpret_cast<const bf16x8 *>(&sB[buf][1][o]);
+            }
+            // product-major over the eight accumulators (same per-tile order hh, hl, lh)
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+                for (int n = 0; n < 4; n++)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m], bh[n], acc[m][n], 0, 0, 0);
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+                for (int n = 0; n < 4; n++)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m], bl[n], acc[m][n], 0, 0, 0);
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+                for (int n = 0; n < 4; n++)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[m], bh[n], acc[m][n], 0, 0, 0);
+        }
+        // the other buffer was last read in the previous round, before its barrier
+        if (more)
+            store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    float *out = part + (size_t)blockIdx.y * W * W;
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int y = y0 + wy + 32 * m + 8 * (r >> 2) + 4 * lh + (r & 3);
+                out[(size_t)y * W + x0 + wx + 32 * n + lr] = acc[m][n][r];
+            }
+}

@@ -311,3 +311,54 @@ def test_batch_matches_single_runs():
         assert r.iterations == its and r.J == J and bool(r.converged) == bool(conv)
         assert np.array_equal(c.get("ci"), img)
         c.close()
+
+
+def _ft_cases():
+    return [
+        ("w64", dict(seed=11, W=64)),
+        ("w192", dict(seed=12, W=192, ntel=7, nch=6, nsnap=5)),
+        ("w256", dict(seed=13, W=256, pix=0.05, ntel=7, nch=8, nsnap=6)),
+        ("w512", dict(seed=14, W=512, pix=0.05, ntel=6, nch=4, nsnap=3)),
+    ]
+
+
+@pytest.mark.parametrize("name,kw", _ft_cases(), ids=[c[0] for c in _ft_cases()])
+def test_bf16x3_factored_tables(name, kw, orc):
+    """BF16x3 DFTs that rebuild table entries from factored tables (tables set from
+    uv) against the same DFTs on the full split tables (tables uploaded with
+    gpu_copy_dft) and against the FP64 oracle.  W = 256 and 512 take the factored
+    256 x 256 quadrant adjoint (512: four quadrants, offsets 0 and 256)."""
+    prob = synth.make_problem("cfg1", **kw)
+    print(f"case {name}: W={prob.W} nuv={prob.nuv} nbis={prob.nbis}", flush=True)
+    P = orc.unpack(prob)
+    img = _image(prob, 5)
+    X, Y = orc.tables(prob.W, prob.au(), prob.av())
+    out = []
+    for upload in (False, True):
+        if upload:
+            ctx = GpairContext(0)
+            ctx.gpu_copy_data(prob.data, prob.err, prob.nuv, prob.phasor, prob.npow, prob.uvpnt4(), prob.sign4(),
+                              prob.model, prob.W)
+            ctx.gpu_copy_dft(X.T.astype(np.complex64), Y.T.astype(np.complex64))
+            ctx.gpu_build_kernels()
+            ctx.gpu_copy_image(img)
+        else:
+            ctx = GpairContext.from_problem(prob, image=img)
+        with ctx:
+            ctx.set_precision(1)
+            chi2 = ctx.gpu_get_chi2_curr()
+            vis = ctx.vis().astype(np.complex128)
+            ctx.gpu_compute_data_gradient_curr()
+            out.append((chi2, vis, ctx.get("dg").astype(np.float64)))
+    J, chi2_o, S_o, vis_o, mock_o, F = orc.objective(P, img.astype(np.float64), ALPHA)
+    gd_o = orc.gradient(P, img.astype(np.float64), ALPHA, vis_o, mock_o)[1]
+    vmax, gmax = np.abs(vis_o).max(), np.abs(gd_o).max()
+    for tag, (chi2, vis, gd) in zip(("factored", "full"), out):
+        ev = np.abs(vis - vis_o).max() / vmax
+        ec = abs(chi2 - chi2_o) / chi2_o
+        eg = np.abs(gd - gd_o).max() / gmax
+        print(f"  {tag}: vis {ev:.2e} chi2 {ec:.2e} grad {eg:.2e}", flush=True)
+        assert ev <= 3e-5 and ec <= 2e-5 and eg <= 2e-4
+    (c1, v1, g1), (c2, v2, g2) = out
+    assert np.abs(v1 - v2).max() <= 3e-5 * vmax
+    assert np.abs(g1 - g2).max() <= 2e-4 * gmax
