@@ -195,7 +195,7 @@ extern "C" int gpu_init(gpair_ctx **out, int device)
     }
     int rc;
     if ((rc = dalloc(c, &c->scal, GPAIR_NSCAL)) || (rc = dalloc(c, &c->invflux, 4)) ||
-        (rc = dalloc(c, &c->part, (size_t)RED_MAX_BLOCKS * RED_MAX_NV)) || (rc = dalloc(c, &c->ticket, 4)) ||
+        (rc = dalloc(c, &c->part, (size_t)RED_MAX_BLOCKS * RED_MAX_NV)) || (rc = dalloc(c, &c->ticket, GPAIR_NTICKET)) ||
         (rc = dalloc(c, &c->ls_out, 2 * LS_K))) {
         gpu_cleanup(c);
         return rc;
@@ -205,7 +205,7 @@ extern "C" int gpu_init(gpair_ctx **out, int device)
         gpu_cleanup(c);
         return GPAIR_E_NOMEM;
     }
-    if (hipMemsetAsync(c->ticket, 0, 4 * sizeof(unsigned), c->stream) != hipSuccess ||
+    if (hipMemsetAsync(c->ticket, 0, GPAIR_NTICKET * sizeof(unsigned), c->stream) != hipSuccess ||
         hipMemsetAsync(c->scal, 0, GPAIR_NSCAL * sizeof(double), c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess) {
         gpu_cleanup(c);
@@ -782,20 +782,28 @@ static int launch_chi2(gpair_ctx *c, const float *img)
 
 static int launch_adjoint(gpair_ctx *c);
 
+static int launch_adjoint_dft(gpair_ctx *c, int *nsplit_out);
+
+// per-uv adjoint coefficients from the visibilities and mock data of the last chi2
+static int launch_coef(gpair_ctx *c)
+{
+    k_coef<<<tuned_blocks(c->coef_blocks, red_blocks(c->nuv)), RED_THREADS, 0, c->stream>>>(
+        c->vis, c->mock, c->data, c->err, c->phasor, c->tri, c->csr_ptr, c->csr_ent, c->npow, c->nuv, c->coef,
+        c->part, c->ticket, c->scal);
+    LAUNCHCHK(c);
+    return GPAIR_OK;
+}
+
 static int launch_data_gradient(gpair_ctx *c, const float *img)
 {
     int rc;
-    if ((rc = launch_flux(c, img)))
+    if ((rc = launch_flux(c, img)) || (rc = launch_coef(c)))
         return rc;
-    k_coef<<<tuned_blocks(c->coef_blocks, red_blocks(c->nuv)), RED_THREADS, 0, c->stream>>>(c->vis, c->mock, c->data, c->err, c->phasor, c->tri,
-                                                               c->csr_ptr, c->csr_ent, c->npow, c->nuv, c->coef,
-                                                               c->part, c->ticket, c->scal);
-    LAUNCHCHK(c);
     return launch_adjoint(c);
 }
 
-// adjoint DFT of the coefficients in c->coef and the fixed-order sum of its partials
-static int launch_adjoint(gpair_ctx *c)
+// adjoint DFT of the coefficients in c->coef into the split partials; *nsplit_out = their count
+static int launch_adjoint_dft(gpair_ctx *c, int *nsplit_out)
 {
     int rc;
     int nsplit = c->adj_nsplit;
@@ -826,6 +834,16 @@ static int launch_adjoint(gpair_ctx *c)
                                                              c->adj_part)));
     }
     LAUNCHCHK(c);
+    *nsplit_out = nsplit;
+    return GPAIR_OK;
+}
+
+// adjoint DFT of the coefficients in c->coef and the fixed-order sum of its partials
+static int launch_adjoint(gpair_ctx *c)
+{
+    int rc, nsplit;
+    if ((rc = launch_adjoint_dft(c, &nsplit)))
+        return rc;
     k_grad_finish<<<(c->n + GF_PIX - 1) / GF_PIX, 256, 0, c->stream>>>(c->adj_part, nsplit, c->n, c->scal,
                                                                           c->gdata.p);
     LAUNCHCHK(c);

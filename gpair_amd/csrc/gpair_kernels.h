@@ -35,6 +35,7 @@ enum {
 #define RED_THREADS 256
 #define RED_MAX_BLOCKS 2048
 #define RED_MAX_NV 32
+#define GPAIR_NTICKET 16   // reduction tickets per context (the line-search batch uses 8)
 
 // batched line search: LS_K candidate steps, s_0 = 0 and s_k = s_ref * 2^((k - LS_TOP) / 2)
 #define LS_K 32

@@ -406,6 +406,52 @@ __global__ __launch_bounds__(256) void k_grad_finish(const float *__restrict__ p
     }
 }
 
+// k_grad_finish, then per pixel the entropy gradient (stale value kept where I <= 0
+// or M <= 0, as k_entropy_grad) and the criterion gradient g = gdata - alpha gent
+// (as k_crit_grad): one launch instead of three in the accelerated loop.
+__global__ __launch_bounds__(256) void k_grad_finish_crit(const float *__restrict__ part, int nsplit, int n,
+                                                          const double *__restrict__ scal,
+                                                          const float *__restrict__ img,
+                                                          const float *__restrict__ model, float alpha,
+                                                          float *__restrict__ gdata, float *__restrict__ gent,
+                                                          float *__restrict__ g)
+{
+    __shared__ double sh[4][GF_PIX];
+    const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * GF_PIX + lane;
+    const int per = (nsplit + 3) / 4;
+    const int k0 = q * per, k1 = min(nsplit, k0 + per);
+    double s = 0.0;
+    if (i < n) {
+        int k = k0;
+        for (; k + 8 <= k1; k += 8) {
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                v[j] = part[(size_t)(k + j) * n + i];
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                s += (double)v[j];
+        }
+        for (; k < k1; k++)
+            s += (double)part[(size_t)k * n + i];
+    }
+    sh[q][lane] = s;
+    __syncthreads();
+    if (q == 0 && i < n) {
+        const double t = ((sh[0][lane] + sh[1][lane]) + sh[2][lane]) + sh[3][lane];
+        const float gd = (float)((t - scal[SC_C0]) / scal[SC_FLUX]);
+        gdata[i] = gd;
+        const float I = img[i], M = model[i];
+        float ge = gent[i];
+        if (I > 0.f && M > 0.f) {
+            ge = -logf(I / M);
+            gent[i] = ge;
+        }
+        g[i] = gd - alpha * ge;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // adjoint DFT in split precision (BF16x3) on v_mfma_f32_32x32x16_bf16.
 // G[y][x] = sum_k A[y][k] B[k][x], k = 2 p + c:
@@ -812,20 +858,26 @@ __global__ void k_vis_to_double(const float2 *__restrict__ vis, int nuv, double2
         Vd[p] = make_double2(vis[p].x, vis[p].y);
 }
 
-// change of chi2 along the line for all LS_K candidates:
-// V_k = (F_I V + s_k U_d) / (F_I + s_k F_d); out[k] = sum r(V_k)^2 - r(V)^2
-__global__ __launch_bounds__(RED_THREADS) void k_ls_dchi2(const double2 *__restrict__ Vd,
-                                                          const float2 *__restrict__ Ud,
-                                                          const double *__restrict__ scal, double s_ref,
-                                                          const float *__restrict__ data,
-                                                          const float *__restrict__ err,
-                                                          const float2 *__restrict__ phasor,
-                                                          const int4 *__restrict__ tri, int npow, int nbis,
-                                                          double *part, unsigned *ticket, double *out)
+// Batched line search, one launch per candidate batch: block columns
+// [0, nb_chi2) take the chi2 change of every candidate, columns [nb_chi2,
+// nb_chi2 + nb_ent) the entropy change; block row y takes candidates
+// 8 y .. 8 y + 7 (LS_G per row: fewer live registers per lane).  Both halves are
+// partitioned reductions of one item per thread per partition, with their own
+// partials and tickets (part: [2][LS_K / LS_G][RED_PARTS][LS_G], tickets
+// 0..3 chi2, 4..7 entropy), so their result does not depend on nb_chi2 / nb_ent.
+// out[k] = chi2 change of candidate k, out[LS_K + k] = its entropy change.
+#define LS_SPAN RED_THREADS
+
+// chi2(s_k) - chi2(0) from V = U / F in FP64 and the direction's DFT U_d:
+// V(s) = (U + s U_d) / (F + s F_d)
+__device__ __forceinline__ void ls_chi2_part(const double2 *__restrict__ Vd, const float2 *__restrict__ Ud,
+                                             const double *__restrict__ scal, double s_ref,
+                                             const float *__restrict__ data, const float *__restrict__ err,
+                                             const float2 *__restrict__ phasor, const int4 *__restrict__ tri,
+                                             int npow, int nbis, int grp, int bx, int nbx, double *part,
+                                             unsigned *ticket, double *out)
 {
-    // candidates g0 .. g0 + LS_G - 1 in this block row: LS_K / LS_G rows of blocks,
-    // each with its own partials, ticket and outputs (fewer live registers per lane)
-    const int grp = blockIdx.y, g0 = grp * LS_G;
+    const int g0 = grp * LS_G;
     const double FI = scal[SC_FLUX], Fd = scal[SC_FD];
     double sk[LS_G], ivk[LS_G];
 #pragma unroll
@@ -879,17 +931,17 @@ __global__ __launch_bounds__(RED_THREADS) void k_ls_dchi2(const double2 *__restr
             }
         }
     };
-    partitioned_reduce<LS_G>(npow + nbis, item, part + (size_t)grp * RED_PARTS * LS_G, ticket + grp, out + g0);
+    partitioned_reduce_at<LS_G>(npow + nbis, LS_SPAN, item, part + (size_t)grp * RED_PARTS * LS_G, ticket + grp,
+                                out + g0, bx, nbx);
 }
 
-// change of the entropy along the line: out[k] = sum S(max(I + s_k d, minval)) - S(I)
-__global__ __launch_bounds__(RED_THREADS) void k_ls_dentropy(const float *__restrict__ img,
-                                                             const float *__restrict__ d,
-                                                             const float *__restrict__ model, double s_ref,
-                                                             double minval, int n, double *part, unsigned *ticket,
-                                                             double *out)
+// S(max(I + s_k d, minval)) - S(I), summed over the pixels
+__device__ __forceinline__ void ls_entropy_part(const float *__restrict__ img, const float *__restrict__ d,
+                                                const float *__restrict__ model, double s_ref, double minval, int n,
+                                                int grp, int bx, int nbx, double *part, unsigned *ticket,
+                                                double *out)
 {
-    const int grp = blockIdx.y, g0 = grp * LS_G;
+    const int g0 = grp * LS_G;
     float sk[LS_G];
 #pragma unroll
     for (int k = 0; k < LS_G; k++)
@@ -915,8 +967,34 @@ __global__ __launch_bounds__(RED_THREADS) void k_ls_dentropy(const float *__rest
             v[k] += (double)(dl * (1.0f - L) - t * lr);
         }
     };
-    partitioned_reduce<LS_G>(n, item, part + (size_t)grp * RED_PARTS * LS_G, ticket + grp, out + g0);
+    partitioned_reduce_at<LS_G>(n, LS_SPAN, item, part + (size_t)grp * RED_PARTS * LS_G, ticket + grp, out + g0,
+                                bx, nbx);
 }
+
+__global__ __launch_bounds__(RED_THREADS) void k_ls_batch(const double2 *__restrict__ Vd,
+                                                          const float2 *__restrict__ Ud,
+                                                          const double *__restrict__ scal, double s_ref,
+                                                          const float *__restrict__ data,
+                                                          const float *__restrict__ err,
+                                                          const float2 *__restrict__ phasor,
+                                                          const int4 *__restrict__ tri, int npow, int nbis,
+                                                          const float *__restrict__ img, const float *__restrict__ d,
+                                                          const float *__restrict__ model, double minval, int n,
+                                                          int nb_chi2, int nb_ent, double *part, unsigned *ticket,
+                                                          double *out)
+{
+    constexpr int ROWS = LS_K / LS_G;
+    const int grp = blockIdx.y, bx = blockIdx.x;
+    if (bx < nb_chi2)
+        ls_chi2_part(Vd, Ud, scal, s_ref, data, err, phasor, tri, npow, nbis, grp, bx, nb_chi2, part, ticket, out);
+    else
+        ls_entropy_part(img, d, model, s_ref, minval, n, grp, bx - nb_chi2, nb_ent,
+                        part + (size_t)ROWS * RED_PARTS * LS_G, ticket + ROWS, out + LS_K);
+}
+
+// block columns of each half of k_ls_batch: one per partition unless a smaller count is given
+__host__ inline int ls_nparts_chi2(int npow, int nbis) { return red_nparts(npow + nbis, LS_SPAN); }
+__host__ inline int ls_nparts_ent(int n) { return red_nparts(n, LS_SPAN); }
 
 // trial image I_t = max(I + s d, minval), the change delta = I_t - I, and
 // out[0] = sum delta, out[1] = S(I_t) - S(I); with th set, also the BF16 hi / lo
@@ -952,26 +1030,23 @@ __global__ __launch_bounds__(RED_THREADS) void k_commit_step(const float *__rest
     reduce_finish<2>(v, part, ticket, out, nullptr);
 }
 
-// visibilities after the change: Vn = (F_I V + U_delta) / (F_I + F_delta) (FP64),
-// also as floats for the adjoint
-__global__ void k_vis_update(const double2 *__restrict__ Vd, const float2 *__restrict__ Udelta,
-                             const double *__restrict__ scal, int nuv, double2 *__restrict__ Vn,
-                             float2 *__restrict__ vis_f)
+// visibilities after the committed change: (F V + U_delta) / (F + F_delta)
+__device__ __forceinline__ double2 vis_after(const double2 *__restrict__ Vd, const float2 *__restrict__ Udelta,
+                                             double FI, double iv, int p)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nuv)
-        return;
-    const double FI = scal[SC_FLUX], Fdl = scal[SC_FDELTA];
-    const double iv = 1.0 / (FI + Fdl);
     const double2 a = Vd[p];
-    const double2 r = make_double2((FI * a.x + Udelta[p].x) * iv, (FI * a.y + Udelta[p].y) * iv);
-    Vn[p] = r;
-    vis_f[p] = make_float2((float)r.x, (float)r.y);
+    const float2 u = Udelta[p];
+    return make_double2((FI * a.x + u.x) * iv, (FI * a.y + u.y) * iv);
 }
 
-// exact chi2 change between Vd and Vn, mock data of Vn; out[0] = sum r(Vn)^2 - r(Vd)^2
+// exact chi2 change between Vd and the visibilities after the committed change,
+// mock data of the latter; out[0] = sum r(Vn)^2 - r(Vd)^2.  The same launch writes
+// Vn (FP64) and vis_f (float) for every uv point; the T3 terms rebuild their legs
+// with the same expression.
 __global__ __launch_bounds__(RED_THREADS) void k_commit_dchi2(const double2 *__restrict__ Vd,
-                                                              const double2 *__restrict__ Vn,
+                                                              const float2 *__restrict__ Udelta,
+                                                              const double *__restrict__ scal, int nuv,
+                                                              double2 *__restrict__ Vn, float2 *__restrict__ vis_f,
                                                               const float *__restrict__ data,
                                                               const float *__restrict__ err,
                                                               const float2 *__restrict__ phasor,
@@ -979,11 +1054,17 @@ __global__ __launch_bounds__(RED_THREADS) void k_commit_dchi2(const double2 *__r
                                                               float *__restrict__ mock_new, double *part,
                                                               unsigned *ticket, double *out)
 {
+    const double FI = scal[SC_FLUX], iv = 1.0 / (FI + scal[SC_FDELTA]);
+    for (int p = blockIdx.x * RED_THREADS + threadIdx.x; p < nuv; p += gridDim.x * RED_THREADS) {
+        const double2 r = vis_after(Vd, Udelta, FI, iv, p);
+        Vn[p] = r;
+        vis_f[p] = make_float2((float)r.x, (float)r.y);
+    }
     double v[1] = {0.0};
     const int total = npow + nbis;
     for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < total; i += gridDim.x * RED_THREADS) {
         if (i < npow) {
-            const double2 a = Vd[i], b = Vn[i];
+            const double2 a = Vd[i], b = vis_after(Vd, Udelta, FI, iv, i);
             const double d0 = data[i], e = err[i];
             const double m = b.x * b.x + b.y * b.y;
             const double r0 = (a.x * a.x + a.y * a.y - d0) * e, r1 = (m - d0) * e;
@@ -997,8 +1078,10 @@ __global__ __launch_bounds__(RED_THREADS) void k_commit_dchi2(const double2 *__r
             const double dr = data[kk], di = data[kk + 1], er = err[kk], ei = err[kk + 1];
             const double2 T0 =
                 dcmul(dcmul(dcmul(dleg(Vd[tr.x], tr.w & 1), dleg(Vd[tr.y], tr.w & 2)), dleg(Vd[tr.z], tr.w & 4)), rho);
-            const double2 T1 =
-                dcmul(dcmul(dcmul(dleg(Vn[tr.x], tr.w & 1), dleg(Vn[tr.y], tr.w & 2)), dleg(Vn[tr.z], tr.w & 4)), rho);
+            const double2 T1 = dcmul(dcmul(dcmul(dleg(vis_after(Vd, Udelta, FI, iv, tr.x), tr.w & 1),
+                                                 dleg(vis_after(Vd, Udelta, FI, iv, tr.y), tr.w & 2)),
+                                           dleg(vis_after(Vd, Udelta, FI, iv, tr.z), tr.w & 4)),
+                                     rho);
             mock_new[kk] = (float)T1.x;
             mock_new[kk + 1] = (float)T1.y;
             const double a0 = (T0.x - dr) * er, a1 = (T0.y - di) * ei;
