@@ -1164,21 +1164,6 @@ static int read_scalars(gpair_ctx *c, int slot, int count, double *out)
     return GPAIR_OK;
 }
 
-// vertex of the parabola through (a, Ja), (b, Jb), (c, Jc), a < b < c, when it is a
-// minimum inside (a, c); otherwise b
-static double parabola_step(double a, double b, double cc, double Ja, double Jb, double Jc)
-{
-    const double p = (b - a) * (Jb - Jc), q = (b - cc) * (Jb - Ja);
-    const double den = p - q;
-    if (!(den < 0.0 || den > 0.0))
-        return b;
-    const double s = b - 0.5 * ((b - a) * p - (b - cc) * q) / den;
-    const double curv = (Ja - Jb) / (a - b) - (Jb - Jc) / (b - cc);
-    if (!(curv < 0.0) || !(s > a && s < cc))
-        return b;
-    return s;
-}
-
 // chi2 and entropy of an image (leaves vis / mock of that image)
 static int exact_eval(gpair_ctx *c, const float *img, double *chi2, double *S)
 {

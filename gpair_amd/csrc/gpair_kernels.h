@@ -29,7 +29,9 @@ enum {
     SC_BETA = 13,    // beta the direction kernel used
     SC_PXF = 14,     // flux of the pixel-move state
     SC_PXC = 15,     // chi2 of the pixel-move state
-    GPAIR_NSCAL = 16
+    SC_STEP = 16,    // step chosen on the device from a line-search batch (0: none)
+    SC_KBEST = 17,   // its candidate index (-1: no decrease, or not a descent direction)
+    GPAIR_NSCAL = 18
 };
 
 #define RED_THREADS 256
@@ -41,6 +43,33 @@ enum {
 #define LS_K 32
 #define LS_TOP 26
 #define LS_G 8   // candidates per block row of the line-search kernels (grid.y = LS_K / LS_G)
+
+// candidate k of a batch: s_0 = 0, s_k = s_ref 2^((k - LS_TOP) / 2), built exactly
+// (a power of two times 1 or the rounded sqrt 2) so host and device agree bit for bit
+__host__ __device__ inline double ls_step(double s_ref, int k)
+{
+    if (k == 0)
+        return 0.0;
+    const int e = k - LS_TOP;
+    const int h = e >= 0 ? e / 2 : -((1 - e) / 2);   // floor(e / 2)
+    return s_ref * ldexp((e & 1) ? 1.4142135623730951 : 1.0, h);
+}
+
+// vertex of the parabola through (a, Ja), (b, Jb), (c, Jc), a < b < c, when it is a
+// minimum inside (a, c); otherwise b.  No contraction: host and device round alike.
+__host__ __device__ inline double parabola_step(double a, double b, double cc, double Ja, double Jb, double Jc)
+{
+#pragma clang fp contract(off)
+    const double p = (b - a) * (Jb - Jc), q = (b - cc) * (Jb - Ja);
+    const double den = p - q;
+    if (!(den < 0.0 || den > 0.0))
+        return b;
+    const double s = b - 0.5 * ((b - a) * p - (b - cc) * q) / den;
+    const double curv = (Ja - Jb) / (a - b) - (Jb - Jc) / (b - cc);
+    if (!(curv < 0.0) || !(s > a && s < cc))
+        return b;
+    return s;
+}
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef float v16f __attribute__((ext_vector_type(16)));

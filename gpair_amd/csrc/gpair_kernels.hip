@@ -829,10 +829,6 @@ __global__ __launch_bounds__(RED_THREADS) void k_pcg_dir(const float *__restrict
 }
 
 // candidate 0 is s = 0 (the linearised current image, the common reference)
-__device__ inline double ls_step(double s_ref, int k)
-{
-    return k == 0 ? 0.0 : s_ref * exp2(0.5 * (double)(k - LS_TOP));
-}
 
 // The line search and the commit work on CHANGES of the objective in FP64: the
 // current visibilities are kept in FP64 (Vd) and only DFTs of the direction d
@@ -996,18 +992,51 @@ __global__ __launch_bounds__(RED_THREADS) void k_ls_batch(const double2 *__restr
 __host__ inline int ls_nparts_chi2(int npow, int nbis) { return red_nparts(npow + nbis, LS_SPAN); }
 __host__ inline int ls_nparts_ent(int n) { return red_nparts(n, LS_SPAN); }
 
+// Step from a line-search batch, on the device so that the commit can follow the
+// batch without a read-back: when d is a descent direction (d.g < 0, sum |d| > 0)
+// and some candidate k > 0 lowers the objective, the best one (first minimum of
+// dJ_k = dchi2_k - alpha dS_k), refined by the parabola through its neighbours;
+// otherwise step 0 and index -1.  The host reads both back with the commit's
+// results.  One thread.
+__global__ void k_ls_choose(const double *__restrict__ ls, double alpha, double s_ref, double *scal)
+{
+#pragma clang fp contract(off)
+    if (blockIdx.x != 0 || threadIdx.x != 0)
+        return;
+    double s = 0.0, kb = -1.0;
+    if (scal[SC_W1] < 0.0 && scal[SC_DABS] > 0.0) {
+        double dJ[LS_K];
+        int b = 0;
+        for (int k = 0; k < LS_K; k++) {
+            dJ[k] = ls[k] - alpha * ls[LS_K + k];
+            if (dJ[k] < dJ[b])
+                b = k;
+        }
+        if (b > 0) {
+            kb = b;
+            s = ls_step(s_ref, b);
+            if (b < LS_K - 1)
+                s = parabola_step(ls_step(s_ref, b - 1), s, ls_step(s_ref, b + 1), dJ[b - 1], dJ[b], dJ[b + 1]);
+        }
+    }
+    scal[SC_STEP] = s;
+    scal[SC_KBEST] = kb;
+}
+
 // trial image I_t = max(I + s d, minval), the change delta = I_t - I, and
 // out[0] = sum delta, out[1] = S(I_t) - S(I); with th set, also the BF16 hi / lo
 // split of delta for the forward DFT that follows
 __global__ __launch_bounds__(RED_THREADS) void k_commit_step(const float *__restrict__ img,
                                                              const float *__restrict__ d,
-                                                             const float *__restrict__ model, float s, float minval,
+                                                             const float *__restrict__ model, float s_arg,
+                                                             const double *__restrict__ s_dev, float minval,
                                                              int n, float *__restrict__ trial,
                                                              float *__restrict__ delta, double *part,
                                                              unsigned *ticket, double *out,
                                                              unsigned short *__restrict__ th,
                                                              unsigned short *__restrict__ tl)
 {
+    const float s = s_dev ? (float)*s_dev : s_arg;
     double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) {
         const float I = img[i];

@@ -658,9 +658,10 @@ int orc_pcg(int W, int nuv, int npow, int nbis, const double *au, const double *
                 if (w1 < 0.0 || beta == 0.0)
                     break;
                 /* the product scores a direction before it reads d.g back: a
-                 * conjugate direction that is not a descent one costs it one DFT
-                 * and one batch (counted here so the counters agree) */
-                nfwd++;
+                 * conjugate direction that is not a descent one costs it the DFT
+                 * of d, one batch and the commit launched behind the batch
+                 * (counted here so the counters agree) */
+                nfwd += 2;
                 nbatch++;
                 beta = 0.0;
             }
@@ -678,7 +679,11 @@ int orc_pcg(int W, int nuv, int npow, int nbis, const double *au, const double *
             /* candidate 0 is s = 0: every candidate is compared with the same
              * linearised evaluation of the current image */
             for (int round = 0; round < 4 && kbest < 0; round++) {
+                /* the product launches the commit at the chosen step right behind
+                 * each batch, before the host sees the batch: one DFT per batch,
+                 * which is the first commit try when the batch gives a step */
                 nbatch++;
+                nfwd++;
                 for (int k = 0; k < LS_K; k++) {
                     sk[k] = k == 0 ? 0.0 : s_ref * pow(2.0, 0.5 * (k - LS_TOP));
                     double c2 = chi2_along(&P, UI, Ud, FI, Fd, sk[k], vt, mt);
@@ -707,7 +712,8 @@ int orc_pcg(int W, int nuv, int npow, int nbis, const double *au, const double *
         for (int tries = 0; tries < 32; tries++) {
             step_image(n, img, d, s, minv, tmp);
             Jn = eval_J(&P, tmp, &chi2n, &Sn);
-            nfwd++;
+            if (tries > 0)
+                nfwd++;
             if (Jn <= J + c1 * s * w1) {
                 ok = 1;
                 break;
