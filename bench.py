@@ -10,8 +10,9 @@ timed region.  A step is one CG iteration.
   image-metric preconditioned PR-CG, batched line search with FP64 objective
   changes) over K timed iterations after W warm-up iterations.
 * to_target: wall time to chi2/ndof <= 1 from the prior for the accelerated
-  loop in FP32 and in BF16x3, from the learned start (network inference
-  included, BF16x3), and for the plain loop with the reference's rules
+  loop in FP32, in BF16x3 and with FP16 DFTs (power-of-two scaling; convergence
+  confirmed on an exact BF16x3 evaluation), from the learned start (network
+  inference included, BF16x3), and for the plain loop with the reference's rules
   (gpair_cg_run, FP32 DFT), the latter bounded by --plain-budget-s.
 * roofline: the forward DFT, timed with HIP events on the context stream, against
   the roofline that binds it.  FP32 reads the full X and Y tables (64 flop per
@@ -24,6 +25,9 @@ timed region.  A step is one CG iteration.
   (err_vs_fp64).
 * batch_cfg4: many independent reconstructions of uneven sizes, sequential
   against gpair_batch_pcg_run.
+* precision_cfg3: the 512x512 precision variant, FP32 / BF16x3 / FP16 with scaling
+  against the FP64 oracle (visibilities, chi2, gradient, DFT times, and the
+  objective after a bounded run).
 * pixel_moves: single-pixel moves scored per second (gpair_pixel_propose, --moves
   per call) against a full forward DFT + chi2 per move, with the kernels' own
   rooflines and the FP64 port on the host cores.
@@ -70,6 +74,8 @@ def parse():
                          "(the iteration count to chi2_r = 1 varies with the trajectory)")
     ap.add_argument("--batch-iters", type=int, default=100)
     ap.add_argument("--moves", type=int, default=4096, help="single-pixel moves scored per call (0: skip)")
+    ap.add_argument("--no-cfg3", action="store_true", help="skip the 512x512 precision workload")
+    ap.add_argument("--cfg3-iters", type=int, default=300, help="accelerated iterations per precision at 512x512")
     ap.add_argument("--move-calls", type=int, default=20, help="timed gpair_pixel_propose calls")
     ap.add_argument("--cpu-kind", default="auto", choices=["auto", "reference", "port"])
     ap.add_argument("--calib-copy", action="store_true",
@@ -160,6 +166,74 @@ def fp64_errors(prob, alpha, finals):
     for name, (img, J) in finals.items():
         J64 = orc.objective(P, img.astype(np.float64), alpha)[0]
         out[name] = {"J_reported": J, "J_fp64": J64, "rel_err": abs(J - J64) / abs(J64)}
+    return out
+
+
+def precision_workload(a):
+    """cfg3, the precision variant: 512x512 px (147,456 V2 + 344,064 T3, ring and disk
+    sky).  One chi2 (forward DFT) and one data gradient (adjoint DFT) of a seeded image
+    in FP32, BF16x3 and FP16 with scaling, against the FP64 oracle: max |dV| / max |V|,
+    relative chi2, max |dg| / max |g|; the DFTs' mean times over 10 evaluations (HIP
+    events); then cfg3_iters accelerated iterations from the prior in each precision:
+    the tracked objective (FP64 sums of the committed changes, drifting by the DFTs'
+    errors between exact evaluations) and an exact evaluation of the final image in the
+    working precision, both against the FP64 objective of that image (checker only)."""
+    from gpair_amd import synth
+    from gpair_amd.api import GpairContext
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    prob = synth.make_problem("cfg3", seed=a.seed)
+    W, alpha = prob.W, prob.meta["alpha"]
+    orc = O.Oracle()
+    P = orc.unpack(prob)
+    rng = np.random.default_rng(7)
+    img = (prob.start.astype(np.float64) * (1.0 + rng.random(W * W))).astype(np.float32)
+    t0 = time.perf_counter()
+    J_o, chi2_o, S_o, vis_o, mock_o, F = orc.objective(P, img.astype(np.float64), alpha)
+    gd_o = orc.gradient(P, img.astype(np.float64), alpha, vis_o, mock_o)[1]
+    t_orc = time.perf_counter() - t0
+    vmax, gmax = np.abs(vis_o).max(), np.abs(gd_o).max()
+    out = {"workload": f"cfg3: {W}x{W} px, {prob.npow} V2 + {prob.nbis} T3, ring and disk sky, seed {a.seed}",
+           "oracle_s": t_orc, "oracle_threads": orc.threads()}
+    for name, p in (("fp32", 0), ("bf16x3", 1), ("fp16s", 2)):
+        ctx = GpairContext.from_problem(prob, image=img)
+        ctx.set_precision(p)
+        chi2 = ctx.gpu_get_chi2_curr()
+        vis = ctx.vis().astype(np.complex128)
+        ctx.gpu_compute_data_gradient_curr()
+        gd = ctx.get("dg").astype(np.float64)
+        L = ctx.L
+        L.gpair_timing_enable(ctx.h, 1)
+        for _ in range(10):
+            ctx.gpu_get_chi2_curr()
+            ctx.gpu_compute_data_gradient_curr()
+        tk = timing(L, ctx.h)
+        L.gpair_timing_enable(ctx.h, 0)
+        ctx.close()
+        c2 = GpairContext.from_problem(prob)
+        c2.set_precision(p)
+        t1 = time.perf_counter()
+        r2, _ = c2.pcg_run(alpha, precond=1, max_iter=a.cfg3_iters, chi2r_target=0.0)
+        c2.L.gpair_sync(c2.h)
+        wall = time.perf_counter() - t1
+        fin = c2.get("ci").astype(np.float64)
+        J_ex = c2.gpu_get_chi2_curr() - alpha * c2.gpu_get_entropy_curr()   # exact, working precision
+        c2.close()
+        J64 = orc.objective(P, fin, alpha)[0]
+        fn, fms = tk["forward"]
+        an, ams = tk["adjoint"]
+        out[name] = {"vis_err": float(np.abs(vis - vis_o).max() / vmax), "chi2_rel_err": abs(chi2 - chi2_o) / chi2_o,
+                     "grad_err": float(np.abs(gd - gd_o).max() / gmax),
+                     "forward_us": fms / max(fn, 1) * 1e3, "adjoint_us": ams / max(an, 1) * 1e3,
+                     "run": {"iterations": r2.iterations, "wall_s": wall, "iters_per_s": r2.iterations / wall,
+                             "chi2r": r2.chi2r, "J_tracked": r2.J, "J_exact": J_ex, "J_fp64": J64,
+                             "J_tracked_rel_err": abs(r2.J - J64) / abs(J64),
+                             "J_exact_rel_err": abs(J_ex - J64) / abs(J64)}}
+        print(f"# cfg3 {name}: vis {out[name]['vis_err']:.2e} chi2 {out[name]['chi2_rel_err']:.2e} "
+              f"grad {out[name]['grad_err']:.2e} fwd {out[name]['forward_us']:.1f} us adj "
+              f"{out[name]['adjoint_us']:.1f} us; {r2.iterations} its J err tracked "
+              f"{out[name]['run']['J_tracked_rel_err']:.2e} exact {out[name]['run']['J_exact_rel_err']:.2e}",
+              file=sys.stderr, flush=True)
     return out
 
 
@@ -398,7 +472,7 @@ def main():
     finals = {}
     if not a.no_target:
         tt = {}
-        for name, p in (("accelerated_fp32", 0), ("accelerated_bf16x3", 1)):
+        for name, p in (("accelerated_fp32", 0), ("accelerated_bf16x3", 1), ("accelerated_fp16s", 2)):
             c2 = GpairContext.from_problem(prob)
             c2.set_precision(p)
             tune(c2)
@@ -492,6 +566,8 @@ def main():
         out["pixel_moves"] = pixel_workload(a, prob, prec, cpu=not a.no_cpu)
     if a.batch > 0:
         out["batch_cfg4"] = batch_workload(a)
+    if not a.no_cfg3:
+        out["precision_cfg3"] = precision_workload(a)
     if not a.no_cpu:
         counts = plain_counts if plain_counts else (10.0, 2.0)
         out["cpu_baseline"] = cpu_baseline(prob, counts[0], counts[1], a.cpu_kind)

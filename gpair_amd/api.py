@@ -15,6 +15,7 @@ from . import _lib
 
 PREC_FP32 = 0
 PREC_BF16X3 = 1
+PREC_FP16S = 2
 
 
 class GpairError(RuntimeError):

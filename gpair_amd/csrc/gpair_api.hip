@@ -539,7 +539,7 @@ extern "C" int gpair_set_precision(gpair_ctx *c, int precision)
 {
     if (!c)
         return GPAIR_E_ARG;
-    if (precision != GPAIR_PREC_FP32 && precision != GPAIR_PREC_BF16X3)
+    if (precision != GPAIR_PREC_FP32 && precision != GPAIR_PREC_BF16X3 && precision != GPAIR_PREC_FP16S)
         return fail(c, GPAIR_E_ARG, "precision %d not available", precision);
     c->precision = precision;
     return GPAIR_OK;
@@ -586,6 +586,15 @@ static int ensure_split_tables(gpair_ctx *c)
     }
     c->split_valid = 1;
     return GPAIR_OK;
+}
+
+// FP16 with scaling: needs the factored tables; the FP16 image shares the BF16x3 hi
+// buffer, the split forward blocks and the quadrant adjoint's uv split are the same
+static int ensure_f16(gpair_ctx *c)
+{
+    if (!c->have_ft)
+        return fail(c, GPAIR_E_STATE, "FP16 DFTs need tables set from uv (gpair_set_uv), not uploaded ones");
+    return ensure_split_tables(c);
 }
 
 // ---------------------------------------------------------------------------
@@ -704,6 +713,39 @@ static int launch_flux(gpair_ctx *c, const float *img)
 // presplit: the BF16 split of img is already in Ih / Il (written by its producer)
 static int launch_forward_to(gpair_ctx *c, const float *img, const float *scale, float2 *vis_out, int presplit = 0)
 {
+    if (c->precision == GPAIR_PREC_FP16S) {
+        int rc;
+        if ((rc = ensure_f16(c)))
+            return rc;
+        // scale: max |img| (deterministic max reduction), then the FP16 image
+        k_absmax<<<red_blocks(c->n), RED_THREADS, 0, c->stream>>>(img, c->n, c->part, c->ticket, c->scal + SC_AMAXI);
+        k_split_image_h<<<ew_blocks(c->n), 256, 0, c->stream>>>(img, c->n, c->scal + SC_AMAXI, c->Ih);
+        LAUNCHCHK(c);
+        const int nblk = (c->nuv + B3_PW - 1) / B3_PW, nchunk = c->W / B3_KC;
+        const int nwg = nblk < B3_WG ? nblk : B3_WG;
+        const int upw = (nblk * nchunk + nwg - 1) / nwg;
+        const int nbk = c->fwd_nbk ? c->fwd_nbk : (c->W < B3_ROWS ? c->W : B3_ROWS) / 32;
+        if (c->W % (32 * nbk) != 0)
+            return fail(c, GPAIR_E_ARG, "FP16 forward: width %d is not a multiple of %d rows", c->W, 32 * nbk);
+        switch (nbk) {
+#define H1_CASE(N)                                                                                            \
+    case N:                                                                                                   \
+        TIMED_LAUNCH(c, GPAIR_TK_FORWARD,                                                                     \
+                     (k_vis_fwd_h1f<N><<<nwg, 256, 0, c->stream>>>(c->Ih, c->FX, c->FY, c->nuv, c->W, upw, scale, \
+                                                                   c->scal + SC_AMAXI, vis_out, c->fwdp),       \
+                      k_vis_fwd_b3_combine<<<nwg > 1 ? nwg - 1 : 1, B3_PW, 0, c->stream>>>(c->fwdp, c->nuv, c->W, \
+                                                                                         upw, scale, vis_out))); \
+        break;
+            H1_CASE(2)
+            H1_CASE(4)
+            H1_CASE(6)
+            H1_CASE(8)
+#undef H1_CASE
+        default: return fail(c, GPAIR_E_ARG, "unsupported width %d", c->W);
+        }
+        LAUNCHCHK(c);
+        return GPAIR_OK;
+    }
     if (c->precision == GPAIR_PREC_BF16X3) {
         int rc;
         if ((rc = ensure_split_tables(c)))
@@ -807,7 +849,19 @@ static int launch_adjoint_dft(gpair_ctx *c, int *nsplit_out)
 {
     int rc;
     int nsplit = c->adj_nsplit;
-    if (c->precision == GPAIR_PREC_BF16X3 && c->W % AB3_XT == 0) {
+    if (c->precision == GPAIR_PREC_FP16S && adj_factored(c)) {
+        if ((rc = ensure_f16(c)))
+            return rc;
+        nsplit = c->adj_b3_nsplit;
+        k_absmax<<<red_blocks(2 * c->nuv), RED_THREADS, 0, c->stream>>>(reinterpret_cast<const float *>(c->coef),
+                                                                         2 * c->nuv, c->part, c->ticket,
+                                                                         c->scal + SC_AMAXC);
+        LAUNCHCHK(c);
+        dim3 grid((c->W / AW_T) * (c->W / AW_T), nsplit);
+        TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,
+                     (k_adj_h1wf<<<grid, 512, 0, c->stream>>>(c->coef, c->FX, c->FY, c->nuv, c->W, c->adj_b3_pper,
+                                                              c->scal + SC_AMAXC, c->adj_part)));
+    } else if (c->precision == GPAIR_PREC_BF16X3 && c->W % AB3_XT == 0) {
         if ((rc = ensure_split_tables(c)))
             return rc;
         nsplit = c->adj_b3_nsplit;

@@ -31,7 +31,9 @@ enum {
     SC_PXC = 15,     // chi2 of the pixel-move state
     SC_STEP = 16,    // step chosen on the device from a line-search batch (0: none)
     SC_KBEST = 17,   // its candidate index (-1: no decrease, or not a descent direction)
-    GPAIR_NSCAL = 18
+    SC_AMAXI = 18,   // max |x| of the image of the last FP16 forward DFT
+    SC_AMAXC = 19,   // max |Re|, |Im| of the adjoint coefficients of the last FP16 adjoint
+    GPAIR_NSCAL = 20
 };
 
 #define RED_THREADS 256
@@ -103,6 +105,27 @@ __device__ inline void split2_bf16(float x, float y, unsigned &h, unsigned &l)
     h = __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
     const f32x2 r = {x - __uint_as_float(h << 16), y - __uint_as_float(h & 0xffff0000u)};
     l = __builtin_bit_cast(unsigned, __builtin_convertvector(r, bf16x2));
+}
+
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+// two floats as a packed FP16 pair (v_cvt_pk_f16_f32, round to nearest even), x low
+__device__ inline unsigned pack_f16(float x, float y)
+{
+    const f32x2 v = {x, y};
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(v, f16x2));
+}
+
+// FP16 with scaling: a power of two that maps the largest magnitude m into
+// [2^13, 2^14), leaving headroom below FP16's 65504; 1 when m is 0 or not finite
+__device__ inline float f16_scale(double m)
+{
+    if (!(m > 0.0) || !(m < 1e300))
+        return 1.0f;
+    int e;
+    (void)frexp(m, &e);   // m = f 2^e, f in [0.5, 1)
+    return ldexpf(1.0f, 14 - e);
 }
 
 // ---------------------------------------------------------------------------

@@ -1815,3 +1815,365 @@ __global__ __launch_bounds__(512) void k_adj_b3wf(const float2 *__restrict__ coe
                 out[(size_t)y * W + x0 + wx + 32 * n + lr] = acc[m][n][r];
             }
 }
+
+// ---------------------------------------------------------------------------
+// FP16 with scaling: one FP16 MFMA product per multiply-add
+// ---------------------------------------------------------------------------
+// Table entries (|X|, |Y| <= 1) go to FP16 as they are; the image and the adjoint
+// coefficients are scaled by the power of two f16_scale(max |x|) first (FP16 tops
+// out at 65504: a direction or a coefficient vector can exceed it), and the result
+// is scaled back exactly.  A product carries the FP16 rounding of both operands,
+// about 2^-11 each: about 500 times the BF16x3 error, at a third of its MFMAs.
+
+// max |x_i| over n floats into *out (FP64 slot); order-free, so deterministic
+__global__ __launch_bounds__(RED_THREADS) void k_absmax(const float *__restrict__ x, int n, double *part,
+                                                        unsigned *ticket, double *out)
+{
+    __shared__ float sh[RED_THREADS / 64];
+    __shared__ int is_last;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    float m = 0.f;
+    for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS)
+        m = fmaxf(m, fabsf(x[i]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        m = fmaxf(m, __shfl_xor(m, o, 64));
+    if (lane == 0)
+        sh[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float b = sh[0];
+        for (int w = 1; w < RED_THREADS / 64; w++)
+            b = fmaxf(b, sh[w]);
+        __hip_atomic_store(&part[blockIdx.x], (double)b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = t == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!is_last)
+        return;
+    float mm = 0.f;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += RED_THREADS)
+        mm = fmaxf(mm, (float)__hip_atomic_load(&part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        mm = fmaxf(mm, __shfl_xor(mm, o, 64));
+    __syncthreads();
+    if (lane == 0)
+        sh[wid] = mm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float b = sh[0];
+        for (int w = 1; w < RED_THREADS / 64; w++)
+            b = fmaxf(b, sh[w]);
+        *out = (double)b;
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// image float [W*W] -> FP16 bits of img * f16_scale(*amax), same [y][x] layout
+__global__ void k_split_image_h(const float *__restrict__ img, int n, const double *__restrict__ amax,
+                                unsigned short *__restrict__ Ih)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float sc = f16_scale(*amax);
+    Ih[i] = (unsigned short)(pack_f16(img[i] * sc, 0.f) & 0xffffu);
+}
+
+// 8 consecutive entries of one component of a table row, as FP16 (see ft_gen8)
+__device__ __forceinline__ void ft_gen8_h(float cr, float ci, const float2 (&g)[8], uint4 &h)
+{
+    float v[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++)
+        v[e] = cr * g[e].x - ci * g[e].y;
+    h.x = pack_f16(v[0], v[1]);
+    h.y = pack_f16(v[2], v[3]);
+    h.z = pack_f16(v[4], v[5]);
+    h.w = pack_f16(v[6], v[7]);
+}
+
+// Forward DFT in FP16: the tiling, stream-K units and summation order of
+// k_vis_fwd_b3f, with one operand plane in LDS and one MFMA per k-step and block
+template <int NBK>
+__global__ __launch_bounds__(256) void k_vis_fwd_h1f(const unsigned short *__restrict__ Ih,
+                                                        const float2 *__restrict__ FX,
+                                                        const float2 *__restrict__ FY, int nuv, int W, int upw,
+                                                        const float *__restrict__ scale,
+                                                        const double *__restrict__ amax, float2 *__restrict__ vis,
+                                                        float2 *__restrict__ vpart)
+{
+    constexpr int ROWS = NBK * 32;
+    constexpr int CH = ROWS * (B3_KC / 8);
+    constexpr int PER = (CH + 255) / 256;
+    static_assert(PER <= 4, "staging holds up to 4 pieces per thread");
+    static_assert(B3_KC == 32, "two k-steps per chunk");
+    __shared__ __attribute__((aligned(16))) unsigned short sB[2][ROWS * B3_STRIDE];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 31, lh = lane >> 5;
+    const int ft = ft_len(W), nco = W / 16;
+    const int nchunk = W / B3_KC;
+    const int nblk = (nuv + B3_PW - 1) / B3_PW;
+    const int total = nblk * nchunk;
+    const int u_end = min(total, (int)(blockIdx.x + 1) * upw);
+    const float unscale = 1.0f / f16_scale(*amax);   // a power of two: exact
+    for (int u = (int)blockIdx.x * upw; u < u_end;) {
+        const int blk = u / nchunk, c0 = u % nchunk;
+        const int c1 = min(nchunk, c0 + (u_end - u));
+        u += c1 - c0;
+        const int p0 = blk * B3_PW + wave * 16;
+        const int pa = p0 + (lr & 15);
+        const float2 *fx = FX + (size_t)(pa < nuv ? pa : 0) * ft;
+        float2 g[8];
+        {
+            const float4 *fp = reinterpret_cast<const float4 *>(fx + nco + 8 * lh);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const float4 f = fp[q];
+                g[2 * q] = make_float2(f.x, f.y);
+                g[2 * q + 1] = make_float2(f.z, f.w);
+            }
+            if (lr >= 16) {
+#pragma unroll
+                for (int e = 0; e < 8; e++)
+                    g[e] = make_float2(g[e].y, -g[e].x);
+            }
+        }
+        float vr[8], vi[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            vr[r] = 0.f;
+            vi[r] = 0.f;
+        }
+        for (int y0 = 0; y0 < W; y0 += ROWS) {
+            v16f acc[NBK];
+#pragma unroll
+            for (int b = 0; b < NBK; b++)
+#pragma unroll
+                for (int r = 0; r < 16; r++)
+                    acc[b][r] = 0.f;
+            uint4 h0 = make_uint4(0, 0, 0, 0), h1 = h0, h2 = h0, h3 = h0;
+            auto fetch = [&](int c) {   // one plane of b3_fetch
+                const int cc = c;
+                auto one = [&](uint4 &h, int q) {
+                    const int e = tid + 256 * q;
+                    if (CH % 256 == 0 || e < CH)
+                        h = *reinterpret_cast<const uint4 *>(Ih + (size_t)(y0 + e / (B3_KC / 8)) * W + cc * B3_KC +
+                                                             8 * (e % (B3_KC / 8)));
+                };
+                if constexpr (PER > 0) one(h0, 0);
+                if constexpr (PER > 1) one(h1, 1);
+                if constexpr (PER > 2) one(h2, 2);
+                if constexpr (PER > 3) one(h3, 3);
+            };
+            auto store = [&](int buf) {
+                auto one = [&](const uint4 &h, int q) {
+                    const int e = tid + 256 * q;
+                    if (CH % 256 == 0 || e < CH)
+                        *reinterpret_cast<uint4 *>(&sB[buf][(e / (B3_KC / 8)) * B3_STRIDE + 8 * (e % (B3_KC / 8))]) = h;
+                };
+                if constexpr (PER > 0) one(h0, 0);
+                if constexpr (PER > 1) one(h1, 1);
+                if constexpr (PER > 2) one(h2, 2);
+                if constexpr (PER > 3) one(h3, 3);
+            };
+            fetch(c0);
+            uint4 a0, a1, b0, b1;
+            {
+                const float4 cx = *reinterpret_cast<const float4 *>(fx + 2 * c0);
+                ft_gen8_h(cx.x, cx.y, g, a0);
+                ft_gen8_h(cx.z, cx.w, g, a1);
+            }
+            __syncthreads();
+            store(c0 & 1);
+            __syncthreads();
+            auto chunk = [&](int c, const uint4 &ca0, const uint4 &ca1, uint4 &d0, uint4 &d1) {
+                const int buf = c & 1;
+                const int cn = c + 1 < c1 ? c + 1 : c;
+                const float4 cxn = *reinterpret_cast<const float4 *>(fx + 2 * cn);
+                fetch(cn);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int ks = 0; ks < B3_KC / 16; ks++) {
+                    const f16x8 a = __builtin_bit_cast(f16x8, ks ? ca1 : ca0);
+#pragma unroll
+                    for (int b = 0; b < NBK; b++) {
+                        const int o = (b * 32 + lr) * B3_STRIDE + ks * 16 + 8 * lh;
+                        const f16x8 bv = *reinterpret_cast<const f16x8 *>(&sB[buf][o]);
+                        acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bv, acc[b], 0, 0, 0);
+                    }
+                }
+                ft_gen8_h(cxn.x, cxn.y, g, d0);
+                ft_gen8_h(cxn.z, cxn.w, g, d1);
+                store(buf ^ 1);
+                __syncthreads();
+            };
+            int c = c0;
+            for (; c + 1 < c1; c += 2) {
+                chunk(c, a0, a1, b0, b1);
+                chunk(c + 1, b0, b1, a0, a1);
+            }
+            if (c < c1)
+                chunk(c, a0, a1, b0, b1);
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const int p = p0 + 8 * (r >> 2) + 4 * lh + (r & 3);
+                const float2 *fy = FY + (size_t)(p < nuv ? p : 0) * ft;
+                const float2 fj = fy[nco + (lr & 15)];
+                const float2 *cy = fy + (y0 >> 4) + (lr >> 4);
+#pragma unroll
+                for (int b = 0; b < NBK; b++) {
+                    const float2 yv = cmul(cy[2 * b], fj);
+                    const float tr = acc[b][r], ti = acc[b][r + 8];
+                    vr[r] += tr * yv.x - ti * yv.y;
+                    vi[r] += tr * yv.y + ti * yv.x;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+#pragma unroll
+            for (int o = 1; o < 32; o <<= 1) {
+                vr[r] += __shfl_xor(vr[r], o, 64);
+                vi[r] += __shfl_xor(vi[r], o, 64);
+            }
+        }
+        if (lr == 0) {
+            const bool whole = c0 == 0 && c1 == nchunk;
+            const float f = (whole && scale ? *scale : 1.0f) * unscale;
+            float2 *dst = whole ? vis : vpart + (c0 == 0 ? 0 : nuv);
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const int p = p0 + 8 * (r >> 2) + 4 * lh + (r & 3);
+                if (p < nuv)
+                    dst[p] = make_float2(vr[r] * f, vi[r] * f);
+            }
+        }
+    }
+}
+
+template __global__ void k_vis_fwd_h1f<2>(const unsigned short *, const float2 *, const float2 *, int, int, int,
+                                          const float *, const double *, float2 *, float2 *);
+template __global__ void k_vis_fwd_h1f<4>(const unsigned short *, const float2 *, const float2 *, int, int, int,
+                                          const float *, const double *, float2 *, float2 *);
+template __global__ void k_vis_fwd_h1f<6>(const unsigned short *, const float2 *, const float2 *, int, int, int,
+                                          const float *, const double *, float2 *, float2 *);
+template __global__ void k_vis_fwd_h1f<8>(const unsigned short *, const float2 *, const float2 *, int, int, int,
+                                          const float *, const double *, float2 *, float2 *);
+
+// Adjoint in FP16: k_adj_b3wf's quadrants, rounds and builder roles with one plane
+// per tile and one MFMA per k-step and tile; the coefficients scaled by
+// f16_scale(*amax) (max over their Re and Im parts) and the partials scaled back
+__global__ __launch_bounds__(512) void k_adj_h1wf(const float2 *__restrict__ coef, const float2 *__restrict__ FX,
+                                                  const float2 *__restrict__ FY, int nuv, int W, int p_per_split,
+                                                  const double *__restrict__ amax, float *__restrict__ part)
+{
+    __shared__ __attribute__((aligned(16))) unsigned short sA[2][AW_T * AW_STRIDE];
+    __shared__ __attribute__((aligned(16))) unsigned short sB[2][AW_T * AW_STRIDE];
+    constexpr int WROW = AW_STRIDE / 2;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 31, lh = lane >> 5;
+    const int ft = ft_len(W), nco = W / 16;
+    const int qx = W / AW_T;
+    const int y0 = (blockIdx.x / qx) * AW_T, x0 = (blockIdx.x % qx) * AW_T;
+    const int pstart = blockIdx.y * p_per_split;
+    const int pend = min(pstart + p_per_split, nuv);
+    const int q = tid & 15, o4 = 4 * (tid >> 4);
+    const float sc = f16_scale(*amax), unscale = 1.0f / sc;
+    float2 ca, cy0, cy1, cx0, cx1;
+    float4 fy01, fy23, fx01, fx23;
+    auto load = [&](int pr) {
+        const int p = pr + q;
+        const int pv = min(p, pend - 1);
+        const float2 cv = coef[pv];
+        const float keep = p < pend ? sc : 0.f;
+        ca = make_float2(cv.x * keep, cv.y * keep);
+        const float2 *fy = FY + (size_t)pv * ft, *fx = FX + (size_t)pv * ft;
+        const float4 *fyf = reinterpret_cast<const float4 *>(fy + nco + (o4 & 15));
+        const float4 *fxf = reinterpret_cast<const float4 *>(fx + nco + (o4 & 15));
+        fy01 = fyf[0];
+        fy23 = fyf[1];
+        fx01 = fxf[0];
+        fx23 = fxf[1];
+        cy0 = fy[(y0 + o4) >> 4];
+        cy1 = fy[(y0 + o4 + 128) >> 4];
+        cx0 = fx[(x0 + o4) >> 4];
+        cx1 = fx[(x0 + o4 + 128) >> 4];
+    };
+    auto store = [&](int buf) {
+        unsigned *aw = reinterpret_cast<unsigned *>(sA[buf]), *bw = reinterpret_cast<unsigned *>(sB[buf]);
+        const float2 fyv[4] = {make_float2(fy01.x, fy01.y), make_float2(fy01.z, fy01.w), make_float2(fy23.x, fy23.y),
+                               make_float2(fy23.z, fy23.w)};
+        const float2 fxv[4] = {make_float2(fx01.x, fx01.y), make_float2(fx01.z, fx01.w), make_float2(fx23.x, fx23.y),
+                               make_float2(fx23.z, fx23.w)};
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const float2 acy = cmul(ca, h ? cy1 : cy0);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float2 w = cmul(acy, fyv[j]);
+                aw[(o4 + 128 * h + j) * WROW + q] = pack_f16(w.x, -w.y);
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const float2 cxh = h ? cx1 : cx0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float2 w = cmul(cxh, fxv[j]);
+                bw[(o4 + 128 * h + j) * WROW + q] = pack_f16(w.x, w.y);
+            }
+        }
+    };
+    const int wy = (wave >> 1) * 64, wx = (wave & 1) * 128;
+    v16f acc[2][4];
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+                acc[m][n][r] = 0.f;
+    int buf = 0;
+    if (pstart < pend) {
+        load(pstart);
+        store(0);
+    }
+    __syncthreads();
+    for (int pr = pstart; pr < pend; pr += AB3_PR) {
+        const bool more = pr + AB3_PR < pend;
+        if (more)
+            load(pr + AB3_PR);
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            f16x8 a[2], b[4];
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+                a[m] = *reinterpret_cast<const f16x8 *>(&sA[buf][(wy + 32 * m + lr) * AW_STRIDE + 16 * ks + 8 * lh]);
+#pragma unroll
+            for (int n = 0; n < 4; n++)
+                b[n] = *reinterpret_cast<const f16x8 *>(&sB[buf][(wx + 32 * n + lr) * AW_STRIDE + 16 * ks + 8 * lh]);
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+                for (int n = 0; n < 4; n++)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[m], b[n], acc[m][n], 0, 0, 0);
+        }
+        if (more)
+            store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    float *out = part + (size_t)blockIdx.y * W * W;
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int y = y0 + wy + 32 * m + 8 * (r >> 2) + 4 * lh + (r & 3);
+                out[(size_t)y * W + x0 + wx + 32 * n + lr] = acc[m][n][r] * unscale;
+            }
+}

@@ -41,7 +41,9 @@ enum {
 /* precision of the forward / adjoint DFT contraction */
 enum {
     GPAIR_PREC_FP32 = 0,   /* FP32 MFMA, FP32 accumulation */
-    GPAIR_PREC_BF16X3 = 1  /* split BF16 hi/lo, 3 MFMA products, FP32 accumulation */
+    GPAIR_PREC_BF16X3 = 1, /* split BF16 hi/lo, 3 MFMA products, FP32 accumulation */
+    GPAIR_PREC_FP16S = 2   /* FP16 with power-of-two scaling, 1 MFMA product, FP32 accumulation;
+                              needs DFT tables set from uv (gpair_set_uv) */
 };
 
 const char *gpair_strerror(int code);

@@ -362,3 +362,62 @@ def test_bf16x3_factored_tables(name, kw, orc):
     (c1, v1, g1), (c2, v2, g2) = out
     assert np.abs(v1 - v2).max() <= 3e-5 * vmax
     assert np.abs(g1 - g2).max() <= 2e-4 * gmax
+
+
+@pytest.mark.parametrize("name,kw", _ft_cases(), ids=[c[0] for c in _ft_cases()])
+def test_fp16_scaled(name, kw, orc):
+    """FP16 with power-of-two scaling (one FP16 MFMA product per multiply-add) against
+    the FP64 oracle.  Each product carries two FP16 roundings (2^-11 each) that
+    average over the W^2 terms of a visibility: bounds |dV| <= 5e-4 max|V|, chi2
+    relative 5e-3, gradient 5e-3 of max|g| (W = 64 and 192 take the FP32 adjoint)."""
+    prob = synth.make_problem("cfg1", **kw)
+    print(f"case {name}: W={prob.W} nuv={prob.nuv} nbis={prob.nbis}", flush=True)
+    P = orc.unpack(prob)
+    img = _image(prob, 5)
+    with GpairContext.from_problem(prob, image=img) as ctx:
+        ctx.set_precision(2)
+        chi2 = ctx.gpu_get_chi2_curr()
+        vis = ctx.vis().astype(np.complex128)
+        ctx.gpu_compute_data_gradient_curr()
+        gd = ctx.get("dg").astype(np.float64)
+    J, chi2_o, S_o, vis_o, mock_o, F = orc.objective(P, img.astype(np.float64), ALPHA)
+    gd_o = orc.gradient(P, img.astype(np.float64), ALPHA, vis_o, mock_o)[1]
+    ev = np.abs(vis - vis_o).max() / np.abs(vis_o).max()
+    ec = abs(chi2 - chi2_o) / chi2_o
+    eg = np.abs(gd - gd_o).max() / np.abs(gd_o).max()
+    print(f"  fp16 scaled {name}: vis {ev:.2e} chi2 {ec:.2e} grad {eg:.2e}", flush=True)
+    assert ev <= 5e-4 and ec <= 5e-3 and eg <= 5e-3
+
+
+def test_fp16_scaled_needs_factored_tables(orc):
+    """uploaded tables cannot be factored: the FP16 DFTs refuse them with an error"""
+    prob = synth.make_problem("cfg1", seed=6)
+    X, Y = orc.tables(prob.W, prob.au(), prob.av())
+    with GpairContext(0) as b:
+        b.gpu_copy_data(prob.data, prob.err, prob.nuv, prob.phasor, prob.npow, prob.uvpnt4(), prob.sign4(),
+                        prob.model, prob.W)
+        b.gpu_copy_dft(X.T.astype(np.complex64), Y.T.astype(np.complex64))
+        b.gpu_build_kernels()
+        b.gpu_copy_image(_image(prob, 1))
+        b.set_precision(2)
+        with pytest.raises(GpairError):
+            b.gpu_get_chi2_curr()
+
+
+def test_fp16_scaled_loop_reaches_target(orc):
+    """the accelerated loop with FP16 DFTs at cfg1 to chi2/ndof <= 1: the FP16 DFTs
+    steer the search, convergence is accepted on an exact (BF16x3) evaluation, so
+    the reported objective agrees with the FP64 objective of the final image"""
+    prob = synth.make_problem("cfg1", seed=1)
+    P = orc.unpack(prob)
+    alpha = prob.meta["alpha"]
+    with GpairContext.from_problem(prob) as ctx:
+        ctx.set_precision(2)
+        res, tr = ctx.pcg_run(alpha, precond=1, max_iter=3000, chi2r_target=1.0)
+        img = ctx.get("ci").astype(np.float64)
+    J_o, chi2_o, *_ = orc.objective(P, img, alpha)
+    rel = abs(res.J - J_o) / abs(J_o)
+    print("iterations", res.iterations, "chi2r", res.chi2r, "J rel err vs FP64", rel)
+    assert res.converged
+    assert rel <= 2e-5
+    assert chi2_o / prob.ndof <= 1.0 + 1e-5
