@@ -554,6 +554,8 @@ static int ensure_split_tables(gpair_ctx *c)
 {
     if (c->split_valid)
         return GPAIR_OK;
+    if (c->have_ft && ft_len(c->W) > FYS_MAX)   // the factored forwards stage a block's FY rows in LDS
+        return fail(c, GPAIR_E_ARG, "width %d above the factored DFTs' maximum of 512", c->W);
     const size_t tsz = (size_t)c->nuv * c->W;
     int rc;
     if ((rc = dalloc(c, &c->Ih, (size_t)c->n)) || (rc = dalloc(c, &c->Il, (size_t)c->n)) ||

@@ -1480,6 +1480,41 @@ struct B3Stage {
     uint4 h0, h1, h2, h3, l0, l1, l2, l3;
 };
 
+// The y-reduction of a 64-point block reads, per lane and point, the fine factor of
+// its column and the coarse factors of its 8 column blocks: staged once per block in
+// LDS (the block's rows of FY are contiguous), so the reduction reads LDS instead of
+// waiting on 72 dependent L2 loads per lane.  FYS_MAX: rows of the widest image (512).
+#define FYS_MAX (512 / 16 + FT_FINE)
+#define FYS_PER ((B3_PW * FYS_MAX + 255) / 256)
+
+struct FyStage {
+    float2 v[FYS_PER];
+};
+
+__device__ __forceinline__ void fy_fetch(FyStage &S, const float2 *__restrict__ FY, int blk, int nuv, int ft, int tid)
+{
+    const int nval = B3_PW * ft;
+    const size_t base = (size_t)blk * B3_PW * ft, lim = (size_t)nuv * ft;
+#pragma unroll
+    for (int q = 0; q < FYS_PER; q++) {
+        const int i = tid + 256 * q;
+        const size_t gi = base + i;
+        if (i < nval)
+            S.v[q] = FY[gi < lim ? gi : 0];   // points past nuv: any valid row (never stored)
+    }
+}
+
+__device__ __forceinline__ void fy_store(const FyStage &S, float2 *sY, int ft, int tid)
+{
+    const int nval = B3_PW * ft;
+#pragma unroll
+    for (int q = 0; q < FYS_PER; q++) {
+        const int i = tid + 256 * q;
+        if (i < nval)
+            sY[i] = S.v[q];
+    }
+}
+
 template <int NBK>
 __global__ __launch_bounds__(256) void k_vis_fwd_b3f(const unsigned short *__restrict__ Ih,
                                                         const unsigned short *__restrict__ Il,
@@ -1494,6 +1529,7 @@ __global__ __launch_bounds__(256) void k_vis_fwd_b3f(const unsigned short *__res
     static_assert(PER <= 4, "staging holds up to 4 pieces per thread");
     static_assert(B3_KC == 32, "two k-steps per chunk");
     __shared__ __attribute__((aligned(16))) unsigned short sB[2][2][ROWS * B3_STRIDE];
+    __shared__ float2 sY[B3_PW * FYS_MAX];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lr = lane & 31, lh = lane >> 5;
     const int ft = ft_len(W), nco = W / 16;
@@ -1549,6 +1585,8 @@ __global__ __launch_bounds__(256) void k_vis_fwd_b3f(const unsigned short *__res
             B3Stage P, Q;
             P.h0 = P.h1 = P.h2 = P.h3 = P.l0 = P.l1 = P.l2 = P.l3 = make_uint4(0, 0, 0, 0);
             Q = P;
+            FyStage FS;
+            fy_fetch(FS, FY, blk, nuv, ft, tid);
             fetch(P, cidx(0));
             fetch(Q, cidx(1));
             float4 cxa = *reinterpret_cast<const float4 *>(fx + 2 * cidx(0));
@@ -1556,8 +1594,9 @@ __global__ __launch_bounds__(256) void k_vis_fwd_b3f(const unsigned short *__res
             uint4 a0h, a0l, a1h, a1l, b0h, b0l, b1h, b1l;
             ft_gen8(cxa.x, cxa.y, g, a0h, a0l);
             ft_gen8(cxa.z, cxa.w, g, a1h, a1l);
-            __syncthreads();   // the previous readers are done with sB
+            __syncthreads();   // the previous readers are done with sB and sY
             stage(P, 0);
+            fy_store(FS, sY, ft, tid);
             __syncthreads();
             // position i: MFMAs on LDS buffer i % 2 with A set `cur`; the A set of i + 1
             // from its coarse factors `cxn`; chunk i + 1's rows (`Sn`) into the other
@@ -1603,13 +1642,11 @@ __global__ __launch_bounds__(256) void k_vis_fwd_b3f(const unsigned short *__res
             if (i < n)
                 step(i, 0, a0h, a0l, a1h, a1l, b0h, b0l, b1h, b1l, Q, P, cxb, cxa);
             // y-reduction: point 8(r/4) + 4 lh + r%4 has Re T in register r and Im T in
-            // r+8 at column y = y0 + 32 b + lr = 16 (y0/16 + 2 b + lr/16) + lr % 16.
-            // No branch on p < nuv (points past nuv read point 0's factors and are never
-            // stored), so all factor loads of the group can be in flight together.
+            // r+8 at column y = y0 + 32 b + lr = 16 (y0/16 + 2 b + lr/16) + lr % 16;
+            // its factors from the block's rows staged in sY
 #pragma unroll
             for (int r = 0; r < 8; r++) {
-                const int p = p0 + 8 * (r >> 2) + 4 * lh + (r & 3);
-                const float2 *fy = FY + (size_t)(p < nuv ? p : 0) * ft;
+                const float2 *fy = sY + (wave * 16 + 8 * (r >> 2) + 4 * lh + (r & 3)) * ft;
                 const float2 fj = fy[nco + (lr & 15)];
                 const float2 *cy = fy + (y0 >> 4) + (lr >> 4);
 #pragma unroll
@@ -1911,6 +1948,7 @@ __global__ __launch_bounds__(256) void k_vis_fwd_h1f(const unsigned short *__res
     static_assert(PER <= 4, "staging holds up to 4 pieces per thread");
     static_assert(B3_KC == 32, "two k-steps per chunk");
     __shared__ __attribute__((aligned(16))) unsigned short sB[2][ROWS * B3_STRIDE];
+    __shared__ float2 sY[B3_PW * FYS_MAX];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lr = lane & 31, lh = lane >> 5;
     const int ft = ft_len(W), nco = W / 16;
@@ -1979,6 +2017,8 @@ __global__ __launch_bounds__(256) void k_vis_fwd_h1f(const unsigned short *__res
                 if constexpr (PER > 2) one(h2, 2);
                 if constexpr (PER > 3) one(h3, 3);
             };
+            FyStage FS;
+            fy_fetch(FS, FY, blk, nuv, ft, tid);
             fetch(c0);
             uint4 a0, a1, b0, b1;
             {
@@ -1986,8 +2026,9 @@ __global__ __launch_bounds__(256) void k_vis_fwd_h1f(const unsigned short *__res
                 ft_gen8_h(cx.x, cx.y, g, a0);
                 ft_gen8_h(cx.z, cx.w, g, a1);
             }
-            __syncthreads();
+            __syncthreads();   // the previous readers are done with sB and sY
             store(c0 & 1);
+            fy_store(FS, sY, ft, tid);
             __syncthreads();
             auto chunk = [&](int c, const uint4 &ca0, const uint4 &ca1, uint4 &d0, uint4 &d1) {
                 const int buf = c & 1;
@@ -2019,8 +2060,7 @@ __global__ __launch_bounds__(256) void k_vis_fwd_h1f(const unsigned short *__res
                 chunk(c, a0, a1, b0, b1);
 #pragma unroll
             for (int r = 0; r < 8; r++) {
-                const int p = p0 + 8 * (r >> 2) + 4 * lh + (r & 3);
-                const float2 *fy = FY + (size_t)(p < nuv ? p : 0) * ft;
+                const float2 *fy = sY + (wave * 16 + 8 * (r >> 2) + 4 * lh + (r & 3)) * ft;
                 const float2 fj = fy[nco + (lr & 15)];
                 const float2 *cy = fy + (y0 >> 4) + (lr >> 4);
 #pragma unroll
