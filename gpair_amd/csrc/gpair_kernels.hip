@@ -997,30 +997,37 @@ __host__ inline int ls_nparts_ent(int n) { return red_nparts(n, LS_SPAN); }
 // and some candidate k > 0 lowers the objective, the best one (first minimum of
 // dJ_k = dchi2_k - alpha dS_k), refined by the parabola through its neighbours;
 // otherwise step 0 and index -1.  The host reads both back with the commit's
-// results.  One thread.
-__global__ void k_ls_choose(const double *__restrict__ ls, double alpha, double s_ref, double *scal)
+// results.  One wave: lane k takes candidate k, a shuffle reduction finds the first
+// minimum, lane 0 evaluates the parabola.
+__global__ __launch_bounds__(64) void k_ls_choose(const double *__restrict__ ls, double alpha, double s_ref,
+                                                  double *scal)
 {
 #pragma clang fp contract(off)
-    if (blockIdx.x != 0 || threadIdx.x != 0)
-        return;
-    double s = 0.0, kb = -1.0;
-    if (scal[SC_W1] < 0.0 && scal[SC_DABS] > 0.0) {
-        double dJ[LS_K];
-        int b = 0;
-        for (int k = 0; k < LS_K; k++) {
-            dJ[k] = ls[k] - alpha * ls[LS_K + k];
-            if (dJ[k] < dJ[b])
-                b = k;
-        }
-        if (b > 0) {
-            kb = b;
-            s = ls_step(s_ref, b);
-            if (b < LS_K - 1)
-                s = parabola_step(ls_step(s_ref, b - 1), s, ls_step(s_ref, b + 1), dJ[b - 1], dJ[b], dJ[b + 1]);
+    const int k = threadIdx.x;
+    const double dj = k < LS_K ? ls[k] - alpha * ls[LS_K + k] : 0.0;
+    double best = k < LS_K && dj == dj ? dj : __builtin_huge_val();   // NaN never the minimum
+    int kb = k;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o, 64);
+        const int ok = __shfl_xor(kb, o, 64);
+        if (ob < best || (ob == best && ok < kb)) {   // first minimum: ties to the smaller index
+            best = ob;
+            kb = ok;
         }
     }
+    const double djm = __shfl(dj, kb > 0 ? kb - 1 : 0, 64), djp = __shfl(dj, kb < LS_K - 1 ? kb + 1 : 0, 64);
+    if (k != 0)
+        return;
+    double s = 0.0, kbo = -1.0;
+    if (scal[SC_W1] < 0.0 && scal[SC_DABS] > 0.0 && kb > 0) {
+        kbo = kb;
+        s = ls_step(s_ref, kb);
+        if (kb < LS_K - 1)
+            s = parabola_step(ls_step(s_ref, kb - 1), s, ls_step(s_ref, kb + 1), djm, best, djp);
+    }
     scal[SC_STEP] = s;
-    scal[SC_KBEST] = kb;
+    scal[SC_KBEST] = kbo;
 }
 
 // trial image I_t = max(I + s d, minval), the change delta = I_t - I, and
