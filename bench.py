@@ -48,7 +48,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "wall-time & CG iters/s to chi2_r=1 at 256², BF16x3/learned vs plain FP32; err vs FP64"
-PEAK = {"fp32": 157.3, "bf16x3": 2500.0 / 3.0}   # TFLOP/s: FP32 MFMA; BF16 MFMA / 3 products
+PEAK = {"fp32": 157.3, "bf16x3": 2500.0 / 3.0, "fp16s": 2500.0}   # TFLOP/s: FP32 MFMA; BF16 MFMA / 3 products; FP16
 HBM_PEAK = 8.0e12                                  # B/s
 PROFILES = os.path.join(ROOT, "profiles")
 
@@ -61,7 +61,7 @@ def parse():
     ap.add_argument("--cfg", default="cfg2")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--alpha", type=float, default=None)
-    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16x3"])
+    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16x3", "fp16s"])
     ap.add_argument("--max-iter", type=int, default=30000)
     ap.add_argument("--plain-budget-s", type=float, default=30.0)
     ap.add_argument("--no-target", action="store_true", help="skip the runs to chi2_r = 1")
@@ -386,7 +386,7 @@ def main():
 
     prob = synth.make_problem(a.cfg, seed=a.seed)
     alpha = a.alpha if a.alpha is not None else prob.meta["alpha"]
-    prec = {"fp32": 0, "bf16x3": 1}[a.precision]
+    prec = {"fp32": 0, "bf16x3": 1, "fp16s": 2}[a.precision]
     W, nuv = prob.W, prob.nuv
     print(f"# problem {a.cfg}: W={W} nuv={nuv} nbis={prob.nbis} ndof={prob.ndof} alpha={alpha} "
           f"prior={prob.meta['prior']}", file=sys.stderr, flush=True)
@@ -420,13 +420,13 @@ def main():
     adj_n, adj_ms = tk["adjoint"]
     avg_fwd = fwd_ms / max(fwd_n, 1) * 1e-3
     achieved = flop / avg_fwd / 1e12
-    nbk = (tuned[3] if tuned and tuned[3] else min(W, 256) // 32) if prec == 1 else 0
-    kname = f"k_vis_fwd_f32<{W // 64}>" if prec == 0 else f"k_vis_fwd_b3f<{nbk}>"
+    nbk = (tuned[3] if tuned and tuned[3] else min(W, 256) // 32) if prec else 0
+    kname = {0: f"k_vis_fwd_f32<{W // 64}>", 1: f"k_vis_fwd_b3f<{nbk}>", 2: f"k_vis_fwd_h1f<{nbk}>"}[prec]
     # compulsory bytes of one forward: the tables (FP32: X and Y, 8 B per (uv, x) entry
     # each; BF16x3: the factored X and Y tables, 8 B per factor, W/16 + 16 factors per
     # uv point each), the image (4 B per pixel: float, or bf16 hi + lo), V out
     table_bytes = 16.0 * nuv * W if prec == 0 else 16.0 * nuv * (W // 16 + 16)
-    nbytes = table_bytes + 4.0 * W * W + 8.0 * nuv
+    nbytes = table_bytes + (2.0 if prec == 2 else 4.0) * W * W + 8.0 * nuv
     intensity = flop / nbytes
     traffic = read_traffic(f"forward_dft_W{W}_{a.precision}_pmc.txt")
     if intensity < PEAK[a.precision] * 1e12 / HBM_PEAK:
@@ -440,7 +440,8 @@ def main():
                      "launches": fwd_n, "flop_per_launch": flop, "bytes_per_launch": nbytes,
                      "flop_per_byte": intensity, "mfma_tflops": achieved,
                      "mfma_frac": achieved / PEAK[a.precision]})
-    aname = "k_adj_f32" if prec == 0 else "k_adj_b3"
+    aname = {0: "k_adj_f32", 1: "k_adj_b3wf", 2: "k_adj_h1wf"}[prec] if W % 256 == 0 else (
+        "k_adj_b3" if prec == 1 else "k_adj_f32")
     out = {
         "metric": METRIC,
         "value": steps_done / dt,
@@ -452,7 +453,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if prec == 0 else "bf16x3 (f32 accumulate)",
+        "dtype": {0: "f32", 1: "bf16x3 (f32 accumulate)", 2: "fp16 scaled (f32 accumulate)"}[prec],
         "data": "synthetic: seeded 9-telescope array, sky rendered on the grid, noise per BASELINE configs",
         "config": {"workload": f"{a.cfg}: MEM-CG reconstruction {W}x{W} px, {prob.npow} V2 + {prob.nbis} T3, "
                                f"accelerated loop (preconditioned PR-CG, batched line search), {a.precision} DFT",
