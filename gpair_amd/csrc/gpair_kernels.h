@@ -270,6 +270,43 @@ __device__ inline void partitioned_reduce_at(int n, int span_min, F item, double
     }
 }
 
+// partitioned_partials: the partition sums of partitioned_reduce_at alone, stored
+// to part[q*NV + j] with plain stores, no ticket and no finish: a later launch sums
+// them in a fixed order.  On MI355X a per-block same-address ticket (an agent-scope
+// RMW plus release / acquire) costs more than the line-search arithmetic it guards
+// once there are a few hundred blocks; a kernel boundary publishes the partials.
+template <int NV, typename F>
+__device__ inline void partitioned_partials(int n, int span_min, F item, double *part, int bx, int nbx)
+{
+    __shared__ double sh[NV][RED_THREADS / 64];
+    const int nparts = red_nparts(n, span_min);
+    const int span = (n + nparts - 1) / nparts;
+    for (int q = bx; q < nparts; q += nbx) {
+        double v[NV];
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+            v[j] = 0.0;
+        const int beg = q * span, end = beg + span < n ? beg + span : n;
+        for (int i = beg + threadIdx.x; i < end; i += RED_THREADS)
+            item(i, v);
+        block_sum<NV>(v, sh);
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int j = 0; j < NV; j++)
+                part[q * NV + j] = v[j];
+        }
+    }
+}
+
+// lane k's value of x (wave-uniform result, kept in scalar registers)
+__device__ __forceinline__ double readlane_d(double x, int k)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 template <int NV, typename F>
 __device__ inline void partitioned_reduce(int n, F item, double *part, unsigned *ticket, double *out)
 {

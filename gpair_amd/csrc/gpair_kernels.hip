@@ -857,85 +857,98 @@ __global__ void k_vis_to_double(const float2 *__restrict__ vis, int nuv, double2
 // Batched line search, one launch per candidate batch: block columns
 // [0, nb_chi2) take the chi2 change of every candidate, columns [nb_chi2,
 // nb_chi2 + nb_ent) the entropy change; block row y takes candidates
-// 8 y .. 8 y + 7 (LS_G per row: fewer live registers per lane).  Both halves are
-// partitioned reductions of one item per thread per partition, with their own
-// partials and tickets (part: [2][LS_K / LS_G][RED_PARTS][LS_G], tickets
-// 0..3 chi2, 4..7 entropy), so their result does not depend on nb_chi2 / nb_ent.
-// out[k] = chi2 change of candidate k, out[LS_K + k] = its entropy change.
-#define LS_SPAN RED_THREADS
+// LS_G y .. LS_G y + LS_G - 1 (fewer live registers per lane).  Both halves store
+// the sums of fixed partitions of LS_SPAN items (part: [2][LS_K / LS_G][RED_PARTS][LS_G],
+// chi2 half first), so that neither depends on nb_chi2 / nb_ent; k_ls_choose adds
+// them up in a fixed order.
+#define LS_SPAN (2 * RED_THREADS)
+#define LS_HALF ((LS_K / LS_G) * RED_PARTS * LS_G)   // doubles of partials per half
 
 // chi2(s_k) - chi2(0) from V = U / F in FP64 and the direction's DFT U_d:
-// V(s) = (U + s U_d) / (F + s F_d)
+// V(s) = (U + s U_d) / (F + s F_d).  Per item the numerator is a polynomial in s
+// (|U + s U_d|^2 for a V2 point, the product of three such legs times the phasor
+// for a T3), expanded once and evaluated per candidate by Horner's rule; the
+// candidates' s_k and powers of 1 / (F + s_k F_d) are wave-uniform (lane k computes
+// candidate k's, the others read them from scalar registers).
 __device__ __forceinline__ void ls_chi2_part(const double2 *__restrict__ Vd, const float2 *__restrict__ Ud,
                                              const double *__restrict__ scal, double s_ref,
                                              const float *__restrict__ data, const float *__restrict__ err,
                                              const float2 *__restrict__ phasor, const int4 *__restrict__ tri,
-                                             int npow, int nbis, int grp, int bx, int nbx, double *part,
-                                             unsigned *ticket, double *out)
+                                             int npow, int nbis, int grp, int bx, int nbx, double *part)
 {
     const int g0 = grp * LS_G;
     const double FI = scal[SC_FLUX], Fd = scal[SC_FD];
-    double sk[LS_G], ivk[LS_G];
+    const double s_l = ls_step(s_ref, g0 + (int)(threadIdx.x & (LS_G - 1)));
+    const double iv_l = 1.0 / (FI + s_l * Fd);
+    const double iv2_l = iv_l * iv_l, iv3_l = iv2_l * iv_l;
+    double sk[LS_G], iv2[LS_G], iv3[LS_G];
 #pragma unroll
     for (int k = 0; k < LS_G; k++) {
-        sk[k] = ls_step(s_ref, g0 + k);
-        ivk[k] = 1.0 / (FI + sk[k] * Fd);
+        sk[k] = readlane_d(s_l, k);
+        iv2[k] = readlane_d(iv2_l, k);
+        iv3[k] = readlane_d(iv3_l, k);
     }
+    const double iF = 1.0 / FI, iF3 = iF * iF * iF;
     auto item = [&](int i, double (&v)[LS_G]) {
         if (i < npow) {
             const double2 a = Vd[i];
             const double bx = Ud[i].x, by = Ud[i].y, d0 = data[i], e = err[i];
             const double r0 = (a.x * a.x + a.y * a.y - d0) * e;
             const double fx = FI * a.x, fy = FI * a.y;
+            // |U + s U_d|^2 = c0 + s (c1 + s c2)
+            const double c0 = fx * fx + fy * fy, c1 = 2.0 * (fx * bx + fy * by), c2 = bx * bx + by * by;
 #pragma unroll
             for (int k = 0; k < LS_G; k++) {
-                const double x = (fx + sk[k] * bx) * ivk[k], y = (fy + sk[k] * by) * ivk[k];
-                const double r = (x * x + y * y - d0) * e;
+                const double q = fma(fma(c2, sk[k], c1), sk[k], c0);
+                const double r = (q * iv2[k] - d0) * e;
                 v[k] += r * r - r0 * r0;
             }
         } else {
             const int t = i - npow;
             const int4 tr = tri[t];
             const int idx[3] = {tr.x, tr.y, tr.z};
-            double2 a[3], b[3];
+            double2 A[3], B[3];   // legs A_j + s B_j (conjugated where the triangle says)
 #pragma unroll
             for (int j = 0; j < 3; j++) {
-                a[j] = Vd[idx[j]];
-                a[j].x *= FI;
-                a[j].y *= FI;
-                b[j] = make_double2(Ud[idx[j]].x, Ud[idx[j]].y);
+                const double2 a = Vd[idx[j]];
+                const float2 u = Ud[idx[j]];
+                const bool cj = (tr.w >> j) & 1;
+                A[j] = make_double2(FI * a.x, cj ? -(FI * a.y) : FI * a.y);
+                B[j] = make_double2((double)u.x, cj ? -(double)u.y : (double)u.y);
             }
             const double2 rho = make_double2(phasor[t].x, phasor[t].y);
+            A[2] = dcmul(A[2], rho);
+            B[2] = dcmul(B[2], rho);
+            // (A0 + s B0)(A1 + s B1) = p0 + s p1 + s^2 p2, times (A2 + s B2) = c0 + s c1 + s^2 c2 + s^3 c3
+            const double2 p0 = dcmul(A[0], A[1]), p2 = dcmul(B[0], B[1]);
+            const double2 p1a = dcmul(A[0], B[1]), p1b = dcmul(B[0], A[1]);
+            const double2 p1 = make_double2(p1a.x + p1b.x, p1a.y + p1b.y);
+            const double2 c0 = dcmul(p0, A[2]), c3 = dcmul(p2, B[2]);
+            const double2 c1a = dcmul(p0, B[2]), c1b = dcmul(p1, A[2]);
+            const double2 c2a = dcmul(p1, B[2]), c2b = dcmul(p2, A[2]);
+            const double2 c1 = make_double2(c1a.x + c1b.x, c1a.y + c1b.y);
+            const double2 c2 = make_double2(c2a.x + c2b.x, c2a.y + c2b.y);
             const int kk = npow + 2 * t;
             const double dr = data[kk], di = data[kk + 1], er = err[kk], ei = err[kk + 1];
-            const double iF = 1.0 / FI;
-            double2 T0 = dcmul(dcmul(dcmul(dleg(a[0], tr.w & 1), dleg(a[1], tr.w & 2)), dleg(a[2], tr.w & 4)), rho);
-            T0.x *= iF * iF * iF;
-            T0.y *= iF * iF * iF;
-            const double q0 = ((T0.x - dr) * er) * ((T0.x - dr) * er) + ((T0.y - di) * ei) * ((T0.y - di) * ei);
+            const double q0x = (c0.x * iF3 - dr) * er, q0y = (c0.y * iF3 - di) * ei;
+            const double q0 = q0x * q0x + q0y * q0y;
 #pragma unroll
             for (int k = 0; k < LS_G; k++) {
-                double2 l[3];
-#pragma unroll
-                for (int j = 0; j < 3; j++)
-                    l[j] = make_double2(a[j].x + sk[k] * b[j].x, a[j].y + sk[k] * b[j].y);
-                double2 T =
-                    dcmul(dcmul(dcmul(dleg(l[0], tr.w & 1), dleg(l[1], tr.w & 2)), dleg(l[2], tr.w & 4)), rho);
-                const double iv3 = ivk[k] * ivk[k] * ivk[k];
-                const double r0 = (T.x * iv3 - dr) * er, r1 = (T.y * iv3 - di) * ei;
-                v[k] += r0 * r0 + r1 * r1 - q0;
+                const double s = sk[k];
+                const double tx = fma(fma(fma(c3.x, s, c2.x), s, c1.x), s, c0.x);
+                const double ty = fma(fma(fma(c3.y, s, c2.y), s, c1.y), s, c0.y);
+                const double rx = (tx * iv3[k] - dr) * er, ry = (ty * iv3[k] - di) * ei;
+                v[k] += rx * rx + ry * ry - q0;
             }
         }
     };
-    partitioned_reduce_at<LS_G>(npow + nbis, LS_SPAN, item, part + (size_t)grp * RED_PARTS * LS_G, ticket + grp,
-                                out + g0, bx, nbx);
+    partitioned_partials<LS_G>(npow + nbis, LS_SPAN, item, part + (size_t)grp * RED_PARTS * LS_G, bx, nbx);
 }
 
 // S(max(I + s_k d, minval)) - S(I), summed over the pixels
 __device__ __forceinline__ void ls_entropy_part(const float *__restrict__ img, const float *__restrict__ d,
                                                 const float *__restrict__ model, double s_ref, double minval, int n,
-                                                int grp, int bx, int nbx, double *part, unsigned *ticket,
-                                                double *out)
+                                                int grp, int bx, int nbx, double *part)
 {
     const int g0 = grp * LS_G;
     float sk[LS_G];
@@ -963,8 +976,7 @@ __device__ __forceinline__ void ls_entropy_part(const float *__restrict__ img, c
             v[k] += (double)(dl * (1.0f - L) - t * lr);
         }
     };
-    partitioned_reduce_at<LS_G>(n, LS_SPAN, item, part + (size_t)grp * RED_PARTS * LS_G, ticket + grp, out + g0,
-                                bx, nbx);
+    partitioned_partials<LS_G>(n, LS_SPAN, item, part + (size_t)grp * RED_PARTS * LS_G, bx, nbx);
 }
 
 __global__ __launch_bounds__(RED_THREADS) void k_ls_batch(const double2 *__restrict__ Vd,
@@ -976,35 +988,58 @@ __global__ __launch_bounds__(RED_THREADS) void k_ls_batch(const double2 *__restr
                                                           const int4 *__restrict__ tri, int npow, int nbis,
                                                           const float *__restrict__ img, const float *__restrict__ d,
                                                           const float *__restrict__ model, double minval, int n,
-                                                          int nb_chi2, int nb_ent, double *part, unsigned *ticket,
-                                                          double *out)
+                                                          int nb_chi2, int nb_ent, double *part)
 {
-    constexpr int ROWS = LS_K / LS_G;
     const int grp = blockIdx.y, bx = blockIdx.x;
     if (bx < nb_chi2)
-        ls_chi2_part(Vd, Ud, scal, s_ref, data, err, phasor, tri, npow, nbis, grp, bx, nb_chi2, part, ticket, out);
+        ls_chi2_part(Vd, Ud, scal, s_ref, data, err, phasor, tri, npow, nbis, grp, bx, nb_chi2, part);
     else
-        ls_entropy_part(img, d, model, s_ref, minval, n, grp, bx - nb_chi2, nb_ent,
-                        part + (size_t)ROWS * RED_PARTS * LS_G, ticket + ROWS, out + LS_K);
+        ls_entropy_part(img, d, model, s_ref, minval, n, grp, bx - nb_chi2, nb_ent, part + LS_HALF);
 }
 
 // block columns of each half of k_ls_batch: one per partition unless a smaller count is given
 __host__ inline int ls_nparts_chi2(int npow, int nbis) { return red_nparts(npow + nbis, LS_SPAN); }
 __host__ inline int ls_nparts_ent(int n) { return red_nparts(n, LS_SPAN); }
 
-// Step from a line-search batch, on the device so that the commit can follow the
-// batch without a read-back: when d is a descent direction (d.g < 0, sum |d| > 0)
+// Sums of a line-search batch and the step they give, on the device so that the
+// commit can follow the batch without a read-back.  First ls[m] (m < LS_K: chi2
+// change of candidate m, else the entropy change of candidate m - LS_K): LS_FIN_SUB
+// lanes per sum take every LS_FIN_SUB-th partition in order, then a fixed butterfly
+// adds them.  Then the step: when d is a descent direction (d.g < 0, sum |d| > 0)
 // and some candidate k > 0 lowers the objective, the best one (first minimum of
 // dJ_k = dchi2_k - alpha dS_k), refined by the parabola through its neighbours;
 // otherwise step 0 and index -1.  The host reads both back with the commit's
 // results.  One wave: lane k takes candidate k, a shuffle reduction finds the first
 // minimum, lane 0 evaluates the parabola.
-__global__ __launch_bounds__(64) void k_ls_choose(const double *__restrict__ ls, double alpha, double s_ref,
-                                                  double *scal)
+#define LS_FIN_SUB 16
+#define LS_FIN_THREADS (2 * LS_K * LS_FIN_SUB)
+__global__ __launch_bounds__(LS_FIN_THREADS) void k_ls_choose(const double *__restrict__ part, int np_chi2,
+                                                              int np_ent, double alpha, double s_ref,
+                                                              double *scal, double *__restrict__ ls)
 {
 #pragma clang fp contract(off)
+    __shared__ double sh[2 * LS_K];
+    {
+        const int m = threadIdx.x / LS_FIN_SUB, sub = threadIdx.x % LS_FIN_SUB;
+        const int half = m / LS_K, c = m % LS_K;
+        const double *p = part + (size_t)half * LS_HALF + (size_t)(c / LS_G) * RED_PARTS * LS_G + c % LS_G;
+        const int np = half ? np_ent : np_chi2;
+        double acc = 0.0;
+        for (int q = sub; q < np; q += LS_FIN_SUB)
+            acc += p[(size_t)q * LS_G];
+#pragma unroll
+        for (int o = LS_FIN_SUB / 2; o > 0; o >>= 1)
+            acc += __shfl_xor(acc, o, 64);
+        if (sub == 0) {
+            sh[m] = acc;
+            ls[m] = acc;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x >= 64)
+        return;
     const int k = threadIdx.x;
-    const double dj = k < LS_K ? ls[k] - alpha * ls[LS_K + k] : 0.0;
+    const double dj = k < LS_K ? sh[k] - alpha * sh[LS_K + k] : 0.0;
     double best = k < LS_K && dj == dj ? dj : __builtin_huge_val();   // NaN never the minimum
     int kb = k;
 #pragma unroll
