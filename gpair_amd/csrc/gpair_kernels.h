@@ -23,9 +23,9 @@ enum {
     SC_W1 = 7,     // d.g, sum d, sum |d|
     SC_FD = 8,
     SC_DABS = 9,
-    SC_FDELTA = 10,  // sum of the committed image change, entropy change, chi2 change
-    SC_DSENT = 11,
-    SC_DCHI2 = 12,
+    SC_FDELTA = 10,  // sum of the committed image change, its entropy change, and the
+    SC_DSENT = 11,   // number of pixels the floor clamped (a diagnostic: the commit's
+    SC_NCLAMP = 12,  // change is s d only where no pixel is clamped)
     SC_BETA = 13,    // beta the direction kernel used
     SC_PXF = 14,     // flux of the pixel-move state
     SC_PXC = 15,     // chi2 of the pixel-move state
@@ -33,7 +33,8 @@ enum {
     SC_KBEST = 17,   // its candidate index (-1: no decrease, or not a descent direction)
     SC_AMAXI = 18,   // max |x| of the image of the last FP16 forward DFT
     SC_AMAXC = 19,   // max |Re|, |Im| of the adjoint coefficients of the last FP16 adjoint
-    GPAIR_NSCAL = 20
+    SC_DCHI2 = 20,   // chi2 change of the committed image
+    GPAIR_NSCAL = 21
 };
 
 #define RED_THREADS 256

@@ -1066,8 +1066,9 @@ __global__ __launch_bounds__(LS_FIN_THREADS) void k_ls_choose(const double *__re
 }
 
 // trial image I_t = max(I + s d, minval), the change delta = I_t - I, and
-// out[0] = sum delta, out[1] = S(I_t) - S(I); with th set, also the BF16 hi / lo
-// split of delta for the forward DFT that follows
+// out[0] = sum delta, out[1] = S(I_t) - S(I), out[2] = number of pixels where the
+// floor replaced I + s d; with th set, also the BF16 hi / lo split of delta for the
+// forward DFT that follows
 __global__ __launch_bounds__(RED_THREADS) void k_commit_step(const float *__restrict__ img,
                                                              const float *__restrict__ d,
                                                              const float *__restrict__ model, float s_arg,
@@ -1079,12 +1080,14 @@ __global__ __launch_bounds__(RED_THREADS) void k_commit_step(const float *__rest
                                                              unsigned short *__restrict__ tl)
 {
     const float s = s_dev ? (float)*s_dev : s_arg;
-    double v[2] = {0.0, 0.0};
+    double v[3] = {0.0, 0.0, 0.0};
     for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) {
         const float I = img[i];
         float t = I + s * d[i];
-        if (t < minval)
+        if (t < minval) {
             t = minval;
+            v[2] += 1.0;
+        }
         trial[i] = t;
         const float dl = t - I;
         delta[i] = dl;
@@ -1098,7 +1101,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_commit_step(const float *__rest
         v[0] += (double)dl;
         v[1] += pix_entropy((double)t, M) - pix_entropy((double)I, M);
     }
-    reduce_finish<2>(v, part, ticket, out, nullptr);
+    reduce_finish<3>(v, part, ticket, out, nullptr);
 }
 
 // visibilities after the committed change: (F V + U_delta) / (F + F_delta)
