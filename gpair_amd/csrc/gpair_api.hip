@@ -47,6 +47,7 @@ struct gpair_ctx {
     double *scal;
     float *invflux;
     double *part;
+    double *zpart;   // z.g, z.g_old partials of k_grad_finish_crit, 2 per block
     unsigned *ticket;
     double *host_scal;   // pinned
     // adjoint split-K partials
@@ -65,7 +66,7 @@ struct gpair_ctx {
     double2 *Vd, *Vn;
     float *mock2;
     float *pfac;
-    double *ls_out, *host_ls;   // device / pinned host, 2*LS_K
+    double *ls_out, *host_ls;   // 2*LS_K line-search sums, right after scal / host_scal
     // launch shapes chosen by gpair_autotune (0: the built-in default)
     int ls_blocks_chi2, ls_blocks_ent, coef_blocks;
     int fwd_nbk;   // image rows per BF16x3 forward group / 32 (0: min(W, 256) / 32)
@@ -194,19 +195,21 @@ extern "C" int gpu_init(gpair_ctx **out, int device)
         return GPAIR_E_HIP;
     }
     int rc;
-    if ((rc = dalloc(c, &c->scal, GPAIR_NSCAL)) || (rc = dalloc(c, &c->invflux, 4)) ||
-        (rc = dalloc(c, &c->part, (size_t)RED_MAX_BLOCKS * RED_MAX_NV)) || (rc = dalloc(c, &c->ticket, GPAIR_NTICKET)) ||
-        (rc = dalloc(c, &c->ls_out, 2 * LS_K))) {
+    // the line-search sums follow the scalars, so that one copy reads both back
+    if ((rc = dalloc(c, &c->scal, GPAIR_NSCAL + 2 * LS_K)) || (rc = dalloc(c, &c->invflux, 4)) ||
+        (rc = dalloc(c, &c->part, (size_t)RED_MAX_BLOCKS * RED_MAX_NV)) || (rc = dalloc(c, &c->ticket, GPAIR_NTICKET))) {
         gpu_cleanup(c);
         return rc;
     }
-    if (hipHostMalloc((void **)&c->host_scal, sizeof(double) * GPAIR_NSCAL, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&c->host_ls, sizeof(double) * 2 * LS_K, hipHostMallocDefault) != hipSuccess) {
+    c->ls_out = c->scal + GPAIR_NSCAL;
+    if (hipHostMalloc((void **)&c->host_scal, sizeof(double) * (GPAIR_NSCAL + 2 * LS_K), hipHostMallocDefault) !=
+        hipSuccess) {
         gpu_cleanup(c);
         return GPAIR_E_NOMEM;
     }
+    c->host_ls = c->host_scal + GPAIR_NSCAL;
     if (hipMemsetAsync(c->ticket, 0, GPAIR_NTICKET * sizeof(unsigned), c->stream) != hipSuccess ||
-        hipMemsetAsync(c->scal, 0, GPAIR_NSCAL * sizeof(double), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->scal, 0, (GPAIR_NSCAL + 2 * LS_K) * sizeof(double), c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess) {
         gpu_cleanup(c);
         return GPAIR_E_HIP;
@@ -226,6 +229,7 @@ static void free_images(gpair_ctx *c)
         dfree(&b->p);
         b->n = 0;
     }
+    dfree(&c->zpart);
 }
 
 extern "C" int gpu_cleanup(gpair_ctx *c)
@@ -240,13 +244,13 @@ extern "C" int gpu_cleanup(gpair_ctx *c)
     dfree(&c->data); dfree(&c->err); dfree(&c->phasor); dfree(&c->tri); dfree(&c->csr_ptr); dfree(&c->csr_ent);
     dfree(&c->X); dfree(&c->Y); dfree(&c->FX); dfree(&c->FY); dfree(&c->vis); dfree(&c->coef); dfree(&c->mock);
     dfree(&c->scal); dfree(&c->invflux); dfree(&c->part); dfree(&c->ticket); dfree(&c->adj_part);
-    dfree(&c->Ud); dfree(&c->Udel); dfree(&c->pfac); dfree(&c->ls_out);
+    dfree(&c->Ud); dfree(&c->Udel); dfree(&c->pfac);
+    c->ls_out = nullptr;   // inside scal
     dfree(&c->vis2); dfree(&c->Vd); dfree(&c->Vn); dfree(&c->mock2);
     dfree(&c->Xh); dfree(&c->Xl); dfree(&c->Ih); dfree(&c->Il); dfree(&c->Xth); dfree(&c->Xtl); dfree(&c->fwdp);
     if (c->host_scal)
         (void)hipHostFree(c->host_scal);
-    if (c->host_ls)
-        (void)hipHostFree(c->host_ls);
+    c->host_ls = nullptr;   // inside host_scal
     for (int k = 0; k < GPAIR_TK_COUNT; k++)
         for (hipEvent_t e : c->ev[k])
             (void)hipEventDestroy(e);
@@ -302,6 +306,8 @@ static int alloc_images(gpair_ctx *c, int W)
         b->n = n;
         HIPCHK(c, hipMemsetAsync(b->p, 0, sizeof(float) * n, c->stream));
     }
+    if ((rc = dalloc(c, &c->zpart, 2 * (size_t)((n + GF_PIX - 1) / GF_PIX))))
+        return rc;
     c->W = W;
     c->n = n;
     return GPAIR_OK;
@@ -838,10 +844,21 @@ static int launch_coef(gpair_ctx *c)
     return GPAIR_OK;
 }
 
+// launch_flux(img) and launch_coef in one launch (the same results)
+static int launch_coef_flux(gpair_ctx *c, const float *img)
+{
+    const int nbc = tuned_blocks(c->coef_blocks, red_blocks(c->nuv)), nbf = red_blocks(c->n);
+    k_coef_flux<<<nbc + nbf, RED_THREADS, 0, c->stream>>>(c->vis, c->mock, c->data, c->err, c->phasor, c->tri,
+                                                           c->csr_ptr, c->csr_ent, c->npow, c->nuv, c->coef, nbc, img,
+                                                           c->n, c->part, c->ticket, c->scal, c->invflux);
+    LAUNCHCHK(c);
+    return GPAIR_OK;
+}
+
 static int launch_data_gradient(gpair_ctx *c, const float *img)
 {
     int rc;
-    if ((rc = launch_flux(c, img)) || (rc = launch_coef(c)))
+    if ((rc = launch_coef_flux(c, img)))
         return rc;
     return launch_adjoint(c);
 }

@@ -225,7 +225,7 @@ __host__ __device__ inline int red_nparts(int n, int span)
 // explicitly (for remapped launches).
 template <int NV, typename F>
 __device__ inline void partitioned_reduce_at(int n, int span_min, F item, double *part, unsigned *ticket,
-                                             double *out, int bx, int nbx)
+                                             double *out, int bx, int nbx, float *invflux_out = nullptr)
 {
     __shared__ double sh[NV][RED_THREADS / 64];
     __shared__ int is_last;
@@ -267,6 +267,8 @@ __device__ inline void partitioned_reduce_at(int n, int span_min, F item, double
 #pragma unroll
         for (int j = 0; j < NV; j++)
             out[j] = acc[j];
+        if (invflux_out)
+            *invflux_out = (float)(1.0 / acc[0]);
         __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -309,9 +311,11 @@ __device__ __forceinline__ double readlane_d(double x, int k)
 }
 
 template <int NV, typename F>
-__device__ inline void partitioned_reduce(int n, F item, double *part, unsigned *ticket, double *out)
+__device__ inline void partitioned_reduce(int n, F item, double *part, unsigned *ticket, double *out,
+                                         float *invflux_out = nullptr)
 {
-    partitioned_reduce_at<NV>(n, 2 * RED_THREADS, item, part, ticket, out, (int)blockIdx.x, (int)gridDim.x);
+    partitioned_reduce_at<NV>(n, 2 * RED_THREADS, item, part, ticket, out, (int)blockIdx.x, (int)gridDim.x,
+                              invflux_out);
 }
 
 // ---------------------------------------------------------------------------

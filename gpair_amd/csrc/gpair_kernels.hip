@@ -38,13 +38,18 @@ __global__ void k_transpose_tables(const float2 *__restrict__ in, int nuv, int W
 // image reductions
 // ---------------------------------------------------------------------------
 // flux F = sum I (gpair1/trunk/gpu.c:860-884), also 1/F as float
+// flux as a partitioned reduction: the same bits for any grid, alone or fused with k_coef
+__device__ __forceinline__ void flux_part(const float *__restrict__ img, int n, int bx, int nbx, double *part,
+                                          unsigned *ticket, double *scal, float *invflux)
+{
+    auto item = [&](int i, double (&v)[1]) { v[0] += (double)img[i]; };
+    partitioned_reduce_at<1>(n, 2 * RED_THREADS, item, part, ticket, scal + SC_FLUX, bx, nbx, invflux);
+}
+
 __global__ __launch_bounds__(RED_THREADS) void k_flux(const float *__restrict__ img, int n, double *part,
                                                       unsigned *ticket, double *scal, float *invflux)
 {
-    double v[1] = {0.0};
-    for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS)
-        v[0] += (double)img[i];
-    reduce_finish<1>(v, part, ticket, scal + SC_FLUX, invflux);
+    flux_part(img, n, (int)blockIdx.x, (int)gridDim.x, part, ticket, scal, invflux);
 }
 
 // Gull-Skilling entropy (gpair1/trunk/kernel_entropy_gs.cl:3-26), per-pixel in FP64
@@ -247,16 +252,12 @@ __global__ __launch_bounds__(RED_THREADS) void k_model_chi2(const float2 *__rest
 //                  a_p += b, or conj(b) for a conjugated leg
 // c0 = Re sum_p a_p V_p
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(RED_THREADS) void k_coef(const float2 *__restrict__ vis,
-                                                      const float *__restrict__ mock,
-                                                      const float *__restrict__ data,
-                                                      const float *__restrict__ err,
-                                                      const float2 *__restrict__ phasor,
-                                                      const int4 *__restrict__ tri,
-                                                      const int *__restrict__ csr_ptr,
-                                                      const int *__restrict__ csr_ent, int npow, int nuv,
-                                                      float2 *__restrict__ coef, double *part, unsigned *ticket,
-                                                      double *scal)
+__device__ __forceinline__ void coef_part(const float2 *__restrict__ vis, const float *__restrict__ mock,
+                                          const float *__restrict__ data, const float *__restrict__ err,
+                                          const float2 *__restrict__ phasor, const int4 *__restrict__ tri,
+                                          const int *__restrict__ csr_ptr, const int *__restrict__ csr_ent, int npow,
+                                          int nuv, float2 *__restrict__ coef, double *part, unsigned *ticket,
+                                          double *scal, int bx, int nbx)
 {
     auto item = [&](int p, double (&v)[1]) {
         const float2 Vp = vis[p];
@@ -286,7 +287,45 @@ __global__ __launch_bounds__(RED_THREADS) void k_coef(const float2 *__restrict__
         coef[p] = a;
         v[0] += (double)a.x * (double)Vp.x - (double)a.y * (double)Vp.y;
     };
-    partitioned_reduce<1>(nuv, item, part, ticket, scal + SC_C0);
+    partitioned_reduce_at<1>(nuv, 2 * RED_THREADS, item, part, ticket, scal + SC_C0, bx, nbx);
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_coef(const float2 *__restrict__ vis,
+                                                      const float *__restrict__ mock,
+                                                      const float *__restrict__ data,
+                                                      const float *__restrict__ err,
+                                                      const float2 *__restrict__ phasor,
+                                                      const int4 *__restrict__ tri,
+                                                      const int *__restrict__ csr_ptr,
+                                                      const int *__restrict__ csr_ent, int npow, int nuv,
+                                                      float2 *__restrict__ coef, double *part, unsigned *ticket,
+                                                      double *scal)
+{
+    coef_part(vis, mock, data, err, phasor, tri, csr_ptr, csr_ent, npow, nuv, coef, part, ticket, scal,
+              (int)blockIdx.x, (int)gridDim.x);
+}
+
+// k_coef and k_flux in one launch: block columns [0, nb_coef) take the
+// coefficients, the rest the flux of img (its own partials at part + RED_PARTS and
+// ticket + 1); each result is that of the kernel alone
+__global__ __launch_bounds__(RED_THREADS) void k_coef_flux(const float2 *__restrict__ vis,
+                                                           const float *__restrict__ mock,
+                                                           const float *__restrict__ data,
+                                                           const float *__restrict__ err,
+                                                           const float2 *__restrict__ phasor,
+                                                           const int4 *__restrict__ tri,
+                                                           const int *__restrict__ csr_ptr,
+                                                           const int *__restrict__ csr_ent, int npow, int nuv,
+                                                           float2 *__restrict__ coef, int nb_coef,
+                                                           const float *__restrict__ img, int n, double *part,
+                                                           unsigned *ticket, double *scal, float *invflux)
+{
+    const int bx = (int)blockIdx.x;
+    if (bx < nb_coef)
+        coef_part(vis, mock, data, err, phasor, tri, csr_ptr, csr_ent, npow, nuv, coef, part, ticket, scal, bx,
+                  nb_coef);
+    else
+        flux_part(img, n, bx - nb_coef, (int)gridDim.x - nb_coef, part + RED_PARTS, ticket + 1, scal, invflux);
 }
 
 // ---------------------------------------------------------------------------
@@ -407,14 +446,19 @@ __global__ __launch_bounds__(256) void k_grad_finish(const float *__restrict__ p
 }
 
 // k_grad_finish, then per pixel the entropy gradient (stale value kept where I <= 0
-// or M <= 0, as k_entropy_grad) and the criterion gradient g = gdata - alpha gent
-// (as k_crit_grad): one launch instead of three in the accelerated loop.
+// or M <= 0, as k_entropy_grad), the criterion gradient g = gdata - alpha gent (as
+// k_crit_grad) and the preconditioned gradient z = P g (P = image or 1, times pfac
+// when set): one launch instead of four in the accelerated loop.  Block b stores its
+// pixels' z.g and z.g_old in zpart[2b], zpart[2b + 1] (plain stores, no ticket);
+// k_pcg_dir adds them up.
 __global__ __launch_bounds__(256) void k_grad_finish_crit(const float *__restrict__ part, int nsplit, int n,
                                                           const double *__restrict__ scal,
                                                           const float *__restrict__ img,
                                                           const float *__restrict__ model, float alpha,
                                                           float *__restrict__ gdata, float *__restrict__ gent,
-                                                          float *__restrict__ g)
+                                                          float *__restrict__ g, const float *__restrict__ g_old,
+                                                          const float *__restrict__ pfac, int precond,
+                                                          float *__restrict__ z, double *__restrict__ zpart)
 {
     __shared__ double sh[4][GF_PIX];
     const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -438,7 +482,10 @@ __global__ __launch_bounds__(256) void k_grad_finish_crit(const float *__restric
     }
     sh[q][lane] = s;
     __syncthreads();
-    if (q == 0 && i < n) {
+    if (q != 0)
+        return;
+    double zg = 0.0, zgo = 0.0;
+    if (i < n) {
         const double t = ((sh[0][lane] + sh[1][lane]) + sh[2][lane]) + sh[3][lane];
         const float gd = (float)((t - scal[SC_C0]) / scal[SC_FLUX]);
         gdata[i] = gd;
@@ -448,7 +495,21 @@ __global__ __launch_bounds__(256) void k_grad_finish_crit(const float *__restric
             ge = -logf(I / M);
             gent[i] = ge;
         }
-        g[i] = gd - alpha * ge;
+        const float gi = gd - alpha * ge;
+        g[i] = gi;
+        float pf = precond ? I : 1.0f;
+        if (pfac)
+            pf *= pfac[i];
+        const float zi = pf * gi;
+        z[i] = zi;
+        zg = (double)zi * (double)gi;
+        zgo = (double)zi * (double)g_old[i];
+    }
+    zg = wave_sum(zg);
+    zgo = wave_sum(zgo);
+    if (lane == 0) {
+        zpart[2 * blockIdx.x] = zg;
+        zpart[2 * blockIdx.x + 1] = zgo;
     }
 }
 
@@ -767,42 +828,47 @@ __global__ void k_step(const float *img, const float *__restrict__ d, float s, f
 // ---------------------------------------------------------------------------
 // accelerated loop: preconditioned direction and batched line search
 // ---------------------------------------------------------------------------
-// z = P g with P = image (MEM metric) or 1, times an optional per-pixel factor;
-// out[0] = z.g, out[1] = z.g_old
-__global__ __launch_bounds__(RED_THREADS) void k_pcg_precond(const float *__restrict__ img,
-                                                             const float *__restrict__ g,
-                                                             const float *__restrict__ g_old,
-                                                             const float *__restrict__ pfac, int precond, int n,
-                                                             float *__restrict__ z, double *part, unsigned *ticket,
-                                                             double *out)
-{
-    double v[2] = {0.0, 0.0};
-    for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) {
-        float pf = precond ? img[i] : 1.0f;
-        if (pfac)
-            pf *= pfac[i];
-        const float zi = pf * g[i];
-        z[i] = zi;
-        v[0] += (double)zi * (double)g[i];
-        v[1] += (double)zi * (double)g_old[i];
-    }
-    reduce_finish<2>(v, part, ticket, out, nullptr);
-}
-
 // d = -z + beta d, projected to zero where the pixel sits on the floor and d < 0;
-// out[0] = d.g, out[1] = sum d, out[2] = sum |d|.  conj != 0: Polak-Ribiere+
-// beta = max(0, (z.g - z.g_old) / zg_old) from the z.g slots the preconditioner
-// kernel left in scal (kept in scal[SC_BETA]); conj == 0: beta = 0 (-z).  With dh
-// set, also the BF16 hi / lo split of d for the forward DFT that follows.
+// out[0] = d.g, out[1] = sum d, out[2] = sum |d|.  With zpart set, every block
+// first adds up the nzp pairs of z.g and z.g_old partials k_grad_finish_crit left
+// (the same fixed order in every block; block 0 keeps them in scal[SC_ZG],
+// scal[SC_ZGO]), else it takes those two slots as they are.  conj != 0:
+// Polak-Ribiere+ beta = max(0, (z.g - z.g_old) / zg_old) (kept in scal[SC_BETA]);
+// conj == 0: beta = 0 (-z).  With dh set, also the BF16 hi / lo split of d for the
+// forward DFT that follows.
 __global__ __launch_bounds__(RED_THREADS) void k_pcg_dir(const float *__restrict__ img, const float *__restrict__ z,
                                                          const float *__restrict__ g, int conj, double zg_old,
                                                          float minval, int n, float *__restrict__ d, double *scal,
+                                                         const double *__restrict__ zpart, int nzp,
                                                          double *part, unsigned *ticket, double *out,
                                                          unsigned short *__restrict__ dh, unsigned short *__restrict__ dl)
 {
+    __shared__ double shz[2][RED_THREADS / 64];
+    __shared__ double zsum[2];
+    if (zpart) {
+        double zv[2] = {0.0, 0.0};
+        for (int b = threadIdx.x; b < nzp; b += RED_THREADS) {
+            zv[0] += zpart[2 * b];
+            zv[1] += zpart[2 * b + 1];
+        }
+        block_sum<2>(zv, shz);
+        if (threadIdx.x == 0) {
+            zsum[0] = zv[0];
+            zsum[1] = zv[1];
+            if (blockIdx.x == 0) {
+                scal[SC_ZG] = zv[0];
+                scal[SC_ZGO] = zv[1];
+            }
+        }
+        __syncthreads();
+    } else if (threadIdx.x == 0) {
+        zsum[0] = scal[SC_ZG];
+        zsum[1] = scal[SC_ZGO];
+    }
+    __syncthreads();
     double b = 0.0;
     if (conj) {
-        b = (scal[SC_ZG] - scal[SC_ZGO]) / zg_old;
+        b = (zsum[0] - zsum[1]) / zg_old;
         if (!(b > 0.0))
             b = 0.0;
     }
