@@ -164,7 +164,7 @@ static int red_blocks(int n)
 {
     int b = (n + RED_THREADS - 1) / RED_THREADS;
     if (b < 1) b = 1;
-    if (b > 128) b = 128;
+    if (b > RED_GRID_MAX) b = RED_GRID_MAX;
     return b;
 }
 
@@ -195,21 +195,21 @@ extern "C" int gpu_init(gpair_ctx **out, int device)
         return GPAIR_E_HIP;
     }
     int rc;
-    // the line-search sums follow the scalars, so that one copy reads both back
-    if ((rc = dalloc(c, &c->scal, GPAIR_NSCAL + 2 * LS_K)) || (rc = dalloc(c, &c->invflux, 4)) ||
+    // the line-search sums and the commit partials follow the scalars, so that one
+    // copy reads them back (RB_* offsets)
+    if ((rc = dalloc(c, &c->scal, RB_COUNT)) || (rc = dalloc(c, &c->invflux, 4)) ||
         (rc = dalloc(c, &c->part, (size_t)RED_MAX_BLOCKS * RED_MAX_NV)) || (rc = dalloc(c, &c->ticket, GPAIR_NTICKET))) {
         gpu_cleanup(c);
         return rc;
     }
-    c->ls_out = c->scal + GPAIR_NSCAL;
-    if (hipHostMalloc((void **)&c->host_scal, sizeof(double) * (GPAIR_NSCAL + 2 * LS_K), hipHostMallocDefault) !=
-        hipSuccess) {
+    c->ls_out = c->scal + RB_LS;
+    if (hipHostMalloc((void **)&c->host_scal, sizeof(double) * RB_COUNT, hipHostMallocDefault) != hipSuccess) {
         gpu_cleanup(c);
         return GPAIR_E_NOMEM;
     }
-    c->host_ls = c->host_scal + GPAIR_NSCAL;
+    c->host_ls = c->host_scal + RB_LS;
     if (hipMemsetAsync(c->ticket, 0, GPAIR_NTICKET * sizeof(unsigned), c->stream) != hipSuccess ||
-        hipMemsetAsync(c->scal, 0, (GPAIR_NSCAL + 2 * LS_K) * sizeof(double), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->scal, 0, RB_COUNT * sizeof(double), c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess) {
         gpu_cleanup(c);
         return GPAIR_E_HIP;

@@ -23,9 +23,9 @@ enum {
     SC_W1 = 7,     // d.g, sum d, sum |d|
     SC_FD = 8,
     SC_DABS = 9,
-    SC_FDELTA = 10,  // sum of the committed image change, its entropy change, and the
-    SC_DSENT = 11,   // number of pixels the floor clamped (a diagnostic: the commit's
-    SC_NCLAMP = 12,  // change is s d only where no pixel is clamped)
+    SC_FDELTA = 10,  // host copy only, added up from the commit partials (RB_CSTEP):
+    SC_DSENT = 11,   // sum of the committed image change, its entropy change, and the
+    SC_NCLAMP = 12,  // number of pixels the floor clamped (a diagnostic)
     SC_BETA = 13,    // beta the direction kernel used
     SC_PXF = 14,     // flux of the pixel-move state
     SC_PXC = 15,     // chi2 of the pixel-move state
@@ -33,12 +33,13 @@ enum {
     SC_KBEST = 17,   // its candidate index (-1: no decrease, or not a descent direction)
     SC_AMAXI = 18,   // max |x| of the image of the last FP16 forward DFT
     SC_AMAXC = 19,   // max |Re|, |Im| of the adjoint coefficients of the last FP16 adjoint
-    SC_DCHI2 = 20,   // chi2 change of the committed image
+    SC_DCHI2 = 20,   // host copy only: chi2 change of the committed image (from RB_DCHI2)
     GPAIR_NSCAL = 21
 };
 
 #define RED_THREADS 256
 #define RED_MAX_BLOCKS 2048
+#define RED_GRID_MAX 128   // blocks of the grid-stride reductions (red_blocks)
 #define RED_MAX_NV 32
 #define GPAIR_NTICKET 16   // reduction tickets per context (the line-search batch uses 8)
 
@@ -46,6 +47,15 @@ enum {
 #define LS_K 32
 #define LS_TOP 26
 #define LS_G 8   // candidates per block row of the line-search kernels (grid.y = LS_K / LS_G)
+
+// The accelerated loop's read-back region, one device allocation read back in one
+// copy: the scalars, the line-search sums, then the commit kernels' per-block
+// partials (k_commit_step: sum delta, entropy change, clamped pixels per block;
+// k_commit_dchi2: chi2 change per block), which the host adds up in block order.
+#define RB_LS GPAIR_NSCAL
+#define RB_CSTEP (RB_LS + 2 * LS_K)
+#define RB_DCHI2 (RB_CSTEP + 3 * RED_GRID_MAX)
+#define RB_COUNT (RB_DCHI2 + RED_GRID_MAX)
 
 // candidate k of a batch: s_0 = 0, s_k = s_ref 2^((k - LS_TOP) / 2), built exactly
 // (a power of two times 1 or the rounded sqrt 2) so host and device agree bit for bit
@@ -161,6 +171,22 @@ __device__ inline void block_sum(double (&v)[NV], double (*sh)[RED_THREADS / 64]
         }
     }
     __syncthreads();
+}
+
+// sum of p[b * stride] over b < count in a fixed order, the same bits in every
+// block, returned to every thread (blockDim.x == RED_THREADS)
+__device__ inline double block_fixed_sum(const double *p, int stride, int count)
+{
+    __shared__ double sh[1][RED_THREADS / 64];
+    __shared__ double total;
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < count; b += RED_THREADS)
+        v[0] += p[(size_t)b * stride];
+    block_sum<1>(v, sh);
+    if (threadIdx.x == 0)
+        total = v[0];
+    __syncthreads();
+    return total;
 }
 
 // Every block calls this once with its thread-partial values.  Block sums go to

@@ -1131,20 +1131,21 @@ __global__ __launch_bounds__(LS_FIN_THREADS) void k_ls_choose(const double *__re
     scal[SC_KBEST] = kbo;
 }
 
-// trial image I_t = max(I + s d, minval), the change delta = I_t - I, and
-// out[0] = sum delta, out[1] = S(I_t) - S(I), out[2] = number of pixels where the
-// floor replaced I + s d; with th set, also the BF16 hi / lo split of delta for the
-// forward DFT that follows
+// trial image I_t = max(I + s d, minval), the change delta = I_t - I, and per
+// block b: cs_part[3b] = sum delta, cs_part[3b + 1] = S(I_t) - S(I), cs_part[3b + 2]
+// = number of pixels where the floor replaced I + s d (plain stores, no ticket:
+// k_commit_dchi2 and the host add them up in block order); with th set, also the
+// BF16 hi / lo split of delta for the forward DFT that follows
 __global__ __launch_bounds__(RED_THREADS) void k_commit_step(const float *__restrict__ img,
                                                              const float *__restrict__ d,
                                                              const float *__restrict__ model, float s_arg,
                                                              const double *__restrict__ s_dev, float minval,
                                                              int n, float *__restrict__ trial,
-                                                             float *__restrict__ delta, double *part,
-                                                             unsigned *ticket, double *out,
+                                                             float *__restrict__ delta, double *__restrict__ cs_part,
                                                              unsigned short *__restrict__ th,
                                                              unsigned short *__restrict__ tl)
 {
+    __shared__ double sh[3][RED_THREADS / 64];
     const float s = s_dev ? (float)*s_dev : s_arg;
     double v[3] = {0.0, 0.0, 0.0};
     for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) {
@@ -1167,7 +1168,12 @@ __global__ __launch_bounds__(RED_THREADS) void k_commit_step(const float *__rest
         v[0] += (double)dl;
         v[1] += pix_entropy((double)t, M) - pix_entropy((double)I, M);
     }
-    reduce_finish<3>(v, part, ticket, out, nullptr);
+    block_sum<3>(v, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            cs_part[3 * blockIdx.x + j] = v[j];
+    }
 }
 
 // visibilities after the committed change: (F V + U_delta) / (F + F_delta)
@@ -1180,21 +1186,25 @@ __device__ __forceinline__ double2 vis_after(const double2 *__restrict__ Vd, con
 }
 
 // exact chi2 change between Vd and the visibilities after the committed change,
-// mock data of the latter; out[0] = sum r(Vn)^2 - r(Vd)^2.  The same launch writes
-// Vn (FP64) and vis_f (float) for every uv point; the T3 terms rebuild their legs
-// with the same expression.
+// mock data of the latter; dc_part[b] = block b's sum of r(Vn)^2 - r(Vd)^2 (plain
+// stores, added up by the host in block order).  The change's flux is the sum of
+// k_commit_step's ncs partials, taken in the same order by every block.  The same
+// launch writes Vn (FP64) and vis_f (float) for every uv point; the T3 terms
+// rebuild their legs with the same expression.
 __global__ __launch_bounds__(RED_THREADS) void k_commit_dchi2(const double2 *__restrict__ Vd,
                                                               const float2 *__restrict__ Udelta,
-                                                              const double *__restrict__ scal, int nuv,
+                                                              const double *__restrict__ scal,
+                                                              const double *__restrict__ cs_part, int ncs, int nuv,
                                                               double2 *__restrict__ Vn, float2 *__restrict__ vis_f,
                                                               const float *__restrict__ data,
                                                               const float *__restrict__ err,
                                                               const float2 *__restrict__ phasor,
                                                               const int4 *__restrict__ tri, int npow, int nbis,
-                                                              float *__restrict__ mock_new, double *part,
-                                                              unsigned *ticket, double *out)
+                                                              float *__restrict__ mock_new,
+                                                              double *__restrict__ dc_part)
 {
-    const double FI = scal[SC_FLUX], iv = 1.0 / (FI + scal[SC_FDELTA]);
+    __shared__ double sh[1][RED_THREADS / 64];
+    const double FI = scal[SC_FLUX], iv = 1.0 / (FI + block_fixed_sum(cs_part, 3, ncs));
     for (int p = blockIdx.x * RED_THREADS + threadIdx.x; p < nuv; p += gridDim.x * RED_THREADS) {
         const double2 r = vis_after(Vd, Udelta, FI, iv, p);
         Vn[p] = r;
@@ -1229,7 +1239,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_commit_dchi2(const double2 *__r
             v[0] += (b0 * b0 + b1 * b1) - (a0 * a0 + a1 * a1);
         }
     }
-    reduce_finish<1>(v, part, ticket, out, nullptr);
+    block_sum<1>(v, sh);
+    if (threadIdx.x == 0)
+        dc_part[blockIdx.x] = v[0];
 }
 
 // ---------------------------------------------------------------------------
