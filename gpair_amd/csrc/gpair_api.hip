@@ -835,16 +835,7 @@ static int launch_adjoint(gpair_ctx *c);
 static int launch_adjoint_dft(gpair_ctx *c, int *nsplit_out);
 
 // per-uv adjoint coefficients from the visibilities and mock data of the last chi2
-static int launch_coef(gpair_ctx *c)
-{
-    k_coef<<<tuned_blocks(c->coef_blocks, red_nparts(c->nuv, COEF_SPAN)), RED_THREADS, 0, c->stream>>>(
-        c->vis, c->mock, c->data, c->err, c->phasor, c->tri, c->csr_ptr, c->csr_ent, c->npow, c->nuv, c->coef,
-        c->part, c->ticket, c->scal);
-    LAUNCHCHK(c);
-    return GPAIR_OK;
-}
-
-// launch_flux(img) and launch_coef in one launch (the same results)
+// (k_coef) and the flux of img (as launch_flux) in one launch, with the same results
 static int launch_coef_flux(gpair_ctx *c, const float *img)
 {
     const int nbc = tuned_blocks(c->coef_blocks, red_nparts(c->nuv, COEF_SPAN)), nbf = red_blocks(c->n);
