@@ -89,9 +89,10 @@ struct gpair_ctx {
     int snap_every;
     float snap_pix;
     // optional per-launch timing of the DFT kernels (HIP events on the context stream)
-    int timing;
+    int timing;                          // 0: off, N: events around every N-th launch of each kind
     std::vector<hipEvent_t> ev[GPAIR_TK_COUNT];
     size_t ev_used[GPAIR_TK_COUNT];
+    unsigned tk_seen[GPAIR_TK_COUNT];    // launches of each kind since timing was enabled
     char msg[512];
 };
 
@@ -666,7 +667,8 @@ static hipEvent_t timing_event(gpair_ctx *c, int k)
 #define TIMED_LAUNCH(c, k, launch)                                                   \
     do {                                                                             \
         hipEvent_t a_ = nullptr, b_ = nullptr;                                       \
-        if ((c)->timing && (c)->ev_used[k] + 2 <= GPAIR_TIMING_MAX_EVENTS) {         \
+        if ((c)->timing && (c)->tk_seen[k]++ % (unsigned)(c)->timing == 0 &&         \
+            (c)->ev_used[k] + 2 <= GPAIR_TIMING_MAX_EVENTS) {                         \
             a_ = timing_event(c, k);                                                 \
             b_ = a_ ? timing_event(c, k) : nullptr;                                  \
             if (a_ && !b_)                                                           \
@@ -683,9 +685,11 @@ extern "C" int gpair_timing_enable(gpair_ctx *c, int on)
 {
     if (!c)
         return GPAIR_E_ARG;
-    c->timing = on ? 1 : 0;
-    for (int k = 0; k < GPAIR_TK_COUNT; k++)
+    c->timing = on > 0 ? on : 0;
+    for (int k = 0; k < GPAIR_TK_COUNT; k++) {
         c->ev_used[k] = 0;
+        c->tk_seen[k] = 0;
+    }
     return GPAIR_OK;
 }
 

@@ -185,8 +185,10 @@ enum {
     GPAIR_TK_PX_DCHI2 = 3,   /* per-move chi2 changes (gpair_pixel_propose) */
     GPAIR_TK_COUNT = 4
 };
-/* start (on != 0) or stop recording HIP events around every launch of the
- * kernels above on the context stream; enabling clears what was recorded */
+/* start (on > 0) or stop (on <= 0) recording HIP events around the launches of
+ * the kernels above on the context stream: every launch for on == 1, every on-th
+ * launch of each kernel otherwise (each recorded pair adds work to the stream);
+ * enabling clears what was recorded */
 int gpair_timing_enable(gpair_ctx *ctx, int on);
 /* number of timed launches of a kernel and their summed duration (ms) */
 int gpair_timing_read(gpair_ctx *ctx, int kernel, int *count, double *total_ms);
