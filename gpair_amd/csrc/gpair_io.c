@@ -13,6 +13,7 @@
 #include "../../include/gpair_hip.h"
 #include "gpair_io_internal.h"
 
+#include <limits.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -137,11 +138,13 @@ static int card_double(const fcard *c, double *out)
     return end == tmp ? -1 : 0;
 }
 
+/* an integer keyword; values outside +-2^53 (or NaN) are refused, so the
+ * conversion to long is always defined */
 static int hdu_long(const fhdu *h, const char *key, long *out)
 {
     const fcard *c = hdu_card(h, key);
     double v;
-    if (!c || card_double(c, &v))
+    if (!c || card_double(c, &v) || !(v >= -9007199254740992.0 && v <= 9007199254740992.0))
         return -1;
     *out = (long)v;
     return 0;
@@ -191,6 +194,8 @@ static int parse_columns(fhdu *h)
         long rep = 1;
         if (*p >= '0' && *p <= '9')
             rep = strtol(p, (char **)&p, 10);
+        if (rep < 0 || rep > (1L << 30))
+            return io_fail(GPAIR_E_ARG, "column %d: repeat count in TFORM '%s' out of range", i + 1, form);
         c->code = *p;
         c->repeat = rep;
         if (tform_size(c->code, rep, &c->size))
@@ -209,6 +214,12 @@ static int parse_columns(fhdu *h)
     }
     if (off != h->naxis1)
         return io_fail(GPAIR_E_ARG, "binary table row of %ld bytes, columns take %ld", h->naxis1, off);
+    long rows_bytes;
+    if (h->naxis1 == 0 && h->naxis2 > 0)
+        return io_fail(GPAIR_E_ARG, "binary table of %ld rows of zero bytes", h->naxis2);
+    if (__builtin_mul_overflow(h->naxis1, h->naxis2, &rows_bytes) || rows_bytes > h->datalen)
+        return io_fail(GPAIR_E_ARG, "binary table of %ld rows of %ld bytes does not fit its %ld data bytes",
+                       h->naxis2, h->naxis1, h->datalen);
     return GPAIR_OK;
 }
 
@@ -289,7 +300,14 @@ static int ff_open(const char *path, ffile *f)
         }
         hdu_long(h, "PCOUNT", &pcount);
         hdu_long(h, "GCOUNT", &gcount);
+        if ((bitpix != 8 && bitpix != 16 && bitpix != 32 && bitpix != 64 && bitpix != -32 && bitpix != -64) ||
+            naxis < 0 || naxis > 999 || pcount < 0 || gcount < 1) {
+            ff_close(f);
+            return io_fail(GPAIR_E_ARG, "%s: HDU %d with BITPIX %ld, NAXIS %ld, PCOUNT %ld, GCOUNT %ld", path,
+                           f->nhdu, bitpix, naxis, pcount, gcount);
+        }
         long n = naxis > 0 ? 1 : 0;
+        int ovf = 0;
         for (int a = 1; a <= naxis; a++) {
             char key[16];
             long na;
@@ -298,15 +316,20 @@ static int ff_open(const char *path, ffile *f)
                 ff_close(f);
                 return io_fail(GPAIR_E_ARG, "%s: HDU %d without %s", path, f->nhdu, key);
             }
-            n *= na;
+            ovf |= __builtin_mul_overflow(n, na, &n);
             if (a == 1)
                 h->naxis1 = na;
             if (a == 2)
                 h->naxis2 = na;
         }
         h->data = pos;
-        h->datalen = (labs(bitpix) / 8) * gcount * (pcount + n);
-        if (h->data + h->datalen > len) {
+        /* data bytes = |BITPIX|/8 * GCOUNT * (PCOUNT + NAXIS1 * ... * NAXISn), every
+         * step checked: the readers below index the buffer with these sizes */
+        long words;
+        ovf |= __builtin_add_overflow(pcount, n, &words);
+        ovf |= __builtin_mul_overflow(words, gcount, &words);
+        ovf |= __builtin_mul_overflow(words, labs(bitpix) / 8, &h->datalen);
+        if (ovf || h->datalen > len - h->data) {
             ff_close(f);
             return io_fail(GPAIR_E_ARG, "%s: HDU %d data runs past the end of the file", path, f->nhdu);
         }
@@ -347,6 +370,8 @@ static uint64_t be_bits(const unsigned char *p, int n)
 /* element el of a numeric / logical cell as double */
 static double cell_double(const ffile *f, const fhdu *h, const fcol *c, long row, long el)
 {
+    if (row < 0 || row >= h->naxis2 || el < 0 || el >= c->repeat)
+        return NAN;   /* outside the table: parse_columns bounds rows x row bytes by the data */
     const unsigned char *p = f->buf + h->data + row * h->naxis1 + c->offset + el * c->size;
     double v;
     switch (c->code) {
@@ -376,8 +401,12 @@ static double cell_double(const ffile *f, const fhdu *h, const fcol *c, long row
 
 static void cell_string(const ffile *f, const fhdu *h, const fcol *c, long row, char *out, size_t cap)
 {
-    const unsigned char *p = f->buf + h->data + row * h->naxis1 + c->offset;
     size_t n = 0;
+    if (row < 0 || row >= h->naxis2) {
+        out[0] = '\0';
+        return;
+    }
+    const unsigned char *p = f->buf + h->data + row * h->naxis1 + c->offset;
     while (n < (size_t)c->repeat && n + 1 < cap && p[n])
         out[n] = (char)p[n], n++;
     out[n] = '\0';
@@ -616,7 +645,7 @@ static int read_targets(const ffile *f, int *ids, char (*names)[17], int max, in
         const fcol *cid = find_col(h, "TARGET_ID"), *cname = find_col(h, "TARGET");
         if (!cid || !cname)
             return io_fail(GPAIR_E_ARG, "OI_TARGET without TARGET_ID / TARGET");
-        *count = (int)h->naxis2;
+        *count = h->naxis2 > INT_MAX ? INT_MAX : (int)h->naxis2;
         for (long r = 0; r < h->naxis2 && r < max; r++) {
             if (ids)
                 ids[r] = (int)cell_double(f, h, cid, r, 0);
@@ -1192,13 +1221,18 @@ int gpair_fits_read_image(const char *path, float *out, long capacity, int *widt
         ff_close(&f);
         return io_fail(GPAIR_E_ARG, "%s: the primary HDU holds no 2-D image", path);
     }
+    const long n = h->naxis1 * h->naxis2;   /* no overflow: bounded by the data size (ff_open) */
+    if (h->naxis1 > INT_MAX || h->naxis2 > INT_MAX || n > h->datalen / (labs(bitpix) / 8)) {
+        const long n1 = h->naxis1, n2 = h->naxis2;
+        ff_close(&f);
+        return io_fail(GPAIR_E_ARG, "%s: %ld x %ld pixels do not fit the primary HDU's data", path, n1, n2);
+    }
     *width = (int)h->naxis1;
     *height = (int)h->naxis2;
-    const long n = h->naxis1 * h->naxis2;
     if (out) {
         if (capacity < n) {
             ff_close(&f);
-            return io_fail(GPAIR_E_ARG, "%s: %ld x %ld pixels, room for %ld", path, h->naxis1, h->naxis2, capacity);
+            return io_fail(GPAIR_E_ARG, "%s: %d x %d pixels, room for %ld", path, *width, *height, capacity);
         }
         double bscale = 1.0, bzero = 0.0;
         const fcard *c;
@@ -1400,6 +1434,8 @@ int gpair_fitsr_get(gpair_fitsr *r, const char *extname, int bitpix, void *out, 
 {
     if (!r || !extname || !out)
         return io_fail(GPAIR_E_ARG, "gpair_fitsr_get: NULL argument");
+    if (bitpix != -32 && bitpix != -64)
+        return io_fail(GPAIR_E_ARG, "gpair_fitsr_get: BITPIX %d (-32 or -64 only)", bitpix);
     const fhdu *h = named_hdu(&r->f, extname);
     long bp = 0;
     if (!h)
@@ -1409,6 +1445,8 @@ int gpair_fitsr_get(gpair_fitsr *r, const char *extname, int bitpix, void *out, 
         return io_fail(GPAIR_E_ARG, "%s: BITPIX %ld, %ld x %ld (expected %d, %ld x %ld)", extname, bp, h->naxis1,
                        h->naxis2, bitpix, n1, n2);
     const int nb = bitpix == -64 ? 8 : 4;
+    if (n1 < 0 || n2 < 0 || (n2 > 0 && n1 > h->datalen / nb / n2))
+        return io_fail(GPAIR_E_ARG, "%s: %ld x %ld values do not fit its data", extname, n1, n2);
     const unsigned char *p = r->f.buf + h->data;
     for (long i = 0; i < n1 * n2; i++, p += nb) {
         const uint64_t bits = be_bits(p, nb);
