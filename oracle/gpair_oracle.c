@@ -356,44 +356,180 @@ static void step_image(int n, const double *cur, const double *d, double s, doub
     }
 }
 
-/* gpair1/trunk/gpu.c:1947-2065 */
-static double zoom(orc_prob *P, const double *cur, const double *d, double *tmp, double *gt, double lo,
-                   double hi, double J_lo, double w1, double J_init, double c1, double c2, double minv,
-                   int *evals)
+/* ------------------------------------------------------------------------
+ * The reference's line search on the 1-D objective phi(s) = J(max(I + s d, floor))
+ * with slope w1 = d.g at s = 0: the bracket phase of gpair1/trunk/gpair.c:791-906
+ * and the GPU path's zoom, gpair1/trunk/gpu.c:1947-2065, with their constants
+ * (c1 = 1e-4, c2 = 0.1, growth x10, s_max = 100, interval width 1e-14) and
+ * quirks: the zoom's curvature test is the weak one (w2 >= c2 w1, gpu.c:2045)
+ * where the bracket's is the strong one (|w2| <= -c2 w1, gpair.c:865); the
+ * zoom never updates phi(lo) (gpu.c:2020 compares with the J_lo it was given);
+ * its step is the minimiser of the quadratic through (0, J0) with slope w1 and
+ * (s_old, J_old) (gpu.c:1976-1978) when that lies inside [lo, hi], else the
+ * midpoint; and it accepts after 10 interpolations whatever the curvature
+ * (counter > 10, gpu.c:2045).  Scalars are FP64 (the reference's are float).
+ *
+ * phi(user, s) evaluates J at step s; dphi(user, s) the slope d.grad J at the
+ * image of the last phi call (the reference's gradient uses the visibilities of
+ * its last chi2 call, gpu.c:1598-1675), and is only called right after phi(s).
+ *
+ * info (may be NULL), 8 doubles about the returned step:
+ *   [0] exit: ORC_LS_BRACKET (strong Wolfe in the bracket), ORC_LS_ZOOM_CURV
+ *       (Armijo and weak curvature in the zoom), ORC_LS_ZOOM_COUNT (Armijo,
+ *       counter > 10), ORC_LS_ZOOM_WIDTH (|hi - lo| < 1e-14: returns lo),
+ *       ORC_LS_CAP (evaluation cap: returns the bracket's s_old or the zoom's lo)
+ *   [1] the returned step, [2] J there (the last evaluation for the first three
+ *   exits, the J the search holds for lo / s_old otherwise), [3] its slope w2
+ *   (NaN when not evaluated), [4] J evaluations, [5] slope evaluations,
+ *   [6] the zoom's counter (-1 without zoom), [7] the zoom's last |hi - lo|.
+ *
+ * Planted mistakes (orc_set_plant, for the tests that show the closed-form
+ * cases catch them; 0 in every other use): bit 0 the zoom takes the strong
+ * curvature test, bit 1 the zoom updates J_lo when lo moves (the textbook
+ * rule), bit 2 the zoom bisects only.
+ * ---------------------------------------------------------------------- */
+enum { ORC_LS_BRACKET = 1, ORC_LS_ZOOM_CURV = 2, ORC_LS_ZOOM_COUNT = 3, ORC_LS_ZOOM_WIDTH = 4, ORC_LS_CAP = 5 };
+typedef double (*orc_phi_fn)(void *user, double s);
+
+static int orc_plant = 0;
+void orc_set_plant(int mask) { orc_plant = mask; }
+
+#define LS_C1 1e-4
+#define LS_C2 0.1
+#define LS_SMAX 100.0
+
+static void ls_info(double *info, int exitc, double s, double J, double w2, int nJ, int nG, int counter,
+                    double width)
 {
-    int n = P->W * P->W;
-    double s = 0.0, s_old = 0.0, J_old = 0.0, chi2, S;
+    if (!info)
+        return;
+    info[0] = exitc; info[1] = s; info[2] = J; info[3] = w2;
+    info[4] = nJ; info[5] = nG; info[6] = counter; info[7] = width;
+}
+
+/* gpair1/trunk/gpu.c:1947-2065; *nJ / *nG count on from their values (the
+ * bracket's), cap bounds *nJ */
+static double ls_zoom(orc_phi_fn phi, orc_phi_fn dphi, void *user, double lo, double hi, double J_lo,
+                      double w1, double J_init, int cap, int *nJ, int *nG, double *info)
+{
+    double s = 0.0, s_old = 0.0, J_old = 0.0, J_at_lo = J_lo;
     int counter = 0;
     for (;;) {
-        if (counter > 0 && (J_old - J_init - w1 * s_old) != 0.0)
+        if (!(orc_plant & 4) && counter > 0 && (J_old - J_init - w1 * s_old) != 0.0)
             s = fabs(w1 * s_old * s_old / (2.0 * (J_old - J_init - w1 * s_old)));
-        if (counter == 0 || s < lo || s > hi)
+        if (counter == 0 || (orc_plant & 4) || s < lo || s > hi)
             s = (hi - lo) / 2.0 + lo;
-        if (fabs(hi - lo) < 1e-14)
+        if (fabs(hi - lo) < 1e-14) {
+            ls_info(info, ORC_LS_ZOOM_WIDTH, lo, J_at_lo, NAN, *nJ, *nG, counter, fabs(hi - lo));
             return lo;
-        if (*evals >= orc_ls_cap)
+        }
+        if (*nJ >= cap) {
+            ls_info(info, ORC_LS_CAP, lo, J_at_lo, NAN, *nJ, *nG, counter, fabs(hi - lo));
             return lo;
-        step_image(n, cur, d, s, minv, tmp);
-        double J = eval_J(P, tmp, &chi2, &S);
-        (*evals)++;
+        }
+        const double J = phi(user, s);
+        (*nJ)++;
         if (orc_verbose)
             fprintf(stderr, "zoom c=%d s=%.6e lo=%.6e hi=%.6e J=%.10e Jlo=%.10e Jinit=%.10e w1=%.4e\n", counter, s,
                     lo, hi, J, J_lo, J_init, w1);
-        if (J > J_init + c1 * s * w1 || J >= J_lo) {
+        if (J > J_init + LS_C1 * s * w1 || J >= J_lo) {
             hi = s;
         } else {
-            eval_grad(P, tmp, gt);
-            double w2 = dot(n, d, gt);
-            if (w2 >= c2 * w1 || counter > 10)
+            const double w2 = dphi(user, s);
+            (*nG)++;
+            const int curv = (orc_plant & 1) ? fabs(w2) <= -LS_C2 * w1 : w2 >= LS_C2 * w1;
+            if (curv || counter > 10) {
+                ls_info(info, curv ? ORC_LS_ZOOM_CURV : ORC_LS_ZOOM_COUNT, s, J, w2, *nJ, *nG, counter,
+                        fabs(hi - lo));
                 return s;
+            }
             if (w2 * (hi - lo) >= 0.0)
                 hi = lo;
             lo = s;
+            J_at_lo = J;
+            if (orc_plant & 2)
+                J_lo = J;
         }
         s_old = s;
         J_old = J;
         counter++;
     }
+}
+
+/* the zoom alone, as gpu_linesearch_zoom is called (tests) */
+double orc_ls_zoom(orc_phi_fn phi, orc_phi_fn dphi, void *user, double lo, double hi, double J_lo, double w1,
+                   double J_init, int cap, double *info)
+{
+    int nJ = 0, nG = 0;
+    return ls_zoom(phi, dphi, user, lo, hi, J_lo, w1, J_init, cap, &nJ, &nG, info);
+}
+
+/* bracket phase, gpair1/trunk/gpair.c:791-906, with the zoom above; J_init is
+ * phi(0), w1 its slope */
+double orc_linesearch(orc_phi_fn phi, orc_phi_fn dphi, void *user, double J_init, double w1, int cap,
+                      double *info)
+{
+    double s = 1.0, s_old = 0.0, J_old = J_init;
+    int lsit = 1, nJ = 0, nG = 0;
+    for (;;) {
+        if (nJ >= cap) {
+            ls_info(info, ORC_LS_CAP, s_old, J_old, NAN, nJ, nG, -1, 0.0);
+            return s_old;
+        }
+        const double Jt = phi(user, s);
+        nJ++;
+        if (orc_verbose)
+            fprintf(stderr, "bracket s=%.6e J=%.10e Jold=%.10e w1=%.4e\n", s, Jt, J_old, w1);
+        if (Jt > J_init + LS_C1 * s * w1 || (Jt >= J_old && lsit > 1))
+            return ls_zoom(phi, dphi, user, s_old, s, J_old, w1, J_init, cap, &nJ, &nG, info);
+        const double w2 = dphi(user, s);
+        nG++;
+        if (fabs(w2) <= -LS_C2 * w1) {
+            ls_info(info, ORC_LS_BRACKET, s, Jt, w2, nJ, nG, -1, 0.0);
+            return s;
+        }
+        if (w2 >= 0.0)
+            return ls_zoom(phi, dphi, user, s, s_old, Jt, w1, J_init, cap, &nJ, &nG, info);
+        s_old = s;
+        J_old = Jt;
+        s = 10.0 * s;
+        if (s > LS_SMAX)
+            s = LS_SMAX;
+        lsit++;
+    }
+}
+
+/* phi and its slope for the MEM-CG objective along d from cur */
+typedef struct {
+    orc_prob *P;
+    const double *cur, *d;
+    double *tmp, *gt, minv;
+} cg_line;
+
+static double cg_phi(void *u, double s)
+{
+    cg_line *L = u;
+    double chi2, S;
+    step_image(L->P->W * L->P->W, L->cur, L->d, s, L->minv, L->tmp);
+    return eval_J(L->P, L->tmp, &chi2, &S);
+}
+
+static double cg_dphi(void *u, double s)
+{
+    cg_line *L = u;
+    (void)s;   /* the image of the last cg_phi call */
+    eval_grad(L->P, L->tmp, L->gt);
+    return dot(L->P->W * L->P->W, L->d, L->gt);
+}
+
+/* optional per-iteration log of orc_cg's line searches: info rows of 8 doubles
+ * (above) followed by w1 and J0 of the search (10 doubles per row) */
+static double *orc_ls_log = NULL;
+static int orc_ls_log_rows = 0;
+void orc_set_ls_log(double *buf, int rows)
+{
+    orc_ls_log = buf;
+    orc_ls_log_rows = buf ? rows : 0;
 }
 
 /* Runs at most max_iter outer iterations of MEM-CG on img (in/out).
@@ -405,7 +541,7 @@ int orc_cg(int W, int nuv, int npow, int nbis, const double *au, const double *a
            const double *err, const int *uvpnt, const int *sign, const double *phasor, const double *model,
            double alpha, int method, int max_iter, double chi2r_target, double *img, double *trace)
 {
-    const double c1 = 1e-4, c2 = 0.1, minv = 1e-8, smax = 100.0;
+    const double minv = 1e-8;
     int n = W * W, ndof = npow + 2 * nbis;
     orc_prob P = {W, nuv, npow, nbis, au, av, data, err, phasor, model, uvpnt, sign, alpha,
                   NULL, NULL, NULL, NULL, NULL, 0, 0};
@@ -451,38 +587,14 @@ int orc_cg(int W, int nuv, int npow, int nbis, const double *au, const double *a
         for (int i = 0; i < n; i++)
             d[i] = beta * d[i] - g[i];
         double w1 = dot(n, d, g);
-        /* bracket phase, gpair1/trunk/gpair.c:791-906 */
-        double s = 1.0, s_old = 0.0, J_init = J, J_old = J;
-        int lsit = 1, evals = 0;
-        for (;;) {
-            if (evals >= orc_ls_cap) {
-                sel = s_old;
-                break;
-            }
-            step_image(n, img, d, s, minv, tmp);
-            double Jt = eval_J(&P, tmp, &chi2, &S);
-            evals++;
-            if (orc_verbose)
-                fprintf(stderr, "it=%d bracket s=%.6e J=%.10e Jold=%.10e w1=%.4e\n", it, s, Jt, J_old, w1);
-            if (Jt > J_init + c1 * s * w1 || (Jt >= J_old && lsit > 1)) {
-                sel = zoom(&P, img, d, tmp, gt, s_old, s, J_old, w1, J_init, c1, c2, minv, &evals);
-                break;
-            }
-            eval_grad(&P, tmp, gt);
-            double w2 = dot(n, d, gt);
-            if (fabs(w2) <= -c2 * w1) {
-                sel = s;
-                break;
-            }
-            if (w2 >= 0.0) {
-                sel = zoom(&P, img, d, tmp, gt, s, s_old, Jt, w1, J_init, c1, c2, minv, &evals);
-                break;
-            }
-            s_old = s;
-            J_old = Jt;
-            s = 10.0 * s;
-            if (s > smax) s = smax;
-            lsit++;
+        /* bracket phase and zoom, gpair1/trunk/gpair.c:791-906, gpu.c:1947-2065 */
+        cg_line L = {&P, img, d, tmp, gt, minv};
+        double info[8];
+        sel = orc_linesearch(cg_phi, cg_dphi, &L, J, w1, orc_ls_cap, info);
+        if (orc_ls_log && it < orc_ls_log_rows) {
+            memcpy(orc_ls_log + 10 * it, info, sizeof(info));
+            orc_ls_log[10 * it + 8] = w1;
+            orc_ls_log[10 * it + 9] = J;
         }
         step_image(n, img, d, sel, minv, img);
         memcpy(g_old, g, sizeof(double) * (size_t)n);
@@ -634,6 +746,7 @@ int orc_pcg(int W, int nuv, int npow, int nbis, const double *au, const double *
         }
         double Jk[LS_K], sk[LS_K], w1 = 0.0;
         int kbest = -1;
+        const double s_ref_it = s_ref;   /* a restart along -z searches from here again */
         /* attempt 0: conjugate direction; attempt 1 (only when attempt 0 used
          * beta > 0 and found no decrease): restart along -z */
         for (int attempt = 0; attempt < 2 && kbest < 0; attempt++) {
@@ -644,6 +757,8 @@ int orc_pcg(int W, int nuv, int npow, int nbis, const double *au, const double *
             }
             if (attempt == 1 && beta == 0.0)
                 break;
+            if (attempt == 1)
+                s_ref = s_ref_it;   /* not the 2^-64 the failed rounds shrank it to */
             beta = b0;
             /* pixels on the floor do not move further down: the direction is
              * projected onto the feasible side there, so d.g is the slope of

@@ -22,6 +22,12 @@ REF_SO = os.path.join(HERE, "_ref", "libgpair_refcpu.so")
 REF_SRC = "/root/reference/gpair1/trunk/cpu.c"
 
 _dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+# phi(user, s) callbacks of the line search (orc_linesearch / orc_ls_zoom)
+PHI = C.CFUNCTYPE(C.c_double, C.c_void_p, C.c_double)
+
+# exit codes of the reference line search (info[0], gpair_oracle.c)
+LS_BRACKET, LS_ZOOM_CURV, LS_ZOOM_COUNT, LS_ZOOM_WIDTH, LS_CAP = 1, 2, 3, 4, 5
+LS_INFO = ("exit", "s", "J", "w2", "nJ", "nG", "counter", "width")
 _ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 
 
@@ -79,6 +85,13 @@ class Oracle:
                                                      C.c_double, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp]
         L.orc_parabola_step.restype = C.c_double
         L.orc_num_threads.restype = C.c_int
+        L.orc_linesearch.argtypes = [PHI, PHI, C.c_void_p, C.c_double, C.c_double, C.c_int, _dp]
+        L.orc_linesearch.restype = C.c_double
+        L.orc_ls_zoom.argtypes = [PHI, PHI, C.c_void_p] + [C.c_double] * 5 + [C.c_int, _dp]
+        L.orc_ls_zoom.restype = C.c_double
+        L.orc_set_plant.argtypes = [C.c_int]
+        L.orc_set_ls_log.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_ls_cap.argtypes = [C.c_int]
 
     # -- problem helpers ----------------------------------------------------
     @staticmethod
@@ -161,13 +174,54 @@ class Oracle:
         ge = self.entropy_grad(img, P["model"])
         return gd - alpha * ge, gd, ge
 
-    def cg(self, P, img, alpha, method=2, max_iter=200, chi2r_target=1.0):
+    def cg(self, P, img, alpha, method=2, max_iter=200, chi2r_target=1.0, ls_log=False):
+        """reference-rule loop; with ls_log also the line-search record of every
+        iteration: rows of (exit, s, J(s), w2, nJ, nG, counter, width, w1, J0)"""
         img = _c64(img).copy()
         trace = np.zeros((max_iter, 8))
-        rows = self.L.orc_cg(P["W"], P["nuv"], P["npow"], P["nbis"], P["au"], P["av"], P["data"], P["err"],
-                             P["uvpnt"], P["sign"], P["phasor"], P["model"], float(alpha), int(method),
-                             int(max_iter), float(chi2r_target), img, trace.ctypes.data)
+        log = np.full((max_iter, 10), np.nan) if ls_log else None
+        if ls_log:
+            self.L.orc_set_ls_log(log.ctypes.data, max_iter)
+        try:
+            rows = self.L.orc_cg(P["W"], P["nuv"], P["npow"], P["nbis"], P["au"], P["av"], P["data"], P["err"],
+                                 P["uvpnt"], P["sign"], P["phasor"], P["model"], float(alpha), int(method),
+                                 int(max_iter), float(chi2r_target), img, trace.ctypes.data)
+        finally:
+            if ls_log:
+                self.L.orc_set_ls_log(None, 0)
+        if ls_log:
+            return img, trace[:rows], log[:rows]
         return img, trace[:rows]
+
+    # -- the reference line search on a 1-D objective ------------------------
+    def _ls(self, fn, phi, dphi, *args, cap=64):
+        calls = []
+
+        def _phi(_u, s):
+            calls.append(("J", s))
+            return float(phi(s))
+
+        def _dphi(_u, s):
+            calls.append(("g", s))
+            return float(dphi(s))
+        info = np.zeros(8)
+        f1, f2 = PHI(_phi), PHI(_dphi)   # kept alive for the call
+        s = fn(f1, f2, None, *[float(a) for a in args], int(cap), info)
+        return s, dict(zip(LS_INFO, info)), calls
+
+    def linesearch(self, phi, dphi, J0, w1, cap=64):
+        """bracket + zoom (gpair.c:791-906, gpu.c:1947-2065) on phi(s); returns
+        the step, the info dict and the list of evaluations ("J" | "g", s)"""
+        return self._ls(self.L.orc_linesearch, phi, dphi, J0, w1, cap=cap)
+
+    def zoom(self, phi, dphi, lo, hi, J_lo, w1, J0, cap=64):
+        """the zoom alone (gpu_linesearch_zoom's arguments)"""
+        return self._ls(self.L.orc_ls_zoom, phi, dphi, lo, hi, J_lo, w1, J0, cap=cap)
+
+    def plant(self, mask):
+        """planted mistakes in the zoom (tests only): 1 strong curvature test,
+        2 J_lo updated when lo moves, 4 bisection only; 0 restores the rules"""
+        self.L.orc_set_plant(int(mask))
 
     def pcg(self, P, img, alpha, precond=1, pfactor=None, max_iter=200, chi2r_target=1.0):
         """accelerated loop (preconditioned PR-CG, batched line search)."""
