@@ -87,3 +87,40 @@ def test_snapshots_of_the_loops(tmp_path):
         ctx.set_snapshot(None)
     img = oifits.read_image(snap2)
     assert img.shape == (prob.W, prob.W) and np.all(np.isfinite(img)) and img.min() > 0
+
+
+@pytest.mark.parametrize("saved_factor,context_factor", [(True, False), (False, True)],
+                         ids=["ck-factor", "context-factor"])
+def test_checkpoint_with_factor_and_graphs(tmp_path, saved_factor, context_factor):
+    """the gradient-phase graphs hold the preconditioner factor's pointer: a resume
+    that restores a factor (or drops the context's) captures them after the
+    checkpoint is loaded, so the resumed graph run equals the uninterrupted run
+    without graphs bit for bit"""
+    prob = synth.make_problem("cfg1", seed=4, W=128, ntel=7, nch=6, nsnap=5)
+    alpha = prob.meta["alpha"]
+    rng = np.random.default_rng(3)
+    pf = (0.5 + rng.random(prob.W ** 2)).astype(np.float32)
+    other = (0.5 + rng.random(prob.W ** 2)).astype(np.float32)
+    N, M = 15, 20
+    ck = str(tmp_path / "ck.fits")
+    with GpairContext.from_problem(prob) as a:          # uninterrupted, launched
+        a.set_precision(1)
+        if saved_factor:
+            a.set_pfactor(pf)
+        ra, tra = a.pcg_run(alpha, max_iter=N + M, chi2r_target=0.0)
+        img_a = a.get("ci")
+    with GpairContext.from_problem(prob) as b:          # first part, graphs
+        b.set_precision(1)
+        b.set_graphs(True)
+        if saved_factor:
+            b.set_pfactor(pf)
+        b.pcg_run_checkpointed(alpha, max_iter=N, chi2r_target=0.0, save_to=ck)
+    with GpairContext.from_problem(prob) as c:          # resumed, graphs, another factor state
+        c.set_precision(1)
+        c.set_graphs(True)
+        if context_factor:
+            c.set_pfactor(other)
+        rc, trc = c.pcg_run_checkpointed(alpha, max_iter=N + M, chi2r_target=0.0, resume_from=ck)
+        img_c = c.get("ci")
+    print(f"  J {ra.J!r} resumed {rc.J!r}", flush=True)
+    assert rc.J == ra.J and np.array_equal(img_c, img_a) and np.array_equal(trc[N:], tra[N:])

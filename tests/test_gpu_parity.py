@@ -1,10 +1,15 @@
 """HIP path vs the FP64 oracle through the C ABI (MI355X only).
 
-Tolerances (FP32 MFMA contraction, FP32 per-datum model, FP64 reductions):
-  visibilities   max|dV| <= 2e-6 * max|V|
-  mock data      max|dm| <= 1e-5 * max|m|
-  chi2, entropy  relative 2e-5
-  data gradient  max|dg| <= 2e-4 * max|g|   (adjoint sums 2*nuv FP32 products)
+Tolerances (FP32 MFMA contraction, FP32 per-datum model, FP64 reductions), about
+3-10x the largest error measured on the MI355X over the cases below:
+  visibilities   max|dV| <= 2e-6 * max|V|      (measured <= 3.5e-7)
+  mock data      max|dm| <= 2e-6 * max|m|      (<= 3.9e-7)
+  chi2           relative 1e-8                 (<= 1.3e-9)
+  entropy        relative 1e-12                (<= 8e-15)
+  data gradient  max|dg| <= 2e-6 * max|g|      (<= 2.8e-7; the adjoint sums 2*nuv FP32 products)
+  BF16x3 (split BF16, 3 MFMA products): V 8e-6, chi2 2e-7, gradient 4e-5 of max
+  (measured <= 2.5e-6, 3.6e-8, 1.1e-5); FP16 with scaling: V 6e-4, chi2 2e-5,
+  gradient 2.5e-3 (measured <= 2.0e-4, 3.8e-6, 6.4e-4)
 """
 import numpy as np
 import pytest
@@ -71,12 +76,12 @@ def test_objective_and_gradient(name, kw, orc):
     print(f"  vis {ev:.2e} mock {em:.2e} chi2 {ec:.2e} S {es:.2e} grad {eg:.2e} egrad {ee:.2e} full {ef:.2e}",
           flush=True)
     assert ev <= 2e-6
-    assert em <= 1e-5
-    assert ec <= 2e-5
-    assert es <= 2e-5
-    assert eg <= 2e-4
-    assert ee <= 1e-5
-    assert ef <= 2e-4
+    assert em <= 2e-6
+    assert ec <= 1e-8
+    assert es <= 1e-12
+    assert eg <= 2e-6
+    assert ee <= 1e-6
+    assert ef <= 2e-6
 
 
 def test_temp_path_and_vector_ops(orc):
@@ -158,7 +163,7 @@ def test_cg_loop_matches_oracle(orc):
     assert res.iterations == nit
     rel = np.abs(tr[:, 0] - tr_o[:, 0]) / np.abs(tr_o[:, 0])
     print("trajectory rel diff", rel)
-    assert rel.max() <= 1e-3
+    assert rel.max() <= 1e-5
     # the objective of the GPU's final image, re-evaluated in FP64
     J_o = orc.objective(P, img_gpu, ALPHA)[0]
     assert abs(J_gpu - J_o) / abs(J_o) <= 2e-5
@@ -178,7 +183,7 @@ def test_pcg_loop_matches_oracle(orc):
     assert res.iterations == nit and len(tr_o) == nit
     rel = np.abs(tr[:, 0] - tr_o[:, 0]) / np.abs(tr_o[:, 0])
     print("trajectory rel diff", rel)
-    assert rel.max() <= 1e-3
+    assert rel.max() <= 2e-4   # FP32 DFT rounding, amplified by the trajectory (measured <= 4.4e-5)
     # same DFT / batch counts (the rules took the same branches)
     assert np.array_equal(tr[:, 5:8], tr_o[:, 5:8])
 
@@ -201,7 +206,8 @@ def test_pcg_reaches_target_and_objective_is_exact(orc):
 @pytest.mark.parametrize("name,kw", _cases(), ids=[c[0] for c in _cases()])
 def test_bf16x3_forward(name, kw, orc):
     """split-precision forward DFT (3 BF16 MFMA products, FP32 accumulation)
-    against the FP64 oracle.  Bounds: |dV| <= 3e-5 max|V|, chi2 relative 2e-5."""
+    against the FP64 oracle.  Bounds: |dV| <= 8e-6 max|V|, chi2 relative 2e-7,
+    gradient 4e-5 of max|g|."""
     prob = _problem(name, kw)
     P = orc.unpack(prob)
     img = _image(prob, 7)
@@ -217,9 +223,9 @@ def test_bf16x3_forward(name, kw, orc):
     ec = abs(chi2 - chi2_o) / chi2_o
     eg = np.abs(gd - gd_o).max() / np.abs(gd_o).max()
     print(f"  bf16x3 {name}: vis {ev:.2e} chi2 {ec:.2e} grad {eg:.2e}", flush=True)
-    assert ev <= 3e-5
-    assert ec <= 2e-5
-    assert eg <= 2e-4   # widths that are not a multiple of 128 use the FP32 adjoint
+    assert ev <= 8e-6
+    assert ec <= 2e-7
+    assert eg <= 4e-5   # widths that are not a multiple of 128 use the FP32 adjoint
 
 
 def test_autotuned_shapes_keep_parity(orc):
@@ -234,7 +240,7 @@ def test_autotuned_shapes_keep_parity(orc):
         chosen = ctx.autotune(reps=3)
         print("autotune", chosen, flush=True)
         assert all(v > 0 for v in chosen)
-        for prec, vb in ((0, 2e-6), (1, 3e-5)):
+        for prec, vb in ((0, 2e-6), (1, 8e-6)):
             ctx.set_precision(prec)
             ctx.gpu_copy_image(img)
             chi2 = ctx.gpu_get_chi2_curr()
@@ -244,7 +250,7 @@ def test_autotuned_shapes_keep_parity(orc):
             ev = np.abs(vis - vis_o).max() / np.abs(vis_o).max()
             eg = np.abs(gd - gd_o).max() / np.abs(gd_o).max()
             print(f"  prec {prec}: vis {ev:.2e} chi2 {abs(chi2 - chi2_o) / chi2_o:.2e} grad {eg:.2e}", flush=True)
-            assert ev <= vb and abs(chi2 - chi2_o) / chi2_o <= 2e-5 and eg <= 2e-4
+            assert ev <= vb and abs(chi2 - chi2_o) / chi2_o <= 2e-7 and eg <= 4e-5
     prob = synth.make_problem("cfg1", seed=1)
     P = orc.unpack(prob)
     alpha = prob.meta["alpha"]
@@ -252,7 +258,7 @@ def test_autotuned_shapes_keep_parity(orc):
         ctx.autotune(reps=3)
         res, tr = ctx.pcg_run(alpha, precond=1, max_iter=12, chi2r_target=0.0)
     img_o, tr_o = orc.pcg(P, prob.start.astype(np.float64), alpha, precond=1, max_iter=12, chi2r_target=0.0)
-    assert np.abs(tr[:, 0] - tr_o[:, 0]).max() / np.abs(tr_o[:, 0]).min() <= 1e-3
+    assert np.abs(tr[:, 0] - tr_o[:, 0]).max() / np.abs(tr_o[:, 0]).min() <= 3e-4   # measured 7e-5
     assert np.array_equal(tr[:, 5:8], tr_o[:, 5:8])
 
 
@@ -358,18 +364,18 @@ def test_bf16x3_factored_tables(name, kw, orc):
         ec = abs(chi2 - chi2_o) / chi2_o
         eg = np.abs(gd - gd_o).max() / gmax
         print(f"  {tag}: vis {ev:.2e} chi2 {ec:.2e} grad {eg:.2e}", flush=True)
-        assert ev <= 3e-5 and ec <= 2e-5 and eg <= 2e-4
+        assert ev <= 8e-6 and ec <= 2e-7 and eg <= 4e-5
     (c1, v1, g1), (c2, v2, g2) = out
-    assert np.abs(v1 - v2).max() <= 3e-5 * vmax
-    assert np.abs(g1 - g2).max() <= 2e-4 * gmax
+    assert np.abs(v1 - v2).max() <= 8e-6 * vmax
+    assert np.abs(g1 - g2).max() <= 4e-5 * gmax
 
 
 @pytest.mark.parametrize("name,kw", _ft_cases(), ids=[c[0] for c in _ft_cases()])
 def test_fp16_scaled(name, kw, orc):
     """FP16 with power-of-two scaling (one FP16 MFMA product per multiply-add) against
     the FP64 oracle.  Each product carries two FP16 roundings (2^-11 each) that
-    average over the W^2 terms of a visibility: bounds |dV| <= 5e-4 max|V|, chi2
-    relative 5e-3, gradient 5e-3 of max|g| (W = 64 and 192 take the FP32 adjoint)."""
+    average over the W^2 terms of a visibility: bounds |dV| <= 6e-4 max|V|, chi2
+    relative 2e-5, gradient 2.5e-3 of max|g| (W = 64 and 192 take the FP32 adjoint)."""
     prob = synth.make_problem("cfg1", **kw)
     print(f"case {name}: W={prob.W} nuv={prob.nuv} nbis={prob.nbis}", flush=True)
     P = orc.unpack(prob)
@@ -386,7 +392,7 @@ def test_fp16_scaled(name, kw, orc):
     ec = abs(chi2 - chi2_o) / chi2_o
     eg = np.abs(gd - gd_o).max() / np.abs(gd_o).max()
     print(f"  fp16 scaled {name}: vis {ev:.2e} chi2 {ec:.2e} grad {eg:.2e}", flush=True)
-    assert ev <= 5e-4 and ec <= 5e-3 and eg <= 5e-3
+    assert ev <= 6e-4 and ec <= 2e-5 and eg <= 2.5e-3
 
 
 def test_fp16_scaled_needs_factored_tables(orc):

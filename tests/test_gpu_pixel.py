@@ -4,7 +4,7 @@ through the C ABI (MI355X only).
 
 The pixel state's visibilities come from one forward DFT (FP32, or BF16x3) of the
 image; every update after that is FP64.  Tolerances:
-  trial visibilities   max|dV| <= 2e-6 max|V| (FP32 state), 3e-5 (BF16x3 state)
+  trial visibilities   max|dV| <= 2e-6 max|V| (FP32 state), 8e-6 (BF16x3 state)
   gpu_new_chi2         relative 2e-5 (chi2 of the float trial visibilities)
   move scores dchi2    |d| <= 1e-3 |dchi2| + 1e-7 chi2 (FP64 differences from the DFT state)
   move scores dS       relative 1e-12 (FP64 from the float pixel values)
@@ -23,7 +23,7 @@ pytestmark = pytest.mark.gpu
 
 CASES = [("cfg1", dict(seed=1)), ("cfg1_w128", dict(seed=2, W=128, ntel=7, nch=6, nsnap=5)),
          ("cfg2", dict(seed=3))]
-VIS_TOL = {0: 2e-6, 1: 3e-5}
+VIS_TOL = {0: 2e-6, 1: 8e-6}
 
 
 @pytest.fixture(scope="module")

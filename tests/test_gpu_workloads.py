@@ -1,0 +1,165 @@
+"""The benchmark workloads at their full sizes, against the FP64 oracle (MI355X only).
+
+* 256^2 headline problem (36,864 V2 + 86,016 T3), BF16x3 accelerated loop:
+  the trace of 8 iterations against orc_pcg (the loop's rules in FP64), and a
+  run to chi2/ndof = 1 whose reported objective is within 1e-4 of the FP64
+  objective of its final image.
+* 512^2 precision workload (147,456 V2 + 344,064 T3): one chi2 and one data
+  gradient in FP32, BF16x3 and FP16 with scaling against the oracle; a run
+  stopped by its iteration cap reports the exact objective of its final image.
+* Batching workload: 64 problems of widths 64-256 and 500-40,000 V2 (T3 about
+  2.3 V2) through gpair_batch_pcg_run: the smallest and the largest equal their
+  single runs bit for bit, every problem's objective agrees with the FP64
+  objective of its final image.
+
+Bounds are stated per test (about 3-5x the errors measured on the MI355X).
+"""
+import numpy as np
+import pytest
+
+from gpair_amd import synth
+from gpair_amd.api import GpairContext, batch_pcg_run
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle()
+
+
+@pytest.fixture(scope="module")
+def cfg2():
+    return synth.make_problem("cfg2", seed=1)
+
+
+@pytest.fixture(scope="module")
+def cfg3():
+    return synth.make_problem("cfg3", seed=1)
+
+
+def test_bf16x3_loop_matches_oracle_at_256(orc, cfg2):
+    """the headline path (BF16x3 DFTs) for 8 iterations against the FP64 rules:
+    objective trace within 1e-5 relative, same DFT / batch counts (same branches)"""
+    prob = cfg2
+    P = orc.unpack(prob)
+    alpha = prob.meta["alpha"]
+    nit = 8
+    print(f"cfg2: W={prob.W} nuv={prob.nuv} nbis={prob.nbis} alpha={alpha}", flush=True)
+    with GpairContext.from_problem(prob) as ctx:
+        ctx.set_precision(1)
+        res, tr = ctx.pcg_run(alpha, precond=1, max_iter=nit, chi2r_target=0.0)
+    img_o, tr_o = orc.pcg(P, prob.start.astype(np.float64), alpha, precond=1, max_iter=nit, chi2r_target=0.0)
+    rel = np.abs(tr[:, 0] - tr_o[:, 0]) / np.abs(tr_o[:, 0])
+    print("gpu J", tr[:, 0], "\norc J", tr_o[:, 0], "\nrel", rel, flush=True)
+    print("steps gpu", tr[:, 3], "\nsteps orc", tr_o[:, 3], flush=True)
+    assert res.iterations == nit and len(tr_o) == nit
+    assert rel.max() <= 1e-5
+    assert np.array_equal(tr[:, 5:8], tr_o[:, 5:8])
+
+
+def test_bf16x3_reaches_target_at_256(orc, cfg2):
+    """the headline metric's run: BF16x3 from the prior to chi2/ndof <= 1; the
+    objective it reports is within 1e-4 (bound here 1e-5) of the FP64 objective of
+    the image it returns, whose chi2/ndof is 1 to the same accuracy"""
+    prob = cfg2
+    P = orc.unpack(prob)
+    alpha = prob.meta["alpha"]
+    with GpairContext.from_problem(prob) as ctx:
+        ctx.set_precision(1)
+        res, _ = ctx.pcg_run(alpha, precond=1, max_iter=20000, chi2r_target=1.0)
+        img = ctx.get("ci").astype(np.float64)
+    J_o, chi2_o, *_ = orc.objective(P, img, alpha)
+    rel = abs(res.J - J_o) / abs(J_o)
+    print(f"iterations {res.iterations} chi2r {res.chi2r:.8f} J {res.J:.10e} FP64 {J_o:.10e} rel {rel:.2e}",
+          flush=True)
+    assert res.converged and res.chi2r <= 1.0
+    assert rel <= 1e-5
+    assert chi2_o / prob.ndof <= 1.0 + 1e-5
+
+
+@pytest.mark.parametrize("prec,bounds", [(0, (1e-6, 1e-8, 3e-6)), (1, (2e-6, 1e-8, 5e-5)),
+                                         (2, (2e-4, 1e-6, 4e-3))], ids=["fp32", "bf16x3", "fp16s"])
+def test_precisions_at_512(orc, cfg3, prec, bounds):
+    """512^2, 147,456 uv: visibilities (max |dV| / max |V|), chi2 (relative) and the
+    data gradient (max |dg| / max |g|) against the FP64 oracle"""
+    prob = cfg3
+    P = orc.unpack(prob)
+    alpha = prob.meta["alpha"]
+    rng = np.random.default_rng(7)
+    img = (prob.start.astype(np.float64) * (1.0 + rng.random(prob.W ** 2))).astype(np.float32)
+    print(f"cfg3: W={prob.W} nuv={prob.nuv} nbis={prob.nbis} precision {prec}", flush=True)
+    with GpairContext.from_problem(prob, image=img) as ctx:
+        ctx.set_precision(prec)
+        chi2 = ctx.gpu_get_chi2_curr()
+        vis = ctx.vis().astype(np.complex128)
+        ctx.gpu_compute_data_gradient_curr()
+        gd = ctx.get("dg").astype(np.float64)
+    J, chi2_o, S_o, vis_o, mock_o, F = orc.objective(P, img.astype(np.float64), alpha)
+    gd_o = orc.gradient(P, img.astype(np.float64), alpha, vis_o, mock_o)[1]
+    ev = np.abs(vis - vis_o).max() / np.abs(vis_o).max()
+    ec = abs(chi2 - chi2_o) / chi2_o
+    eg = np.abs(gd - gd_o).max() / np.abs(gd_o).max()
+    print(f"  vis {ev:.2e} chi2 {ec:.2e} grad {eg:.2e}", flush=True)
+    bv, bc, bg = bounds
+    assert ev <= bv and ec <= bc and eg <= bg
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2], ids=["fp32", "bf16x3", "fp16s"])
+def test_capped_run_reports_exact_objective_at_512(orc, cfg3, prec):
+    """a run stopped by its iteration cap (300 iterations at 512^2, far from the
+    target) reports the objective of the image it returns, evaluated exactly: within
+    1e-4 of FP64 (bound here 1e-5; the tracked sums alone drift by about 6e-4)"""
+    prob = cfg3
+    P = orc.unpack(prob)
+    alpha = prob.meta["alpha"]
+    with GpairContext.from_problem(prob) as ctx:
+        ctx.set_precision(prec)
+        res, _ = ctx.pcg_run(alpha, precond=1, max_iter=300, chi2r_target=0.0)
+        img = ctx.get("ci").astype(np.float64)
+    J_o, chi2_o, S_o, *_ = orc.objective(P, img, alpha)
+    rel = abs(res.J - J_o) / abs(J_o)
+    print(f"precision {prec}: {res.iterations} its, J {res.J:.10e} FP64 {J_o:.10e} rel {rel:.2e}", flush=True)
+    assert res.iterations == 300 and not res.converged
+    assert rel <= 1e-5
+    assert abs(res.chi2 - chi2_o) <= 1e-5 * chi2_o and abs(res.entropy - S_o) <= 1e-5 * abs(S_o)
+
+
+def test_batch_workload_64_problems(orc):
+    """64 problems of uneven sizes, 100 iterations each in one batched run: the
+    smallest and largest problems take exactly the steps of their single runs, and
+    every problem reports the exact objective of its final image (FP64 within 1e-5)"""
+    probs = synth.batch_problems(64)
+    K = 100
+    sizes = [p.W * p.W * (p.npow + p.nbis) for p in probs]
+    lo, hi = int(np.argmin(sizes)), int(np.argmax(sizes))
+    print(f"batch: widths {sorted({p.W for p in probs})}, V2 {min(p.npow for p in probs)}-"
+          f"{max(p.npow for p in probs)}, smallest #{lo} largest #{hi}", flush=True)
+    single = {}
+    for i in (lo, hi):
+        with GpairContext.from_problem(probs[i]) as c:
+            c.set_precision(1)
+            r, _ = c.pcg_run(probs[i].meta["alpha"], precond=1, max_iter=K, chi2r_target=0.0)
+            single[i] = (r, c.get("ci"))
+    ctxs = [GpairContext.from_problem(p) for p in probs]
+    try:
+        for c in ctxs:
+            c.set_precision(1)
+            c.set_graphs(True)
+        res = batch_pcg_run(ctxs, [p.meta["alpha"] for p in probs], max_iter=K, chi2r_target=0.0)
+        imgs = [c.get("ci") for c in ctxs]
+    finally:
+        for c in ctxs:
+            c.close()
+    for i in (lo, hi):
+        r, img = single[i]
+        assert res[i].iterations == r.iterations and res[i].J == r.J and res[i].ls_batches == r.ls_batches
+        assert np.array_equal(imgs[i], img)
+    worst = 0.0
+    for p, r, img in zip(probs, res, imgs):
+        assert r.status == 0 and r.iterations == K
+        J_o = orc.objective(orc.unpack(p), img.astype(np.float64), p.meta["alpha"])[0]
+        worst = max(worst, abs(r.J - J_o) / abs(J_o))
+    print(f"worst J rel err vs FP64 over {len(probs)} problems: {worst:.2e}", flush=True)
+    assert worst <= 1e-5
