@@ -121,6 +121,7 @@ SIGNATURES = {
     "gpair_pixel_propose": (_i, [_vp, _vp, _i, _vp, _vp]),
     "gpair_pixel_apply": (_i, [_vp, _vp, _i]),
     "gpair_pixel_state": (_i, [_vp, _pd, _pd, _pd]),
+    "gpair_chi2_noalloc": (_i, [_vp, _vp, _pd]),
     # data I/O (include/gpair_io.h)
     "gpair_io_last_error": (C.c_char_p, []),
     "gpair_oifits_targets": (_i, [C.c_char_p, _vp, _vp, _i, _pi]),

@@ -204,6 +204,13 @@ class GpairContext:
                                              C.byref(v)), "gpu_get_chi2_curr")
         return v.value
 
+    def chi2_noalloc(self, which="ci"):
+        """gpair_chi2_noalloc: chi2 of an image buffer with the library's static
+        scratch (no device allocation)"""
+        out = C.c_double()
+        self._check(self.L.gpair_chi2_noalloc(self.h, self.buf(which), C.byref(out)), "gpair_chi2_noalloc")
+        return out.value
+
     def gpu_get_chi2_temp(self):
         v = C.c_double()
         self._check(self.L.gpu_get_chi2_temp(self.h, self.nuv, self.npow, self.nbis, self.ndof, self.nuv,

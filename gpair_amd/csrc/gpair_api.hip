@@ -173,6 +173,9 @@ static inline int ew_blocks(int n) { return (n + 255) / 256; }
 
 static inline int tuned_blocks(int v, int dflt) { return v > 0 ? v : dflt; }
 
+// device address of the no-alloc scratch on this device (gpair_noalloc.inc)
+static int noalloc_setup(int device);
+
 // ---------------------------------------------------------------------------
 // setup / teardown
 // ---------------------------------------------------------------------------
@@ -196,6 +199,10 @@ extern "C" int gpu_init(gpair_ctx **out, int device)
         return GPAIR_E_HIP;
     }
     int rc;
+    if ((rc = noalloc_setup(device))) {
+        gpu_cleanup(c);
+        return rc;
+    }
     // the line-search sums and the commit partials follow the scalars, so that one
     // copy reads them back (RB_* offsets)
     if ((rc = dalloc(c, &c->scal, RB_COUNT)) || (rc = dalloc(c, &c->invflux, 4)) ||
@@ -1291,3 +1298,4 @@ extern "C" int gpair_snapshot_write(gpair_ctx *c)
 #include "gpair_pcg.inc"
 #include "gpair_tune.inc"
 #include "gpair_pixel.inc"
+#include "gpair_noalloc.inc"

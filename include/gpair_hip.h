@@ -332,6 +332,22 @@ int gpair_pixel_apply(gpair_ctx *ctx, const gpair_pixel_move *moves, int count);
  * flux of the pixel state */
 int gpair_pixel_state(gpair_ctx *ctx, double *chi2, double *entropy, double *flux);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (harnesses that forbid hipMalloc, tight serving loops).  chi2 of `image` (an
+ * image buffer of the context, e.g. gpu_getp_ci) as gpu_get_chi2_curr computes it
+ * -- flux, forward DFT (FP32, or BF16x3 on tables set with gpair_set_uv), V2/T3
+ * model, chi2, the same kernels and launch shapes, so the same value -- but every
+ * intermediate (visibilities, mock data, split image, forward partials, reduction
+ * partials, tickets, scalars) lives in the library's static device scratch
+ * (11.3 MiB per device, held until the process ends) instead of context buffers:
+ * the call makes no hipMalloc / hipFree.  Limits: nuv <= 262144, npow + 2 nbis <=
+ * 1048576, W <= 512; larger problems and FP16 DFTs get GPAIR_E_ARG.  Unlike
+ * gpu_get_chi2_curr it leaves the context's visibilities and mock data as they
+ * were.  The scratch is shared by every call on one device and holds the last
+ * call's contents: calls that overlap on one device (other contexts, other host
+ * threads) race. */
+int gpair_chi2_noalloc(gpair_ctx *ctx, gpair_buf image, double *chi2);
+
 /* ---- host-side data preparation (no device needed) ----------------------- */
 /* set_model, gpair1/trunk/cpu.c:645-749: prior image [W*W] of total flux `flux`.
  * type 0 flat, 1 centred Dirac, 2 uniform disk (radius width mas), 3 Gaussian
