@@ -376,3 +376,45 @@ class RefCPU:
 
 def ref_available() -> bool:
     return os.path.exists(REF_SO)
+
+
+REF_OIFITS = os.path.join(HERE, "_ref", "ref_oifits")
+REF_LOOP = os.path.join(HERE, "_ref", "gpair_cpu")
+
+
+def ref_oifits_read(path, target_id=-1, minband=-1.0, maxband=-1.0, workdir=None):
+    """the reference's own loader (get_oi_fits_selection + get_oi_fits_data,
+    gpair1/trunk/getoifits.c, compiled into oracle/_ref/ref_oifits): a dict of
+    arrays in the keys of oifits_oracle.read.  The band defaults as the product's
+    (1 .. 50000 nm); a target id goes to the reference's prompt, which it reads
+    only for files with several targets."""
+    import tempfile
+    if minband < 0 or maxband <= minband:
+        minband, maxband = 1.0, 50000.0
+    with tempfile.TemporaryDirectory(dir=workdir) as d:
+        out = os.path.join(d, "oi.bin")
+        stdin = f"{target_id}\n" if target_id >= 0 else ""
+        r = subprocess.run([REF_OIFITS, os.path.abspath(path), out, repr(float(minband)), repr(float(maxband))], input=stdin,
+                           capture_output=True, text=True, timeout=60, cwd=d)
+        if r.returncode != 0:
+            raise RuntimeError(f"ref_oifits failed ({r.returncode}): {r.stderr[-500:]}")
+        raw = open(out, "rb").read()
+    npow, nbis, nuv = np.frombuffer(raw[:12], np.int32)
+    o = 12
+    res = {}
+
+    def take(name, dtype, count, shape=None):
+        nonlocal o
+        a = np.frombuffer(raw, dtype, count, o).copy()
+        o += a.nbytes
+        res[name] = a.reshape(shape) if shape else a
+    for k in ("pow", "powerr"):
+        take(k, np.float32, npow)
+    for k in ("bisamp", "bisamperr", "bisphs", "bisphserr"):
+        take(k, np.float32, nbis)
+    for k in ("u", "v", "wavelength", "bandwidth"):
+        take(k, np.float32, nuv)
+    take("uvpnt", np.int64, 3 * nbis, (-1, 3))
+    take("sign", np.int16, 3 * nbis, (-1, 3))
+    assert o == len(raw)
+    return res

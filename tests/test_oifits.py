@@ -15,6 +15,7 @@ import pytest
 
 import fits_files as FF
 import oifits_oracle as OO
+import oracle as O
 from gpair_amd import oifits, synth
 
 # the reference's own data file (gpair1/trunk/2004contest1.oifits), kept as a fixture
@@ -62,9 +63,7 @@ def test_reference_data_file():
     assert d.uv_compares <= 2 * 3 * d.nbis
 
 
-def test_threshold_conjugate_and_zero_cases(tmp_path):
-    """legs that match a stored point within the relative threshold 5e-5 (same or
-    conjugate direction), legs just outside it, and zero coordinates"""
+def _write_cases(path):
     lam = 1.0e-6
     base_u = np.array([10.0, 25.0, 0.0, 40.0]) * lam * 1e6
     base_v = np.array([3.0, -7.0, 5.0, 0.0]) * lam * 1e6
@@ -76,11 +75,42 @@ def test_threshold_conjugate_and_zero_cases(tmp_path):
     u2 = np.array([-base_u[1], -base_u[1], base_u[3], -base_u[3]])
     v2 = np.array([-base_v[1], -base_v[1], 0.0, 0.0])
     nt = u1.size
-    path = str(tmp_path / "cases.oifits")
     FF.write(path, FF.primary(), FF.oi_target([7], ["STAR"]), FF.oi_wavelength("INS", [lam], [1e-8]),
              FF.oi_vis2("INS", [7] * n, base_u, base_v, np.full(n, 0.5), np.full(n, 0.01)),
              FF.oi_t3("INS", [7] * nt, u1, v1, u2, v2, np.full(nt, 0.1), np.full(nt, 0.01), np.zeros(nt),
                       np.ones(nt)))
+    return path
+
+
+def _write_multi(path):
+    """two targets, two instruments of three channels, flags, a NaN T3 amplitude"""
+    rng = np.random.default_rng(3)
+    wa = np.array([1.5e-6, 1.65e-6, 1.8e-6])
+    wb = np.array([0.7e-6, 0.8e-6, 0.9e-6])
+    hd = [FF.primary(), FF.oi_target([1, 2], ["ALPHA", "BETA"]), FF.oi_wavelength("A", wa, wa * 0.02),
+          FF.oi_wavelength("B", wb, wb * 0.02)]
+    for ins, tgt, nrec in (("A", 1, 6), ("B", 1, 4), ("A", 2, 5)):
+        u, v = rng.uniform(-80, 80, nrec), rng.uniform(-80, 80, nrec)
+        flag = rng.random((nrec, 3)) < 0.2
+        ids = [tgt] * nrec
+        hd.append(FF.oi_vis2(ins, ids, u, v, rng.random((nrec, 3)), rng.random((nrec, 3)) * 0.1, flag,
+                             data_form="E" if ins == "B" else "D"))
+        # T3 over pairs of those baselines (legs match V2 points of the same channel)
+        i = np.arange(nrec - 1)
+        amp = rng.random((nrec - 1, 3))
+        amp[0, 1] = np.nan
+        hd.append(FF.oi_t3(ins, [tgt] * (nrec - 1), u[i], v[i], u[i + 1], v[i + 1], amp, amp * 0.1,
+                           rng.uniform(-180, 180, (nrec - 1, 3)), np.ones((nrec - 1, 3)),
+                           rng.random((nrec - 1, 3)) < 0.2))
+    FF.write(path, *hd)
+    return path
+
+
+def test_threshold_conjugate_and_zero_cases(tmp_path):
+    """legs that match a stored point within the relative threshold 5e-5 (same or
+    conjugate direction), legs just outside it, and zero coordinates"""
+    path = _write_cases(str(tmp_path / "cases.oifits"))
+    n = 4
     d = oifits.read(path)
     _same(d, OO.read(path))
     assert d.uvpnt[0, 0] == 0 and d.sign[0, 0] == 1              # within 4e-5: point 0
@@ -102,26 +132,7 @@ def test_threshold_conjugate_and_zero_cases(tmp_path):
 def test_selection_flags_band_and_nan_amplitudes(tmp_path):
     """two targets (tables kept by their first record's id), two instruments with
     three channels, flagged data, a waveband cut and a NaN T3 amplitude"""
-    rng = np.random.default_rng(3)
-    wa = np.array([1.5e-6, 1.65e-6, 1.8e-6])
-    wb = np.array([0.7e-6, 0.8e-6, 0.9e-6])
-    hd = [FF.primary(), FF.oi_target([1, 2], ["ALPHA", "BETA"]), FF.oi_wavelength("A", wa, wa * 0.02),
-          FF.oi_wavelength("B", wb, wb * 0.02)]
-    for ins, tgt, nrec in (("A", 1, 6), ("B", 1, 4), ("A", 2, 5)):
-        u, v = rng.uniform(-80, 80, nrec), rng.uniform(-80, 80, nrec)
-        flag = rng.random((nrec, 3)) < 0.2
-        ids = [tgt] * nrec
-        hd.append(FF.oi_vis2(ins, ids, u, v, rng.random((nrec, 3)), rng.random((nrec, 3)) * 0.1, flag,
-                             data_form="E" if ins == "B" else "D"))
-        # T3 over pairs of those baselines (legs match V2 points of the same channel)
-        i = np.arange(nrec - 1)
-        amp = rng.random((nrec - 1, 3))
-        amp[0, 1] = np.nan
-        hd.append(FF.oi_t3(ins, [tgt] * (nrec - 1), u[i], v[i], u[i + 1], v[i + 1], amp, amp * 0.1,
-                           rng.uniform(-180, 180, (nrec - 1, 3)), np.ones((nrec - 1, 3)),
-                           rng.random((nrec - 1, 3)) < 0.2))
-    path = str(tmp_path / "multi.oifits")
-    FF.write(path, *hd)
+    path = _write_multi(str(tmp_path / "multi.oifits"))
     assert oifits.targets(path) == [(1, "ALPHA"), (2, "BETA")]
     with pytest.raises(oifits.OifitsError, match="2 targets"):
         oifits.read(path)
@@ -213,3 +224,33 @@ def test_fits_images(tmp_path):
     bad.write_bytes(b"not a fits file" * 300)
     with pytest.raises(oifits.OifitsError):
         oifits.read_image(str(bad))
+
+
+@pytest.mark.skipif(not os.path.exists(O.REF_OIFITS), reason="reference loader not built (oracle/_ref)")
+def test_reference_loader_agrees(tmp_path):
+    """the reference's own loader (get_oi_fits_selection + get_oi_fits_data,
+    gpair1/trunk/getoifits.c:65-714 with read_fits.c and cfitsio, compiled into
+    oracle/_ref by oracle/Makefile) on every kind of test file -- the reference's
+    data file, the threshold / conjugate / zero cases, two targets with two
+    instruments, flags, band cuts and a NaN amplitude, and synthetic layouts
+    written by gpair_oifits_write -- gives the same arrays as the product and the
+    numpy restatement, bit for bit (uv order, first-match references, signs)"""
+    cases = []
+    if os.path.exists(REF_FILE):
+        cases.append((REF_FILE, -1, -1.0, -1.0))
+    cases.append((_write_cases(str(tmp_path / "cases.oifits")), -1, -1.0, -1.0))
+    multi = _write_multi(str(tmp_path / "multi.oifits"))
+    cases += [(multi, 1, -1.0, -1.0), (multi, 2, -1.0, -1.0), (multi, 1, 1000.0, 2000.0), (multi, 1, 600.0, 850.0)]
+    for k, kw in enumerate((dict(ntel=5, nch=3, nsnap=2), dict(ntel=9, nch=6, nsnap=12))):
+        prob = synth.make_problem("cfg1", seed=4 + k, **kw)
+        path = str(tmp_path / f"prob{k}.oifits")
+        oifits.write_problem(path, prob)
+        cases.append((path, -1, -1.0, -1.0))
+    for path, tid, lo, hi in cases:
+        ref = O.ref_oifits_read(path, tid, lo, hi, workdir=str(tmp_path))
+        d = oifits.read(path, tid, lo, hi)
+        print(f"{os.path.basename(path)} target {tid} band {lo}-{hi}: {ref['pow'].size} V2, {ref['bisamp'].size} T3, "
+              f"{ref['u'].size} uv", flush=True)
+        _same(d, ref)
+        if ref["bisamp"].size < 1000:      # the numpy linear scan is slow on the 6,048-T3 layout
+            _same(d, OO.read(path, tid, lo, hi))
