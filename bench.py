@@ -16,10 +16,12 @@ timed region.  A step is one CG iteration.
   (gpair_cg_run, FP32 DFT), the latter bounded by --plain-budget-s.
 * roofline: the forward DFT, timed with HIP events on the context stream, against
   the roofline that binds it.  FP32 reads the full X and Y tables (64 flop per
-  compulsory byte at 256^2, above the FP32 balance of 20): MFMA.  BF16x3 rebuilds
-  table entries from the factored tables (8 (W/16 + 16) bytes per uv point and
-  axis: 502 flop per compulsory byte at 256^2, above its balance of 104): MFMA,
-  against the BF16 peak over the 3 products per multiply-add.
+  compulsory byte at 256^2, above the FP32 balance of 20): MFMA, against 157.3
+  TFLOP/s.  BF16x3 rebuilds table entries from the factored tables (8 (W/16 + 16)
+  bytes per uv point and axis: 502 flop per compulsory byte at 256^2, above the
+  BF16 balance of 312): MFMA.  frac = algorithmic flops per launch / launch time /
+  the published dense BF16 (or FP16) peak of 2.5 PFLOP/s; mfma_pipe_frac counts the
+  MFMA work actually issued (3 BF16 products per multiply-add) against the same peak.
 * cpu_baseline: the reference's own CPU model (oracle/_ref) on one core; the
   same leg re-evaluates each final image's objective with the FP64 oracle
   (err_vs_fp64).
@@ -48,7 +50,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "wall-time & CG iters/s to chi2_r=1 at 256², BF16x3/learned vs plain FP32; err vs FP64"
-PEAK = {"fp32": 157.3, "bf16x3": 2500.0 / 3.0, "fp16s": 2500.0}   # TFLOP/s: FP32 MFMA; BF16 MFMA / 3 products; FP16
+PEAK = {"fp32": 157.3, "bf16x3": 2500.0, "fp16s": 2500.0}   # TFLOP/s, dense MFMA: FP32, BF16, FP16
+PRODUCTS = {"fp32": 1, "bf16x3": 3, "fp16s": 1}               # MFMA products per algorithmic multiply-add
 HBM_PEAK = 8.0e12                                  # B/s
 TIMING_EVERY = 8   # DFT launches per timed launch in the headline loop
 PROFILES = os.path.join(ROOT, "profiles")
@@ -79,6 +82,8 @@ def parse():
     ap.add_argument("--cfg3-iters", type=int, default=300, help="accelerated iterations per precision at 512x512")
     ap.add_argument("--move-calls", type=int, default=20, help="timed gpair_pixel_propose calls")
     ap.add_argument("--cpu-kind", default="auto", choices=["auto", "reference", "port"])
+    ap.add_argument("--ref-loop-s", type=float, default=20.0,
+                    help="seconds of the reference's own CPU program on the 64x64 workload (0: skip)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 256 MiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -97,13 +102,17 @@ def calib_copy():
     print(json.dumps({"calib_copy_bytes": n * 4}))
 
 
-def read_traffic(name):
-    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if present."""
+def read_traffic(name, kernel):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a device copy of known size), if present."""
     path = os.path.join(PROFILES, name)
     if not os.path.exists(path):
         return None
+    seen = False
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch"):
+        if line.startswith(kernel + ":"):
+            seen = True
+        elif seen and line.startswith("traffic_bytes_per_launch"):
             return float(line.split("=")[1])
     return None
 
@@ -156,6 +165,75 @@ def cpu_baseline(prob, evals_J, evals_G, kind):
             "sample": f"FP64 oracle: one J ({tJ:.2f} s) + one gradient ({tG:.2f} s) x evals per iteration"}
 
 
+def reference_loop_cfg1(a):
+    """The reference's own CPU reconstruction program (gpair1/branches/cpu_dev, built
+    by oracle/Makefile into oracle/_ref/gpair_cpu) on the 64x64 workload (480 V2 +
+    640 T3, flat prior, alpha 0.025) written as OIFITS, pinned to one core and stopped
+    after a.ref_loop_s seconds; its CG iterations are counted from the per-iteration
+    line it prints.  Beside it, the same OIFITS file through the library: the
+    reference rules (gpair_cg_run, FP32 DFTs) for as many iterations, and the
+    accelerated loop to chi2_r = 1."""
+    import subprocess
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from gpair_amd import oifits, synth
+    from gpair_amd.api import GpairContext
+    if not os.path.exists(O.REF_LOOP):
+        return None
+    prob = synth.make_problem("cfg1", seed=a.seed)
+    W, alpha = prob.W, prob.meta["alpha"]
+    cpu = sorted(os.sched_getaffinity(0))[0]
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "cfg1.oifits")
+        oifits.write_problem(path, prob)
+        cmd = [O.REF_LOOP, "-d", path, "-w", str(W), "-s", repr(prob.pixellation), "-mt", "0", "-mf",
+               repr(float(W * W)), "-alpha", repr(alpha), "-wavmin", "1", "-wavmax", "50000"]
+        log = os.path.join(d, "ref.log")
+        with open(log, "w") as fh:
+            t0 = time.perf_counter()
+            pr = subprocess.Popen(cmd, stdin=subprocess.DEVNULL, stdout=fh, stderr=subprocess.STDOUT, cwd=d,
+                                  preexec_fn=lambda: os.sched_setaffinity(0, {cpu}))
+            try:
+                pr.wait(timeout=a.ref_loop_s)
+            except subprocess.TimeoutExpired:
+                pr.kill()
+                pr.wait()
+            wall = time.perf_counter() - t0
+        lines = [ln for ln in open(log, errors="replace") if "Grad evals" in ln or "J = " in ln]
+        tops = [ln for ln in open(log, errors="replace") if ln.startswith("Grad evals")]
+        js = [ln for ln in open(log, errors="replace") if ln.startswith("J = ")]
+        its = max(len(tops) - 1, 0)   # each line opens an iteration; the last one was cut off
+        last = js[-1].strip() if js else ""
+        gp = oifits.problem(path, W, prob.pixellation, model_type=0, model_flux=float(W * W))
+    out = {"workload": f"cfg1 written as OIFITS: {W}x{W} px, {prob.npow} V2 + {prob.nbis} T3, flat prior, "
+                       f"alpha {alpha}",
+           "reference_cpu": {"program": "gpair1/branches/cpu_dev (oracle/_ref/gpair_cpu)", "cores": 1,
+                             "iterations": its, "wall_s": wall, "iters_per_s": its / wall,
+                             "last_line": last, "lines": len(lines)}}
+    n_plain = max(its, 1)
+    with GpairContext.from_problem(gp) as c:
+        t0 = time.perf_counter()
+        r, _ = c.cg_run(alpha, method=2, max_iter=n_plain, chi2r_target=0.0)
+        c.L.gpair_sync(c.h)
+        tw = time.perf_counter() - t0
+    out["gpu_reference_rules_fp32"] = {"iterations": r.iterations, "wall_s": tw, "iters_per_s": r.iterations / tw,
+                                       "chi2r": r.chi2r}
+    with GpairContext.from_problem(gp) as c:
+        c.set_precision(1)
+        t0 = time.perf_counter()
+        r, _ = c.pcg_run(alpha, precond=1, max_iter=a.max_iter, chi2r_target=1.0)
+        c.L.gpair_sync(c.h)
+        tw = time.perf_counter() - t0
+    out["gpu_accelerated_bf16x3_to_target"] = {"iterations": r.iterations, "wall_s": tw,
+                                               "iters_per_s": r.iterations / tw, "converged": bool(r.converged),
+                                               "chi2r": r.chi2r}
+    print(f"# reference CPU loop cfg1: {its} its in {wall:.1f} s on 1 core; GPU reference rules "
+          f"{out['gpu_reference_rules_fp32']['iters_per_s']:.0f} it/s, accelerated to target "
+          f"{out['gpu_accelerated_bf16x3_to_target']['wall_s']:.3f} s", file=sys.stderr, flush=True)
+    return out
+
+
 def fp64_errors(prob, alpha, finals):
     """relative difference between each run's reported objective and the FP64
     oracle's objective of its final image (checker only)."""
@@ -176,9 +254,9 @@ def precision_workload(a):
     in FP32, BF16x3 and FP16 with scaling, against the FP64 oracle: max |dV| / max |V|,
     relative chi2, max |dg| / max |g|; the DFTs' mean times over 10 evaluations (HIP
     events); then cfg3_iters accelerated iterations from the prior in each precision:
-    the tracked objective (FP64 sums of the committed changes, drifting by the DFTs'
-    errors between exact evaluations) and an exact evaluation of the final image in the
-    working precision, both against the FP64 objective of that image (checker only)."""
+    the objective the run reports (evaluated exactly at its iteration cap) and a
+    separate evaluation of the final image in the working precision, both against the
+    FP64 objective of that image (checker only)."""
     from gpair_amd import synth
     from gpair_amd.api import GpairContext
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -227,13 +305,13 @@ def precision_workload(a):
                      "grad_err": float(np.abs(gd - gd_o).max() / gmax),
                      "forward_us": fms / max(fn, 1) * 1e3, "adjoint_us": ams / max(an, 1) * 1e3,
                      "run": {"iterations": r2.iterations, "wall_s": wall, "iters_per_s": r2.iterations / wall,
-                             "chi2r": r2.chi2r, "J_tracked": r2.J, "J_exact": J_ex, "J_fp64": J64,
-                             "J_tracked_rel_err": abs(r2.J - J64) / abs(J64),
+                             "chi2r": r2.chi2r, "J_reported": r2.J, "J_exact": J_ex, "J_fp64": J64,
+                             "J_reported_rel_err": abs(r2.J - J64) / abs(J64),
                              "J_exact_rel_err": abs(J_ex - J64) / abs(J64)}}
         print(f"# cfg3 {name}: vis {out[name]['vis_err']:.2e} chi2 {out[name]['chi2_rel_err']:.2e} "
               f"grad {out[name]['grad_err']:.2e} fwd {out[name]['forward_us']:.1f} us adj "
-              f"{out[name]['adjoint_us']:.1f} us; {r2.iterations} its J err tracked "
-              f"{out[name]['run']['J_tracked_rel_err']:.2e} exact {out[name]['run']['J_exact_rel_err']:.2e}",
+              f"{out[name]['adjoint_us']:.1f} us; {r2.iterations} its J err reported "
+              f"{out[name]['run']['J_reported_rel_err']:.2e} exact {out[name]['run']['J_exact_rel_err']:.2e}",
               file=sys.stderr, flush=True)
     return out
 
@@ -431,7 +509,7 @@ def main():
     table_bytes = 16.0 * nuv * W if prec == 0 else 16.0 * nuv * (W // 16 + 16)
     nbytes = table_bytes + (2.0 if prec == 2 else 4.0) * W * W + 8.0 * nuv
     intensity = flop / nbytes
-    traffic = read_traffic(f"forward_dft_W{W}_{a.precision}_pmc.txt")
+    traffic = read_traffic(f"forward_dft_W{W}_{a.precision}_pmc.txt", kname)
     if intensity < PEAK[a.precision] * 1e12 / HBM_PEAK:
         # below the machine balance: the HBM roofline binds (BF16x3)
         roofline = {"bound": "hbm", "achieved": nbytes / avg_fwd / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -441,9 +519,9 @@ def main():
                     "frac": achieved / PEAK[a.precision]}
     roofline.update({"traffic": traffic, "kernel": f"{kname} (forward DFT)", "avg_us": avg_fwd * 1e6,
                      "launches": fwd_n, "timed_every": TIMING_EVERY, "flop_per_launch": flop,
-                     "bytes_per_launch": nbytes,
-                     "flop_per_byte": intensity, "mfma_tflops": achieved,
-                     "mfma_frac": achieved / PEAK[a.precision]})
+                     "bytes_per_launch": nbytes, "flop_per_byte": intensity,
+                     "mfma_pipe_tflops": achieved * PRODUCTS[a.precision],
+                     "mfma_pipe_frac": achieved * PRODUCTS[a.precision] / PEAK[a.precision]})
     aname = {0: "k_adj_f32", 1: "k_adj_b3wf", 2: "k_adj_h1wf"}[prec] if W % 256 == 0 else (
         "k_adj_b3" if prec == 1 else "k_adj_f32")
     out = {
@@ -577,6 +655,8 @@ def main():
     if not a.no_cpu:
         counts = plain_counts if plain_counts else (10.0, 2.0)
         out["cpu_baseline"] = cpu_baseline(prob, counts[0], counts[1], a.cpu_kind)
+        if a.ref_loop_s > 0:
+            out["reference_loop_cfg1"] = reference_loop_cfg1(a)
         if finals:
             out["err_vs_fp64"] = fp64_errors(prob, alpha, finals)
     print(json.dumps(out), flush=True)

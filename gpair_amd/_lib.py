@@ -41,7 +41,7 @@ class PcgParams(C.Structure):
 class PcgResult(C.Structure):
     _fields_ = [("iterations", _i), ("forward_dfts", _i), ("adjoint_dfts", _i), ("ls_batches", _i), ("J", _d),
                 ("chi2", _d), ("entropy", _d), ("chi2r", _d), ("converged", _i), ("stalled", _i), ("status", _i),
-                ("resyncs", _i)]
+                ("resyncs", _i), ("sparse_commits", _i)]
 
 
 # name -> (restype, argtypes)
@@ -109,6 +109,7 @@ SIGNATURES = {
     "gpair_set_pfactor": (_i, [_vp, _vp, _i]),
     "gpair_autotune": (_i, [_vp, _i, _vp, _i]),
     "gpair_set_graphs": (_i, [_vp, _i]),
+    "gpair_set_sparse_commit": (_i, [_vp, _i]),
     "gpair_pcg_run_checkpointed": (_i, [_vp, C.POINTER(PcgParams), C.POINTER(PcgResult), _vp, _i, C.c_char_p,
                                         C.c_char_p]),
     "gpair_set_snapshot": (_i, [_vp, C.c_char_p, _i, _f]),

@@ -338,6 +338,10 @@ class GpairContext:
         self._check(self.L.gpair_set_snapshot(self.h, None if path is None else str(path).encode(), int(every),
                                               float(pixsize)), "gpair_set_snapshot")
 
+    def set_sparse_commit(self, max_pixels=-1):
+        """gpair_set_sparse_commit: -1 default cap (W^2/16), 0 always the full DFT"""
+        self._check(self.L.gpair_set_sparse_commit(self.h, int(max_pixels)), "gpair_set_sparse_commit")
+
     def set_graphs(self, on=True):
         self._check(self.L.gpair_set_graphs(self.h, int(bool(on))), "gpair_set_graphs")
 

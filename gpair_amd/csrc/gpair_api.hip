@@ -71,6 +71,14 @@ struct gpair_ctx {
     int ls_blocks_chi2, ls_blocks_ent, coef_blocks;
     int fwd_nbk;   // image rows per BF16x3 forward group / 32 (0: min(W, 256) / 32)
     int graphs;    // replay the accelerated loop's gradient phase from a hipGraph
+    // sparse commits: the clamped pixels of a committed step, per k_commit_step block
+    // (clist_cper slots each), and the most pixels for which the change's DFT is taken
+    // as s U_d plus a correction over them (-1: W^2 / 16, 0: always the full DFT)
+    int2 *clist;
+    int *ccount;
+    int *dense;   // set by the sparse kernel: 1 when the commit takes the full DFT
+    int clist_cper;
+    int commit_cap;
     // single-pixel moves: state image, FP64 unnormalised visibilities, x-major
     // tables, trial visibilities and mock data, per-move buffers
     float *px_img;
@@ -194,6 +202,7 @@ extern "C" int gpu_init(gpair_ctx **out, int device)
         return GPAIR_E_NOMEM;
     c->device = device;
     c->precision = GPAIR_PREC_FP32;
+    c->commit_cap = -1;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return GPAIR_E_HIP;
@@ -238,6 +247,9 @@ static void free_images(gpair_ctx *c)
         b->n = 0;
     }
     dfree(&c->zpart);
+    dfree(&c->clist);
+    dfree(&c->ccount);
+    dfree(&c->dense);
 }
 
 extern "C" int gpu_cleanup(gpair_ctx *c)
@@ -315,6 +327,12 @@ static int alloc_images(gpair_ctx *c, int W)
         HIPCHK(c, hipMemsetAsync(b->p, 0, sizeof(float) * n, c->stream));
     }
     if ((rc = dalloc(c, &c->zpart, 2 * (size_t)((n + GF_PIX - 1) / GF_PIX))))
+        return rc;
+    // k_commit_step's blocks take RED_THREADS pixels per pass, grid-stride
+    const int nb = red_blocks(n);
+    c->clist_cper = (n + nb * RED_THREADS - 1) / (nb * RED_THREADS) * RED_THREADS;
+    if ((rc = dalloc(c, &c->clist, (size_t)nb * c->clist_cper)) || (rc = dalloc(c, &c->ccount, (size_t)nb)) ||
+        (rc = dalloc(c, &c->dense, 1)))
         return rc;
     c->W = W;
     c->n = n;
@@ -730,7 +748,10 @@ static int launch_flux(gpair_ctx *c, const float *img)
 
 // vis_out = scale ? (*scale) A img : A img  (A = unnormalised DFT)
 // presplit: the BF16 split of img is already in Ih / Il (written by its producer)
-static int launch_forward_to(gpair_ctx *c, const float *img, const float *scale, float2 *vis_out, int presplit = 0)
+// run_if (FP32 and factored BF16x3 only): a device flag; the launches do nothing when
+// it is 0 (a commit the sparse correction gave); such launches are not timed
+static int launch_forward_to(gpair_ctx *c, const float *img, const float *scale, float2 *vis_out, int presplit = 0,
+                             const int *run_if = nullptr)
 {
     if (c->precision == GPAIR_PREC_FP16S) {
         int rc;
@@ -753,7 +774,8 @@ static int launch_forward_to(gpair_ctx *c, const float *img, const float *scale,
                      (k_vis_fwd_h1f<N><<<nwg, 256, 0, c->stream>>>(c->Ih, c->FX, c->FY, c->nuv, c->W, upw, scale, \
                                                                    c->scal + SC_AMAXI, vis_out, c->fwdp),       \
                       k_vis_fwd_b3_combine<<<nwg > 1 ? nwg - 1 : 1, B3_PW, 0, c->stream>>>(c->fwdp, c->nuv, c->W, \
-                                                                                         upw, scale, vis_out))); \
+                                                                                         upw, scale, vis_out, \
+                                                                                         nullptr)));          \
         break;
             H1_CASE(2)
             H1_CASE(4)
@@ -784,18 +806,23 @@ static int launch_forward_to(gpair_ctx *c, const float *img, const float *scale,
         switch (nbk) {
 #define B3_CASE(N)                                                                                             \
     case N:                                                                                                    \
-        if (c->have_ft)                                                                                        \
+        if (c->have_ft && run_if) {                                                                            \
+            k_vis_fwd_b3f<N><<<nwg, 256, 0, c->stream>>>(c->Ih, c->Il, c->FX, c->FY, c->nuv, c->W, upw, scale,      \
+                                                          vis_out, c->fwdp, run_if);                               \
+            k_vis_fwd_b3_combine<<<nwg > 1 ? nwg - 1 : 1, B3_PW, 0, c->stream>>>(c->fwdp, c->nuv, c->W, upw,       \
+                                                                                 scale, vis_out, run_if);          \
+        } else if (c->have_ft)                                                                                 \
             TIMED_LAUNCH(c, GPAIR_TK_FORWARD,                                                                  \
                          (k_vis_fwd_b3f<N><<<nwg, 256, 0, c->stream>>>(c->Ih, c->Il, c->FX, c->FY, c->nuv, c->W, \
-                                                                       upw, scale, vis_out, c->fwdp),          \
+                                                                       upw, scale, vis_out, c->fwdp, nullptr), \
                           k_vis_fwd_b3_combine<<<nwg > 1 ? nwg - 1 : 1, B3_PW, 0, c->stream>>>(             \
-                              c->fwdp, c->nuv, c->W, upw, scale, vis_out)));                                   \
+                              c->fwdp, c->nuv, c->W, upw, scale, vis_out, nullptr)));                          \
         else                                                                                                   \
             TIMED_LAUNCH(c, GPAIR_TK_FORWARD,                                                                  \
                          (k_vis_fwd_b3<N><<<nwg, 256, 0, c->stream>>>(c->Ih, c->Il, c->Xh, c->Xl, c->Y, c->nuv,  \
                                                                       c->W, upw, scale, vis_out, c->fwdp),     \
                           k_vis_fwd_b3_combine<<<nwg > 1 ? nwg - 1 : 1, B3_PW, 0, c->stream>>>(             \
-                              c->fwdp, c->nuv, c->W, upw, scale, vis_out)));                                   \
+                              c->fwdp, c->nuv, c->W, upw, scale, vis_out, nullptr)));                          \
         break;
             B3_CASE(2)
             B3_CASE(4)
@@ -811,8 +838,11 @@ static int launch_forward_to(gpair_ctx *c, const float *img, const float *scale,
     switch (c->W / 64) {
 #define FWD_CASE(NB)                                                                                       \
     case NB:                                                                                               \
-        TIMED_LAUNCH(c, GPAIR_TK_FORWARD,                                                                  \
-                     (k_vis_fwd_f32<NB><<<grid, 256, 0, c->stream>>>(img, c->X, c->Y, c->nuv, scale, vis_out))); \
+        if (run_if)                                                                                        \
+            k_vis_fwd_f32<NB><<<grid, 256, 0, c->stream>>>(img, c->X, c->Y, c->nuv, scale, vis_out, run_if); \
+        else                                                                                               \
+            TIMED_LAUNCH(c, GPAIR_TK_FORWARD, (k_vis_fwd_f32<NB><<<grid, 256, 0, c->stream>>>(               \
+                                                  img, c->X, c->Y, c->nuv, scale, vis_out, nullptr)));     \
         break;
     FWD_CASE(1)
     FWD_CASE(2)
@@ -1207,6 +1237,16 @@ extern "C" void gpair_pcg_default_params(gpair_pcg_params *p)
     p->minvalue = 1e-8;
     p->wolfe1 = 1e-4;
     p->verbose = 0;
+}
+
+extern "C" int gpair_set_sparse_commit(gpair_ctx *c, int max_pixels)
+{
+    if (!c)
+        return GPAIR_E_ARG;
+    if (max_pixels < -1)
+        return fail(c, GPAIR_E_ARG, "gpair_set_sparse_commit: max_pixels %d (-1, 0 or a count)", max_pixels);
+    c->commit_cap = max_pixels;
+    return GPAIR_OK;
 }
 
 extern "C" int gpair_set_pfactor(gpair_ctx *c, const float *pf, int size)

@@ -87,8 +87,11 @@ __global__ __launch_bounds__(RED_THREADS) void k_dot(const float *__restrict__ a
 template <int NB>
 __global__ __launch_bounds__(256) void k_vis_fwd_f32(const float *__restrict__ img, const float2 *__restrict__ X,
                                                      const float2 *__restrict__ Y, int nuv,
-                                                     const float *__restrict__ invflux, float2 *__restrict__ vis)
+                                                     const float *__restrict__ invflux, float2 *__restrict__ vis,
+                                                     const int *__restrict__ run_if)
 {
+    if (run_if && *run_if == 0)   // a commit whose change DFT the sparse correction already gave
+        return;
     constexpr int W = 64 * NB;
     constexpr int SB = W + 16;
     __shared__ float sA[FWD_KC * FWD_SA];
@@ -1148,36 +1151,68 @@ __global__ __launch_bounds__(RED_THREADS) void k_commit_step(const float *__rest
                                                              int n, float *__restrict__ trial,
                                                              float *__restrict__ delta, double *__restrict__ cs_part,
                                                              unsigned short *__restrict__ th,
-                                                             unsigned short *__restrict__ tl)
+                                                             unsigned short *__restrict__ tl,
+                                                             int2 *__restrict__ clist, int *__restrict__ ccount,
+                                                             int cper, int W)
 {
     __shared__ double sh[3][RED_THREADS / 64];
+    __shared__ int wcnt[RED_THREADS / 64];
     const float s = s_dev ? (float)*s_dev : s_arg;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double v[3] = {0.0, 0.0, 0.0};
-    for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) {
-        const float I = img[i];
-        float t = I + s * d[i];
-        if (t < minval) {
-            t = minval;
-            v[2] += 1.0;
+    int base = 0;   // clamped pixels listed by this block so far
+    for (int i0 = blockIdx.x * RED_THREADS; i0 < n; i0 += gridDim.x * RED_THREADS) {
+        const int i = i0 + (int)threadIdx.x;
+        bool clamped = false;
+        float corr = 0.f;
+        if (i < n) {
+            const float I = img[i];
+            float t = I + s * d[i];
+            if (t < minval) {
+                t = minval;
+                v[2] += 1.0;
+                clamped = true;
+            }
+            trial[i] = t;
+            const float dl = t - I;
+            delta[i] = dl;
+            corr = dl - s * d[i];   // the change beyond s d at a clamped pixel
+            if (th) {
+                unsigned short h, l;
+                split_bf16(dl, h, l);
+                th[i] = h;
+                tl[i] = l;
+            }
+            const double M = model[i];
+            v[0] += (double)dl;
+            v[1] += pix_entropy((double)t, M) - pix_entropy((double)I, M);
         }
-        trial[i] = t;
-        const float dl = t - I;
-        delta[i] = dl;
-        if (th) {
-            unsigned short h, l;
-            split_bf16(dl, h, l);
-            th[i] = h;
-            tl[i] = l;
+        if (clist) {
+            // clamped pixels in index order: wave ballots, then the waves in order
+            const unsigned long long m = __ballot(clamped);
+            if (lane == 0)
+                wcnt[wave] = __popcll(m);
+            __syncthreads();
+            int off = base;
+            for (int w = 0; w < wave; w++)
+                off += wcnt[w];
+            if (clamped) {
+                const int pos = off + __popcll(m & ((1ull << lane) - 1ull));
+                const int y = i / W, x = i - y * W;   // W <= 512: 16 bits each
+                clist[(size_t)blockIdx.x * cper + pos] = make_int2(x | (y << 16), __float_as_int(corr));
+            }
+            for (int w = 0; w < RED_THREADS / 64; w++)
+                base += wcnt[w];
+            __syncthreads();   // wcnt is rewritten by the next pass
         }
-        const double M = model[i];
-        v[0] += (double)dl;
-        v[1] += pix_entropy((double)t, M) - pix_entropy((double)I, M);
     }
     block_sum<3>(v, sh);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int j = 0; j < 3; j++)
             cs_part[3 * blockIdx.x + j] = v[j];
+        if (ccount)
+            ccount[blockIdx.x] = base;
     }
 }
 
@@ -1519,8 +1554,11 @@ __global__ __launch_bounds__(256) void k_vis_fwd_b3(const unsigned short *__rest
 // the blocks split between two workgroups of k_vis_fwd_b3: one 64-thread block per
 // workgroup boundary, vis = (part 0 + part 1) * scale
 __global__ void k_vis_fwd_b3_combine(const float2 *__restrict__ vpart, int nuv, int W, int upw,
-                                     const float *__restrict__ scale, float2 *__restrict__ vis)
+                                     const float *__restrict__ scale, float2 *__restrict__ vis,
+                                     const int *__restrict__ run_if)
 {
+    if (run_if && *run_if == 0)
+        return;
     const int nchunk = W / B3_KC;
     const int total = (nuv + B3_PW - 1) / B3_PW * nchunk;
     const int ub = (int)(blockIdx.x + 1) * upw;
@@ -1548,11 +1586,16 @@ template __global__ void k_vis_fwd_b3<8>(const unsigned short *, const unsigned 
                                          float2 *);
 
 // explicit instantiations for the supported widths (W = 64 * NB)
-template __global__ void k_vis_fwd_f32<1>(const float *, const float2 *, const float2 *, int, const float *, float2 *);
-template __global__ void k_vis_fwd_f32<2>(const float *, const float2 *, const float2 *, int, const float *, float2 *);
-template __global__ void k_vis_fwd_f32<3>(const float *, const float2 *, const float2 *, int, const float *, float2 *);
-template __global__ void k_vis_fwd_f32<4>(const float *, const float2 *, const float2 *, int, const float *, float2 *);
-template __global__ void k_vis_fwd_f32<8>(const float *, const float2 *, const float2 *, int, const float *, float2 *);
+template __global__ void k_vis_fwd_f32<1>(const float *, const float2 *, const float2 *, int, const float *, float2 *,
+                                          const int *);
+template __global__ void k_vis_fwd_f32<2>(const float *, const float2 *, const float2 *, int, const float *, float2 *,
+                                          const int *);
+template __global__ void k_vis_fwd_f32<3>(const float *, const float2 *, const float2 *, int, const float *, float2 *,
+                                          const int *);
+template __global__ void k_vis_fwd_f32<4>(const float *, const float2 *, const float2 *, int, const float *, float2 *,
+                                          const int *);
+template __global__ void k_vis_fwd_f32<8>(const float *, const float2 *, const float2 *, int, const float *, float2 *,
+                                          const int *);
 
 // ---------------------------------------------------------------------------
 // BF16x3 DFTs on factored tables
@@ -1615,6 +1658,152 @@ struct B3Stage {
 #define FYS_MAX (512 / 16 + FT_FINE)
 #define FYS_PER ((B3_PW * FYS_MAX + 255) / 256)
 
+// The committed change's unnormalised DFT without a full DFT when few pixels clamp:
+// the change is s d plus, at each pixel the floor clamped, a correction c_j, so
+//   U_delta[p] = s U_d[p] + sum_j c_j X[p][x_j] Y[p][y_j]
+// with U_d the direction's DFT (already on the device).  k_commit_step lists the
+// clamped pixels per block in index order (ncb blocks of cper slots, ccount[b] in
+// use); a block of this kernel owns 64 uv points (lane = point) with their factor
+// rows staged in LDS, and each of its CS_WAVES waves walks a contiguous share of the list
+// (one pixel at a time, the same pixel across the wave; entries hold x | y << 16 and
+// the correction; the list is read 64 entries
+// per load and broadcast lane by lane), rebuilding X and Y as coarse x fine factors
+// and Y only when the row changes.  The waves' sums are added in wave order: the
+// result depends on the list only.  With more than cap pixels the kernel writes
+// nothing but *dense = 1, and the full DFT of the change launched behind it runs
+// (it exits at once when *dense = 0).
+#define CS_PTS 64
+// soff[b] = sum of ccount[0 .. b-1] for b <= ncb (<= 128), by one wave: an inclusive
+// scan of two counts per lane (a serial loop would wait on 128 dependent loads)
+__device__ __forceinline__ void cs_offsets(const int *__restrict__ ccount, int ncb, int *soff, int lane)
+{
+    int a = lane < ncb ? ccount[lane] : 0;
+    int b = lane + 64 < ncb ? ccount[lane + 64] : 0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int ta = __shfl_up(a, o, 64), tb = __shfl_up(b, o, 64);
+        if (lane >= o) {
+            a += ta;
+            b += tb;
+        }
+    }
+    const int ha = __shfl(a, 63, 64);
+    if (lane == 0)
+        soff[0] = 0;
+    if (lane < ncb)
+        soff[lane + 1] = a;
+    if (lane + 64 < ncb)
+        soff[lane + 65] = ha + b;
+}
+#define CS_WAVES 8
+// dynamic LDS: the 64 points' X and Y factor rows, padded (ft + 1 entries each)
+__host__ __device__ inline size_t cs_lds_bytes(int W) { return 2 * sizeof(float2) * CS_PTS * (size_t)(ft_len(W) + 1); }
+
+__global__ __launch_bounds__(64 * CS_WAVES) void k_commit_sparse(const float2 *__restrict__ Ud,
+                                                                const double *__restrict__ s_dev, float s_arg,
+                                                                const int2 *__restrict__ clist,
+                                                                const int *__restrict__ ccount, int ncb, int cper,
+                                                                const float2 *__restrict__ FX,
+                                                                const float2 *__restrict__ FY, int nuv, int W,
+                                                                int cap, float2 *__restrict__ Udel,
+                                                                int *__restrict__ dense)
+{
+    extern __shared__ float2 cs_rows[];
+    __shared__ int soff[RED_GRID_MAX + 1];
+    __shared__ float2 sacc[CS_WAVES - 1][CS_PTS];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // the same in every lane of a wave
+    if (wave == 0)
+        cs_offsets(ccount, ncb, soff, lane);
+    __syncthreads();
+    const int total = soff[ncb];
+    if (blockIdx.x == 0 && tid == 0)
+        *dense = total > cap;   // read by the full DFT launched behind this kernel
+    if (total > cap)
+        return;
+    const int ft = ft_len(W), nco = W / 16, st = ft + 1;   // padded rows: lanes hit different banks
+    float2 *sX = cs_rows, *sY = cs_rows + CS_PTS * st;
+    const int p0 = blockIdx.x * CS_PTS;
+    {
+        // CS_WAVES threads per row of 64, consecutive factors per thread group
+        const int q = tid / CS_WAVES, e0 = tid % CS_WAVES, p = p0 + q;
+        const size_t g = (size_t)(p < nuv ? p : 0) * ft;   // points past nuv: any valid row
+        for (int e = e0; e < ft; e += CS_WAVES) {
+            sX[q * st + e] = FX[g + e];
+            sY[q * st + e] = FY[g + e];
+        }
+    }
+    __syncthreads();
+    const float2 *rx = sX + lane * st, *ry = sY + lane * st;
+    const int j0 = (int)((long long)total * wave / CS_WAVES), j1 = (int)((long long)total * (wave + 1) / CS_WAVES);
+    float2 acc = make_float2(0.f, 0.f), row = make_float2(0.f, 0.f), yv = make_float2(0.f, 0.f);
+    int prev_y = -1;
+    for (int jb = j0; jb < j1; jb += 64) {
+        // this lane's entry of the next 64: its block by a search over the offsets
+        const int j = jb + lane;
+        int2 en = make_int2(0, 0);
+        if (j < j1) {
+            int lo = 0, hi = ncb;   // soff[lo] <= j < soff[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (soff[mid] <= j)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            en = clist[(size_t)lo * cper + (j - soff[lo])];
+        }
+        const int cnt = min(64, j1 - jb);
+        // the factors of pixel k are read one pixel ahead (each wave's LDS latency
+        // overlaps the previous pixel's arithmetic); Y only when the row changes
+        int xy = __builtin_amdgcn_readlane(en.x, 0);
+        float2 fxc = rx[(xy & 0xffff) >> 4], fxf = rx[nco + (xy & 15)];
+        for (int k = 0; k < cnt; k++) {
+            const int y = xy >> 16;
+            const float cj = __int_as_float(__builtin_amdgcn_readlane(en.y, k));
+            const float2 xc = fxc, xf = fxf;
+            if (k + 1 < cnt) {
+                xy = __builtin_amdgcn_readlane(en.x, k + 1);
+                fxc = rx[(xy & 0xffff) >> 4];
+                fxf = rx[nco + (xy & 15)];
+            }
+            if (y != prev_y) {
+                if (prev_y >= 0) {
+                    acc.x += row.x * yv.x - row.y * yv.y;
+                    acc.y += row.x * yv.y + row.y * yv.x;
+                }
+                yv = cmul(ry[y >> 4], ry[nco + (y & 15)]);
+                row = make_float2(0.f, 0.f);
+                prev_y = y;
+            }
+            const float2 xv = cmul(xc, xf);
+            row.x += cj * xv.x;
+            row.y += cj * xv.y;
+        }
+    }
+    if (prev_y >= 0) {
+        acc.x += row.x * yv.x - row.y * yv.y;
+        acc.y += row.x * yv.y + row.y * yv.x;
+    }
+    if (wave > 0)
+        sacc[wave - 1][lane] = acc;
+    __syncthreads();
+    const int p = p0 + lane;
+    if (wave == 0 && p < nuv) {
+        const float s = s_dev ? (float)*s_dev : s_arg;
+        const float2 u = Ud[p];
+        float sx = acc.x, sy = acc.y;
+#pragma unroll
+        for (int w = 0; w < CS_WAVES - 1; w++) {
+            sx += sacc[w][lane].x;
+            sy += sacc[w][lane].y;
+        }
+        Udel[p] = make_float2(s * u.x + sx, s * u.y + sy);
+    }
+}
+
+
+
 struct FyStage {
     float2 v[FYS_PER];
 };
@@ -1649,8 +1838,10 @@ __global__ __launch_bounds__(256) void k_vis_fwd_b3f(const unsigned short *__res
                                                         const float2 *__restrict__ FX,
                                                         const float2 *__restrict__ FY, int nuv, int W, int upw,
                                                         const float *__restrict__ scale, float2 *__restrict__ vis,
-                                                        float2 *__restrict__ vpart)
+                                                        float2 *__restrict__ vpart, const int *__restrict__ run_if)
 {
+    if (run_if && *run_if == 0)   // a commit whose change DFT the sparse correction already gave
+        return;
     constexpr int ROWS = NBK * 32;
     constexpr int CH = ROWS * (B3_KC / 8);
     constexpr int PER = (CH + 255) / 256;
@@ -1809,13 +2000,17 @@ __global__ __launch_bounds__(256) void k_vis_fwd_b3f(const unsigned short *__res
 }
 
 template __global__ void k_vis_fwd_b3f<2>(const unsigned short *, const unsigned short *, const float2 *,
-                                          const float2 *, int, int, int, const float *, float2 *, float2 *);
+                                          const float2 *, int, int, int, const float *, float2 *, float2 *,
+                                          const int *);
 template __global__ void k_vis_fwd_b3f<4>(const unsigned short *, const unsigned short *, const float2 *,
-                                          const float2 *, int, int, int, const float *, float2 *, float2 *);
+                                          const float2 *, int, int, int, const float *, float2 *, float2 *,
+                                          const int *);
 template __global__ void k_vis_fwd_b3f<6>(const unsigned short *, const unsigned short *, const float2 *,
-                                          const float2 *, int, int, int, const float *, float2 *, float2 *);
+                                          const float2 *, int, int, int, const float *, float2 *, float2 *,
+                                          const int *);
 template __global__ void k_vis_fwd_b3f<8>(const unsigned short *, const unsigned short *, const float2 *,
-                                          const float2 *, int, int, int, const float *, float2 *, float2 *);
+                                          const float2 *, int, int, int, const float *, float2 *, float2 *,
+                                          const int *);
 
 // Adjoint as k_adj_b3w (256 x 256 quadrant per workgroup and uv split, 8 waves of
 // 2 x 4 accumulator tiles, LDS double-buffered per round of 16 points), with both

@@ -220,6 +220,8 @@ typedef struct {
     int stalled;           /* no decrease found along the direction */
     int status;
     int resyncs;           /* exact re-evaluations of the tracked objective (their DFTs are not in forward_dfts) */
+    int sparse_commits;    /* commits whose change DFT was s U_d plus the correction over the clamped pixels
+                              (counted in forward_dfts: full forward DFTs = forward_dfts - sparse_commits) */
 } gpair_pcg_result;
 
 void gpair_pcg_default_params(gpair_pcg_params *p);
@@ -228,6 +230,13 @@ void gpair_pcg_default_params(gpair_pcg_params *p);
  * (the last three cumulative). */
 int gpair_pcg_run(gpair_ctx *ctx, const gpair_pcg_params *params, gpair_pcg_result *result, double *trace,
                   int trace_rows);
+/* No reference counterpart.  A commit's trial image is max(I + s d, floor), so its
+ * change is s d plus a correction c_j at the pixels the floor clamps; its DFT is then
+ * taken as s U_d (the direction's DFT, at hand) plus sum_j c_j X[p][x_j] Y[p][y_j]
+ * when at most max_pixels pixels clamp (tables set from uv only), else by a full
+ * forward DFT (the choice is made on the device).  -1 (default): W*W/40, about
+ * where the correction costs as much as the full DFT; 0: always the full DFT. */
+int gpair_set_sparse_commit(gpair_ctx *ctx, int max_pixels);
 /* per-pixel preconditioner factor [W*W] multiplying P (NULL clears it) */
 int gpair_set_pfactor(gpair_ctx *ctx, const float *pfactor, int size);
 /* on != 0: gpair_pcg_run replays each iteration's gradient phase (flux, adjoint

@@ -163,3 +163,40 @@ def test_batch_workload_64_problems(orc):
         worst = max(worst, abs(r.J - J_o) / abs(J_o))
     print(f"worst J rel err vs FP64 over {len(probs)} problems: {worst:.2e}", flush=True)
     assert worst <= 1e-5
+
+
+@pytest.mark.parametrize("prec", [1, 0], ids=["bf16x3", "fp32"])
+def test_sparse_commits_against_full_dft(orc, cfg2, prec):
+    """commits whose change DFT is s U_d plus the correction over the clamped pixels
+    (the default) against the same loop with every commit taking the full forward DFT
+    of the change: the same branches (DFT and batch counts), objective traces within
+    1e-5 over 8 iterations (5e-4 over 30, as the trajectory amplifies rounding), and
+    the tracked objective of the image after 30 iterations as close to its
+    FP64 value as with full DFTs (within 2x + 1e-8)"""
+    prob = cfg2
+    P = orc.unpack(prob)
+    alpha = prob.meta["alpha"]
+    N = 30
+    out = {}
+    for cap in (-1, 0):
+        with GpairContext.from_problem(prob) as ctx:
+            ctx.set_precision(prec)
+            ctx.set_sparse_commit(cap)
+            r1, tr = ctx.pcg_run(alpha, precond=1, max_iter=N + 1, chi2r_target=0.0)
+        with GpairContext.from_problem(prob) as ctx:
+            ctx.set_precision(prec)
+            ctx.set_sparse_commit(cap)
+            r0, _ = ctx.pcg_run(alpha, precond=1, max_iter=N, chi2r_target=0.0)
+            img = ctx.get("ci").astype(np.float64)
+        J64 = orc.objective(P, img, alpha)[0]
+        out[cap] = (tr, r1, abs(tr[N, 0] - J64) / abs(J64))
+        print(f"cap {cap}: sparse commits {r1.sparse_commits} forward {r1.forward_dfts} "
+              f"tracked J err {out[cap][2]:.2e}", flush=True)
+    (ts, rs, es), (td, rd, ed) = out[-1], out[0]
+    assert rs.sparse_commits > 0 and rd.sparse_commits == 0
+    assert np.array_equal(ts[:, 5:8], td[:, 5:8])
+    rel = np.abs(ts[:, 0] - td[:, 0]) / np.abs(td[:, 0])
+    print("trace rel diff sparse vs full", rel, flush=True)
+    # rounding-level differences of the change's DFT, amplified along the trajectory
+    assert rel[:9].max() <= 1e-5 and rel.max() <= 5e-4
+    assert es <= 2.0 * ed + 1e-8
