@@ -111,9 +111,11 @@ def read_traffic(name, kernel):
     seen = False
     for line in open(path):
         if line.startswith(kernel + ":"):
-            seen = True
+            seen = "not dispatched" not in line
         elif seen and line.startswith("traffic_bytes_per_launch"):
             return float(line.split("=")[1])
+        elif not line.strip():
+            seen = False   # each kernel's figures end at a blank line
     return None
 
 
@@ -509,7 +511,7 @@ def main():
     table_bytes = 16.0 * nuv * W if prec == 0 else 16.0 * nuv * (W // 16 + 16)
     nbytes = table_bytes + (2.0 if prec == 2 else 4.0) * W * W + 8.0 * nuv
     intensity = flop / nbytes
-    traffic = read_traffic(f"forward_dft_W{W}_{a.precision}_pmc.txt", kname)
+    traffic = read_traffic(f"hbm_traffic_W{W}_{a.precision}_pmc.txt", kname)
     if intensity < PEAK[a.precision] * 1e12 / HBM_PEAK:
         # below the machine balance: the HBM roofline binds (BF16x3)
         roofline = {"bound": "hbm", "achieved": nbytes / avg_fwd / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -522,8 +524,10 @@ def main():
                      "bytes_per_launch": nbytes, "flop_per_byte": intensity,
                      "mfma_pipe_tflops": achieved * PRODUCTS[a.precision],
                      "mfma_pipe_frac": achieved * PRODUCTS[a.precision] / PEAK[a.precision]})
-    aname = {0: "k_adj_f32", 1: "k_adj_b3wf", 2: "k_adj_h1wf"}[prec] if W % 256 == 0 else (
+    aname = {0: "k_adj_f32", 1: "k_adj_b3wf<256, 256>", 2: "k_adj_h1wf<256, 256>"}[prec] if W % 256 == 0 else (
         "k_adj_b3" if prec == 1 else "k_adj_f32")
+    adj_traffic = read_traffic(f"hbm_traffic_W{W}_{a.precision}_pmc.txt", aname)
+    fin_traffic = read_traffic(f"hbm_traffic_W{W}_{a.precision}_pmc.txt", "k_grad_finish_crit")
     out = {
         "metric": METRIC,
         "value": steps_done / dt,
@@ -544,7 +548,11 @@ def main():
         "roofline": roofline,
         "adjoint": {"kernel": aname, "avg_us": adj_ms / max(adj_n, 1) * 1e3, "launches": adj_n,
                     "timed_every": TIMING_EVERY,
-                    "achieved_tflops": flop / (adj_ms / max(adj_n, 1) * 1e-3) / 1e12 if adj_n else None},
+                    "achieved_tflops": flop / (adj_ms / max(adj_n, 1) * 1e-3) / 1e12 if adj_n else None,
+                    # HBM bytes per launch from the committed rocprofv3 --pmc summary: the adjoint
+                    # (its split-K partials included) and the kernel that adds the partials up
+                    "traffic": adj_traffic, "finish_traffic": fin_traffic,
+                    "bytes_per_launch": table_bytes + 8.0 * nuv + 4.0 * W * W},
         "autotune": None if tuned is None else dict(zip(
             ("ls_chi2_blocks", "ls_entropy_blocks", "coef_blocks", "fwd_bf16x3_rows_per_group_32"), tuned)),
         "dfts_per_iter": {"forward": (res.forward_dfts) / max(steps_done, 1),

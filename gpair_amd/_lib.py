@@ -110,6 +110,7 @@ SIGNATURES = {
     "gpair_autotune": (_i, [_vp, _i, _vp, _i]),
     "gpair_set_graphs": (_i, [_vp, _i]),
     "gpair_set_sparse_commit": (_i, [_vp, _i]),
+    "gpair_set_adjoint_tile": (_i, [_vp, _i, _i, _i]),
     "gpair_pcg_run_checkpointed": (_i, [_vp, C.POINTER(PcgParams), C.POINTER(PcgResult), _vp, _i, C.c_char_p,
                                         C.c_char_p]),
     "gpair_set_snapshot": (_i, [_vp, C.c_char_p, _i, _f]),

@@ -339,8 +339,13 @@ class GpairContext:
                                               float(pixsize)), "gpair_set_snapshot")
 
     def set_sparse_commit(self, max_pixels=-1):
-        """gpair_set_sparse_commit: -1 default cap (W^2/16), 0 always the full DFT"""
+        """gpair_set_sparse_commit: -1 default cap (W^2/40), 0 always the full DFT"""
         self._check(self.L.gpair_set_sparse_commit(self.h, int(max_pixels)), "gpair_set_sparse_commit")
+
+    def set_adjoint_tile(self, rows, cols, workgroups=0):
+        """gpair_set_adjoint_tile: 256 x 256, 128 x 256 or 128 x 128; about `workgroups` in total (0: 256)"""
+        self._check(self.L.gpair_set_adjoint_tile(self.h, int(rows), int(cols), int(workgroups)),
+                    "gpair_set_adjoint_tile")
 
     def set_graphs(self, on=True):
         self._check(self.L.gpair_set_graphs(self.h, int(bool(on))), "gpair_set_graphs")

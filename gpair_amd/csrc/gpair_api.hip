@@ -79,6 +79,8 @@ struct gpair_ctx {
     int *dense;   // set by the sparse kernel: 1 when the commit takes the full DFT
     int clist_cper;
     int commit_cap;
+    int adj_tr, adj_tc;   // output tile of the factored split-precision adjoints (rows x columns)
+    int adj_groups;       // their workgroups in total (tiles x uv splits), about
     // single-pixel moves: state image, FP64 unnormalised visibilities, x-major
     // tables, trial visibilities and mock data, per-move buffers
     float *px_img;
@@ -203,6 +205,9 @@ extern "C" int gpu_init(gpair_ctx **out, int device)
     c->device = device;
     c->precision = GPAIR_PREC_FP32;
     c->commit_cap = -1;
+    c->adj_tr = ADJ_TILE_R;
+    c->adj_tc = ADJ_TILE_C;
+    c->adj_groups = ADJ_WIDE_GROUPS;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return GPAIR_E_HIP;
@@ -519,9 +524,13 @@ extern "C" int gpu_copy_dft(gpair_ctx *c, const float *dft_x, const float *dft_y
 // the split-precision adjoint takes 256 x 256 quadrants (k_adj_b3w) when the width allows
 static inline int adj_wide(const gpair_ctx *c) { return ADJ_WIDE && c->W % AW_T == 0; }
 
+static inline int adj_factored(const gpair_ctx *c);
+
 static int set_adj_split(gpair_ctx *c, int b3, int groups)
 {
-    const int tiles = b3 ? (adj_wide(c) ? (c->W / AW_T) * (c->W / AW_T) : (c->W / AB3_YT) * (c->W / AB3_XT))
+    const int tiles = b3 ? (adj_factored(c) ? (c->W / c->adj_tr) * (c->W / c->adj_tc)
+                            : adj_wide(c)   ? (c->W / AW_T) * (c->W / AW_T)
+                                            : (c->W / AB3_YT) * (c->W / AB3_XT))
                          : (c->W / 64) * (c->W / 64);
     const int gran = b3 ? AB3_PR : 8;
     int nsplit = groups / tiles;
@@ -615,7 +624,8 @@ static int ensure_split_tables(gpair_ctx *c)
             dfree(&c->Xth);
             dfree(&c->Xtl);
         }
-        if (!c->adj_b3_nsplit && (rc = set_adj_split(c, 1, adj_wide(c) ? ADJ_WIDE_GROUPS : 512)))
+        if (!c->adj_b3_nsplit &&
+            (rc = set_adj_split(c, 1, adj_factored(c) ? c->adj_groups : adj_wide(c) ? ADJ_WIDE_GROUPS : 512)))
             return rc;
     }
     c->split_valid = 1;
@@ -908,19 +918,36 @@ static int launch_adjoint_dft(gpair_ctx *c, int *nsplit_out)
                                                                          2 * c->nuv, c->part, c->ticket,
                                                                          c->scal + SC_AMAXC);
         LAUNCHCHK(c);
-        dim3 grid((c->W / AW_T) * (c->W / AW_T), nsplit);
-        TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,
-                     (k_adj_h1wf<<<grid, 512, 0, c->stream>>>(c->coef, c->FX, c->FY, c->nuv, c->W, c->adj_b3_pper,
-                                                              c->scal + SC_AMAXC, c->adj_part)));
+        const unsigned grid = (unsigned)((c->W / c->adj_tr) * (c->W / c->adj_tc) * nsplit);
+#define ADJ_H1(TR, TC)                                                                                              \
+    TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,                                                                               \
+                 (k_adj_h1wf<TR, TC><<<grid, 512, 0, c->stream>>>(c->coef, c->FX, c->FY, c->nuv, c->W,              \
+                                                                  c->adj_b3_pper, nsplit, c->scal + SC_AMAXC,       \
+                                                                  c->adj_part)))
+        if (c->adj_tr == 256 && c->adj_tc == 256)
+            ADJ_H1(256, 256);
+        else if (c->adj_tr == 128 && c->adj_tc == 256)
+            ADJ_H1(128, 256);
+        else
+            ADJ_H1(128, 128);
+#undef ADJ_H1
     } else if (c->precision == GPAIR_PREC_BF16X3 && c->W % AB3_XT == 0) {
         if ((rc = ensure_split_tables(c)))
             return rc;
         nsplit = c->adj_b3_nsplit;
         if (adj_factored(c)) {
-            dim3 grid((c->W / AW_T) * (c->W / AW_T), nsplit);
-            TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,
-                         (k_adj_b3wf<<<grid, 512, 0, c->stream>>>(c->coef, c->FX, c->FY, c->nuv, c->W,
-                                                                  c->adj_b3_pper, c->adj_part)));
+            const unsigned grid = (unsigned)((c->W / c->adj_tr) * (c->W / c->adj_tc) * nsplit);
+#define ADJ_B3(TR, TC)                                                                                              \
+    TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,                                                                               \
+                 (k_adj_b3wf<TR, TC><<<grid, 512, 0, c->stream>>>(c->coef, c->FX, c->FY, c->nuv, c->W,              \
+                                                                  c->adj_b3_pper, nsplit, c->adj_part)))
+            if (c->adj_tr == 256 && c->adj_tc == 256)
+                ADJ_B3(256, 256);
+            else if (c->adj_tr == 128 && c->adj_tc == 256)
+                ADJ_B3(128, 256);
+            else
+                ADJ_B3(128, 128);
+#undef ADJ_B3
         } else if (adj_wide(c)) {
             dim3 grid((c->W / AW_T) * (c->W / AW_T), nsplit);
             TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,
@@ -1246,6 +1273,24 @@ extern "C" int gpair_set_sparse_commit(gpair_ctx *c, int max_pixels)
     if (max_pixels < -1)
         return fail(c, GPAIR_E_ARG, "gpair_set_sparse_commit: max_pixels %d (-1, 0 or a count)", max_pixels);
     c->commit_cap = max_pixels;
+    return GPAIR_OK;
+}
+
+extern "C" int gpair_set_adjoint_tile(gpair_ctx *c, int rows, int cols, int workgroups)
+{
+    if (!c)
+        return GPAIR_E_ARG;
+    if (!((rows == 256 && cols == 256) || (rows == 128 && cols == 256) || (rows == 128 && cols == 128)))
+        return fail(c, GPAIR_E_ARG, "gpair_set_adjoint_tile: %d x %d (256 x 256, 128 x 256 or 128 x 128)", rows,
+                    cols);
+    if (workgroups < 0 || workgroups > 65536)
+        return fail(c, GPAIR_E_ARG, "gpair_set_adjoint_tile: %d workgroups (0: the default, up to 65536)",
+                    workgroups);
+    c->adj_tr = rows;
+    c->adj_tc = cols;
+    c->adj_groups = workgroups ? workgroups : ADJ_WIDE_GROUPS;
+    c->adj_b3_nsplit = 0;   // the split follows the tile: set again on the next split-precision adjoint
+    c->split_valid = 0;
     return GPAIR_OK;
 }
 

@@ -657,6 +657,10 @@ __global__ __launch_bounds__(256) void k_adj_b3(const float2 *__restrict__ coef,
 #ifndef ADJ_WIDE_GROUPS
 #define ADJ_WIDE_GROUPS 256          // workgroups of k_adj_b3w (quadrants x uv splits)
 #endif
+#ifndef ADJ_TILE_R
+#define ADJ_TILE_R 256               // output tile of k_adj_b3wf / k_adj_h1wf (rows x columns)
+#define ADJ_TILE_C 256
+#endif
 
 __global__ __launch_bounds__(512) void k_adj_b3w(const float2 *__restrict__ coef, const float2 *__restrict__ Y,
                                                  const unsigned short *__restrict__ Xth,
@@ -2012,44 +2016,40 @@ template __global__ void k_vis_fwd_b3f<8>(const unsigned short *, const unsigned
                                           const float2 *, int, int, int, const float *, float2 *, float2 *,
                                           const int *);
 
-// Adjoint as k_adj_b3w (256 x 256 quadrant per workgroup and uv split, 8 waves of
-// 2 x 4 accumulator tiles, LDS double-buffered per round of 16 points), with both
-// tiles built from the factored tables.  Builder role of thread t: point q = t % 16
-// of the round and the row (A) and column (B) runs 4 (t / 16) + 128 h .. + 3,
-// h = 0, 1: the four entries of a run share one coarse factor and the two runs
+// Adjoint as k_adj_b3w with both tiles built from the factored tables, for output
+// tiles of TR x TC pixels (TR, TC in {128, 256}) per workgroup and uv split: 8 waves
+// of TR / 128 x TC / 64 accumulator tiles (wave w owns rows TR / 4 (w / 2) .. and
+// columns TC / 2 (w % 2) ..), LDS double-buffered per round of 16 points.  Smaller
+// tiles need fewer uv splits to fill the chip (the partials the finish kernel adds
+// up shrink with the splits) at more builder work per MFMA.  The grid is 1-D,
+// block b = split b % nsplit of tile b / nsplit, so that the tiles of one split
+// run on one XCD (workgroups go to the XCDs in turn) and share its L2 for the
+// split's factors.  Builder role of thread t: point q = t % 16 of the round and
+// the row (A) and column (B) runs 4 (t / 16) + 128 h .. + 3, h < TR / 128 (A),
+// h < TC / 128 (B): the four entries of a run share one coarse factor and the runs
 // the same four fine factors.  In a wave the 16 points are consecutive 32-bit
 // words and the four runs start 80 words apart (bank offsets 0, 16, 32, 48), so
 // the builders' LDS writes are free of bank conflicts.
-// Three stages per round k: the MFMAs of round k read their tile, the tile of
-// round k + 1 is built from factor loads issued at the end of the previous round
-// (in the same basic block as the MFMAs, so that the builder's VALU work can fill
-// the MFMA pipeline's shadow), then the loads of round k + 2 are issued into the
-// same registers.  Rounds past the last reload the last round (valid addresses,
-// tiles never read).
-// Adjoint as k_adj_b3w (256 x 256 quadrant per workgroup and uv split, 8 waves of
-// 2 x 4 accumulator tiles, LDS double-buffered per round of 16 points), with both
-// tiles built from the factored tables.  Builder role of thread t: point q = t % 16
-// of the round and the row (A) and column (B) runs 4 (t / 16) + 128 h .. + 3,
-// h = 0, 1: the four entries of a run share one coarse factor and the two runs
-// the same four fine factors.  In a wave the 16 points are consecutive 32-bit
-// words and the four runs start 80 words apart (bank offsets 0, 16, 32, 48), so
-// the builders' LDS writes are free of bank conflicts.
+template <int TR, int TC>
 __global__ __launch_bounds__(512) void k_adj_b3wf(const float2 *__restrict__ coef, const float2 *__restrict__ FX,
                                                   const float2 *__restrict__ FY, int nuv, int W, int p_per_split,
-                                                  float *__restrict__ part)
+                                                  int nsplit, float *__restrict__ part)
 {
-    __shared__ __attribute__((aligned(16))) unsigned short sA[2][2][AW_T * AW_STRIDE];   // [buf][hi/lo]
-    __shared__ __attribute__((aligned(16))) unsigned short sB[2][2][AW_T * AW_STRIDE];
+    static_assert((TR == 128 || TR == 256) && (TC == 128 || TC == 256), "tile");
+    constexpr int HA = TR / 128, HB = TC / 128, MT = TR / 128, NT = TC / 64;
+    __shared__ __attribute__((aligned(16))) unsigned short sA[2][2][TR * AW_STRIDE];   // [buf][hi/lo]
+    __shared__ __attribute__((aligned(16))) unsigned short sB[2][2][TC * AW_STRIDE];
     constexpr int WROW = AW_STRIDE / 2;   // 32-bit words per LDS row
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lr = lane & 31, lh = lane >> 5;
     const int ft = ft_len(W), nco = W / 16;
-    const int qx = W / AW_T;
-    const int y0 = (blockIdx.x / qx) * AW_T, x0 = (blockIdx.x % qx) * AW_T;
-    const int pstart = blockIdx.y * p_per_split;
+    const int split = blockIdx.x % nsplit, tile = blockIdx.x / nsplit;
+    const int qx = W / TC;
+    const int y0 = (tile / qx) * TR, x0 = (tile % qx) * TC;
+    const int pstart = split * p_per_split;
     const int pend = min(pstart + p_per_split, nuv);
     const int q = tid & 15, o4 = 4 * (tid >> 4);
-    float2 ca, cy0, cy1, cx0, cx1;
+    float2 ca, cy[HA], cx[HB];
     float4 fy01, fy23, fx01, fx23;
     auto load = [&](int pr) {
         // points past the split get a = 0 (their entries, built from the split's last
@@ -2066,10 +2066,12 @@ __global__ __launch_bounds__(512) void k_adj_b3wf(const float2 *__restrict__ coe
         fy23 = fyf[1];
         fx01 = fxf[0];
         fx23 = fxf[1];
-        cy0 = fy[(y0 + o4) >> 4];
-        cy1 = fy[(y0 + o4 + 128) >> 4];
-        cx0 = fx[(x0 + o4) >> 4];
-        cx1 = fx[(x0 + o4 + 128) >> 4];
+#pragma unroll
+        for (int h = 0; h < HA; h++)
+            cy[h] = fy[(y0 + o4 + 128 * h) >> 4];
+#pragma unroll
+        for (int h = 0; h < HB; h++)
+            cx[h] = fx[(x0 + o4 + 128 * h) >> 4];
     };
     auto store = [&](int buf) {
         unsigned *ah = reinterpret_cast<unsigned *>(sA[buf][0]), *al = reinterpret_cast<unsigned *>(sA[buf][1]);
@@ -2079,9 +2081,9 @@ __global__ __launch_bounds__(512) void k_adj_b3wf(const float2 *__restrict__ coe
         const float2 fxv[4] = {make_float2(fx01.x, fx01.y), make_float2(fx01.z, fx01.w), make_float2(fx23.x, fx23.y),
                                make_float2(fx23.z, fx23.w)};
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int h = 0; h < HA; h++) {
             // A[y][2q] = Re(a Y[p][y]), A[y][2q + 1] = -Im(a Y[p][y])
-            const float2 acy = cmul(ca, h ? cy1 : cy0);
+            const float2 acy = cmul(ca, cy[h]);
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const float2 w = cmul(acy, fyv[j]);
@@ -2093,12 +2095,11 @@ __global__ __launch_bounds__(512) void k_adj_b3wf(const float2 *__restrict__ coe
             }
         }
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int h = 0; h < HB; h++) {
             // B[2q][x] = Re X[p][x], B[2q + 1][x] = Im X[p][x], stored by column x
-            const float2 cxh = h ? cx1 : cx0;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const float2 w = cmul(cxh, fxv[j]);
+                const float2 w = cmul(cx[h], fxv[j]);
                 unsigned hh, ll;
                 split2_bf16(w.x, w.y, hh, ll);
                 const int o = (o4 + 128 * h + j) * WROW + q;
@@ -2107,12 +2108,12 @@ __global__ __launch_bounds__(512) void k_adj_b3wf(const float2 *__restrict__ coe
             }
         }
     };
-    const int wy = (wave >> 1) * 64, wx = (wave & 1) * 128;
-    v16f acc[2][4];
+    const int wy = (wave >> 1) * (TR / 4), wx = (wave & 1) * (TC / 2);
+    v16f acc[MT][NT];
 #pragma unroll
-    for (int m = 0; m < 2; m++)
+    for (int m = 0; m < MT; m++)
 #pragma unroll
-        for (int n = 0; n < 4; n++)
+        for (int n = 0; n < NT; n++)
 #pragma unroll
             for (int r = 0; r < 16; r++)
                 acc[m][n][r] = 0.f;
@@ -2128,34 +2129,34 @@ __global__ __launch_bounds__(512) void k_adj_b3wf(const float2 *__restrict__ coe
             load(pr + AB3_PR);
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) {
-            bf16x8 ah[2], al[2], bh[4], bl[4];
+            bf16x8 ah[MT], al[MT], bh[NT], bl[NT];
 #pragma unroll
-            for (int m = 0; m < 2; m++) {
+            for (int m = 0; m < MT; m++) {
                 const int o = (wy + 32 * m + lr) * AW_STRIDE + 16 * ks + 8 * lh;
                 ah[m] = *reinterpret_cast<const bf16x8 *>(&sA[buf][0][o]);
                 al[m] = *reinterpret_cast<const bf16x8 *>(&sA[buf][1][o]);
             }
 #pragma unroll
-            for (int n = 0; n < 4; n++) {
+            for (int n = 0; n < NT; n++) {
                 const int o = (wx + 32 * n + lr) * AW_STRIDE + 16 * ks + 8 * lh;
                 bh[n] = *reinterpret_cast<const bf16x8 *>(&sB[buf][0][o]);
                 bl[n] = *reinterpret_cast<const bf16x8 *>(&sB[buf][1][o]);
             }
-            // product-major over the eight accumulators (same per-tile order hh, hl, lh)
+            // product-major over the accumulators (same per-tile order hh, hl, lh)
 #pragma unroll
-            for (int m = 0; m < 2; m++)
+            for (int m = 0; m < MT; m++)
 #pragma unroll
-                for (int n = 0; n < 4; n++)
+                for (int n = 0; n < NT; n++)
                     acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m], bh[n], acc[m][n], 0, 0, 0);
 #pragma unroll
-            for (int m = 0; m < 2; m++)
+            for (int m = 0; m < MT; m++)
 #pragma unroll
-                for (int n = 0; n < 4; n++)
+                for (int n = 0; n < NT; n++)
                     acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m], bl[n], acc[m][n], 0, 0, 0);
 #pragma unroll
-            for (int m = 0; m < 2; m++)
+            for (int m = 0; m < MT; m++)
 #pragma unroll
-                for (int n = 0; n < 4; n++)
+                for (int n = 0; n < NT; n++)
                     acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[m], bh[n], acc[m][n], 0, 0, 0);
         }
         // the other buffer was last read in the previous round, before its barrier
@@ -2164,17 +2165,24 @@ __global__ __launch_bounds__(512) void k_adj_b3wf(const float2 *__restrict__ coe
         __syncthreads();
         buf ^= 1;
     }
-    float *out = part + (size_t)blockIdx.y * W * W;
+    float *out = part + (size_t)split * W * W;
 #pragma unroll
-    for (int m = 0; m < 2; m++)
+    for (int m = 0; m < MT; m++)
 #pragma unroll
-        for (int n = 0; n < 4; n++)
+        for (int n = 0; n < NT; n++)
 #pragma unroll
             for (int r = 0; r < 16; r++) {
                 const int y = y0 + wy + 32 * m + 8 * (r >> 2) + 4 * lh + (r & 3);
                 out[(size_t)y * W + x0 + wx + 32 * n + lr] = acc[m][n][r];
             }
 }
+
+#define ADJ_TILE_INST(K, TR, TC)                                                                                   \
+    template __global__ void K<TR, TC>(const float2 *, const float2 *, const float2 *, int, int, int, int, float *);
+ADJ_TILE_INST(k_adj_b3wf, 256, 256)
+ADJ_TILE_INST(k_adj_b3wf, 128, 256)
+ADJ_TILE_INST(k_adj_b3wf, 128, 128)
+#undef ADJ_TILE_INST
 
 // ---------------------------------------------------------------------------
 // FP16 with scaling: one FP16 MFMA product per multiply-add
@@ -2426,26 +2434,31 @@ template __global__ void k_vis_fwd_h1f<6>(const unsigned short *, const float2 *
 template __global__ void k_vis_fwd_h1f<8>(const unsigned short *, const float2 *, const float2 *, int, int, int,
                                           const float *, const double *, float2 *, float2 *);
 
-// Adjoint in FP16: k_adj_b3wf's quadrants, rounds and builder roles with one plane
+// Adjoint in FP16: k_adj_b3wf's tiles, grid, rounds and builder roles with one plane
 // per tile and one MFMA per k-step and tile; the coefficients scaled by
 // f16_scale(*amax) (max over their Re and Im parts) and the partials scaled back
+template <int TR, int TC>
 __global__ __launch_bounds__(512) void k_adj_h1wf(const float2 *__restrict__ coef, const float2 *__restrict__ FX,
                                                   const float2 *__restrict__ FY, int nuv, int W, int p_per_split,
-                                                  const double *__restrict__ amax, float *__restrict__ part)
+                                                  int nsplit, const double *__restrict__ amax,
+                                                  float *__restrict__ part)
 {
-    __shared__ __attribute__((aligned(16))) unsigned short sA[2][AW_T * AW_STRIDE];
-    __shared__ __attribute__((aligned(16))) unsigned short sB[2][AW_T * AW_STRIDE];
+    static_assert((TR == 128 || TR == 256) && (TC == 128 || TC == 256), "tile");
+    constexpr int HA = TR / 128, HB = TC / 128, MT = TR / 128, NT = TC / 64;
+    __shared__ __attribute__((aligned(16))) unsigned short sA[2][TR * AW_STRIDE];
+    __shared__ __attribute__((aligned(16))) unsigned short sB[2][TC * AW_STRIDE];
     constexpr int WROW = AW_STRIDE / 2;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lr = lane & 31, lh = lane >> 5;
     const int ft = ft_len(W), nco = W / 16;
-    const int qx = W / AW_T;
-    const int y0 = (blockIdx.x / qx) * AW_T, x0 = (blockIdx.x % qx) * AW_T;
-    const int pstart = blockIdx.y * p_per_split;
+    const int split = blockIdx.x % nsplit, tile = blockIdx.x / nsplit;
+    const int qx = W / TC;
+    const int y0 = (tile / qx) * TR, x0 = (tile % qx) * TC;
+    const int pstart = split * p_per_split;
     const int pend = min(pstart + p_per_split, nuv);
     const int q = tid & 15, o4 = 4 * (tid >> 4);
     const float sc = f16_scale(*amax), unscale = 1.0f / sc;
-    float2 ca, cy0, cy1, cx0, cx1;
+    float2 ca, cy[HA], cx[HB];
     float4 fy01, fy23, fx01, fx23;
     auto load = [&](int pr) {
         const int p = pr + q;
@@ -2460,10 +2473,12 @@ __global__ __launch_bounds__(512) void k_adj_h1wf(const float2 *__restrict__ coe
         fy23 = fyf[1];
         fx01 = fxf[0];
         fx23 = fxf[1];
-        cy0 = fy[(y0 + o4) >> 4];
-        cy1 = fy[(y0 + o4 + 128) >> 4];
-        cx0 = fx[(x0 + o4) >> 4];
-        cx1 = fx[(x0 + o4 + 128) >> 4];
+#pragma unroll
+        for (int h = 0; h < HA; h++)
+            cy[h] = fy[(y0 + o4 + 128 * h) >> 4];
+#pragma unroll
+        for (int h = 0; h < HB; h++)
+            cx[h] = fx[(x0 + o4 + 128 * h) >> 4];
     };
     auto store = [&](int buf) {
         unsigned *aw = reinterpret_cast<unsigned *>(sA[buf]), *bw = reinterpret_cast<unsigned *>(sB[buf]);
@@ -2472,8 +2487,8 @@ __global__ __launch_bounds__(512) void k_adj_h1wf(const float2 *__restrict__ coe
         const float2 fxv[4] = {make_float2(fx01.x, fx01.y), make_float2(fx01.z, fx01.w), make_float2(fx23.x, fx23.y),
                                make_float2(fx23.z, fx23.w)};
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const float2 acy = cmul(ca, h ? cy1 : cy0);
+        for (int h = 0; h < HA; h++) {
+            const float2 acy = cmul(ca, cy[h]);
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const float2 w = cmul(acy, fyv[j]);
@@ -2481,21 +2496,20 @@ __global__ __launch_bounds__(512) void k_adj_h1wf(const float2 *__restrict__ coe
             }
         }
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const float2 cxh = h ? cx1 : cx0;
+        for (int h = 0; h < HB; h++) {
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const float2 w = cmul(cxh, fxv[j]);
+                const float2 w = cmul(cx[h], fxv[j]);
                 bw[(o4 + 128 * h + j) * WROW + q] = pack_f16(w.x, w.y);
             }
         }
     };
-    const int wy = (wave >> 1) * 64, wx = (wave & 1) * 128;
-    v16f acc[2][4];
+    const int wy = (wave >> 1) * (TR / 4), wx = (wave & 1) * (TC / 2);
+    v16f acc[MT][NT];
 #pragma unroll
-    for (int m = 0; m < 2; m++)
+    for (int m = 0; m < MT; m++)
 #pragma unroll
-        for (int n = 0; n < 4; n++)
+        for (int n = 0; n < NT; n++)
 #pragma unroll
             for (int r = 0; r < 16; r++)
                 acc[m][n][r] = 0.f;
@@ -2511,17 +2525,17 @@ __global__ __launch_bounds__(512) void k_adj_h1wf(const float2 *__restrict__ coe
             load(pr + AB3_PR);
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) {
-            f16x8 a[2], b[4];
+            f16x8 a[MT], b[NT];
 #pragma unroll
-            for (int m = 0; m < 2; m++)
+            for (int m = 0; m < MT; m++)
                 a[m] = *reinterpret_cast<const f16x8 *>(&sA[buf][(wy + 32 * m + lr) * AW_STRIDE + 16 * ks + 8 * lh]);
 #pragma unroll
-            for (int n = 0; n < 4; n++)
+            for (int n = 0; n < NT; n++)
                 b[n] = *reinterpret_cast<const f16x8 *>(&sB[buf][(wx + 32 * n + lr) * AW_STRIDE + 16 * ks + 8 * lh]);
 #pragma unroll
-            for (int m = 0; m < 2; m++)
+            for (int m = 0; m < MT; m++)
 #pragma unroll
-                for (int n = 0; n < 4; n++)
+                for (int n = 0; n < NT; n++)
                     acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[m], b[n], acc[m][n], 0, 0, 0);
         }
         if (more)
@@ -2529,14 +2543,21 @@ __global__ __launch_bounds__(512) void k_adj_h1wf(const float2 *__restrict__ coe
         __syncthreads();
         buf ^= 1;
     }
-    float *out = part + (size_t)blockIdx.y * W * W;
+    float *out = part + (size_t)split * W * W;
 #pragma unroll
-    for (int m = 0; m < 2; m++)
+    for (int m = 0; m < MT; m++)
 #pragma unroll
-        for (int n = 0; n < 4; n++)
+        for (int n = 0; n < NT; n++)
 #pragma unroll
             for (int r = 0; r < 16; r++) {
                 const int y = y0 + wy + 32 * m + 8 * (r >> 2) + 4 * lh + (r & 3);
                 out[(size_t)y * W + x0 + wx + 32 * n + lr] = acc[m][n][r] * unscale;
             }
 }
+
+template __global__ void k_adj_h1wf<256, 256>(const float2 *, const float2 *, const float2 *, int, int, int, int,
+                                              const double *, float *);
+template __global__ void k_adj_h1wf<128, 256>(const float2 *, const float2 *, const float2 *, int, int, int, int,
+                                              const double *, float *);
+template __global__ void k_adj_h1wf<128, 128>(const float2 *, const float2 *, const float2 *, int, int, int, int,
+                                              const double *, float *);

@@ -237,6 +237,14 @@ int gpair_pcg_run(gpair_ctx *ctx, const gpair_pcg_params *params, gpair_pcg_resu
  * forward DFT (the choice is made on the device).  -1 (default): W*W/40, about
  * where the correction costs as much as the full DFT; 0: always the full DFT. */
 int gpair_set_sparse_commit(gpair_ctx *ctx, int max_pixels);
+/* No reference counterpart.  Output tile of the split-precision adjoint on factored
+ * tables (W a multiple of 256): rows x cols = 256 x 256, 128 x 256 or 128 x 128 per
+ * workgroup, with the uv points split so that about `workgroups` workgroups run
+ * (0: 256).  Smaller tiles mean fewer uv splits for as many workgroups, and so
+ * fewer partial images to add up, at more operand building per product.  The
+ * split fixes the summation order: results are reproducible for a given setting,
+ * and differ between settings by rounding. */
+int gpair_set_adjoint_tile(gpair_ctx *ctx, int rows, int cols, int workgroups);
 /* per-pixel preconditioner factor [W*W] multiplying P (NULL clears it) */
 int gpair_set_pfactor(gpair_ctx *ctx, const float *pfactor, int size);
 /* on != 0: gpair_pcg_run replays each iteration's gradient phase (flux, adjoint
