@@ -5,11 +5,13 @@
 // Tensors are FP32 [N][C][H][W] with an explicit batch stride, so a channel
 // concatenation is a channel offset inside one larger buffer (the encoder
 // writes its skip output straight into the decoder's concatenation buffer).
-// 3x3 convolutions (zero padding 1, cross-correlation) are direct: one thread
-// per output pixel and a block of output channels, the weights uniform per
-// block.  Data gradients are the transposed convolution; weight gradients are
-// a two-stage, fixed-order reduction over the batch and the pixels, so
-// training is deterministic.
+// 3x3 convolutions (zero padding 1, cross-correlation) are implicit GEMMs on
+// v_mfma_f32_32x32x16_bf16: BF16 weights and activations (each operand rounded
+// to nearest even from its FP32 value as it is loaded), FP32 accumulation, FP32
+// bias.  The forward and the data gradient (the transposed convolution) share
+// one kernel; weight and bias gradients are a GEMM over the pixels of the batch,
+// split in fixed pixel ranges whose partials a second kernel adds in order, so
+// training is deterministic.  The master weights and Adam's moments stay FP32.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -18,178 +20,186 @@
 #include <vector>
 
 #include "../../include/gpair_nn.h"
+#include "gpair_kernels.h"
 
-#define NN_COB 8          // output channels per thread (forward / data gradient)
-#define NN_WG_CHUNK 2048  // pixels per block of the weight-gradient reduction
 #define NN_MSE_BLOCKS 256
+#define NN_PT 128          // pixels (columns) per block of the convolution GEMMs: 4 waves x 32
+#define NN_WG_TARGET 2048  // blocks the weight-gradient split aims at
 
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-// out[n][co][p] = b[co] + sum_ci sum_k w[co][ci][k] in[n][ci][p + off_k]; optional ReLU
-__global__ __launch_bounds__(256) void k_conv3_fwd(const float *__restrict__ in, long in_bs, int Ci,
-                                                   const float *__restrict__ w, const float *__restrict__ b,
-                                                   float *__restrict__ out, long out_bs, int Co, int H, int W,
-                                                   int relu)
+// MFMA operand layout (32x32x16 BF16): lane l holds A[l % 32][8 (l / 32) .. + 7] and
+// B[8 (l / 32) .. + 7][l % 32]; accumulator register r holds D[8 (r / 4) + 4 (l / 32) +
+// r % 4][l % 32].
+__device__ inline bf16x8 nn_pack8(const float (&v)[8])
 {
-    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-    const int co0 = blockIdx.y * NN_COB, n = blockIdx.z;
-    if (pix >= H * W)
-        return;
-    const int y = pix / W, x = pix % W;
-    const bool up = y > 0, dn = y < H - 1, lf = x > 0, rt = x < W - 1;
-    float acc[NN_COB];
-#pragma unroll
-    for (int j = 0; j < NN_COB; j++)
-        acc[j] = (co0 + j < Co) ? b[co0 + j] : 0.f;
-    const long HW = (long)H * W;
-    const float *src = in + n * in_bs + pix;
-    for (int ci = 0; ci < Ci; ci++) {
-        const float *p = src + ci * HW;
-        float v[9];
-        v[0] = (up && lf) ? p[-W - 1] : 0.f;
-        v[1] = up ? p[-W] : 0.f;
-        v[2] = (up && rt) ? p[-W + 1] : 0.f;
-        v[3] = lf ? p[-1] : 0.f;
-        v[4] = p[0];
-        v[5] = rt ? p[1] : 0.f;
-        v[6] = (dn && lf) ? p[W - 1] : 0.f;
-        v[7] = dn ? p[W] : 0.f;
-        v[8] = (dn && rt) ? p[W + 1] : 0.f;
-#pragma unroll
-        for (int j = 0; j < NN_COB; j++) {
-            if (co0 + j < Co) {
-                const float *wk = w + ((long)(co0 + j) * Ci + ci) * 9;
-#pragma unroll
-                for (int k = 0; k < 9; k++)
-                    acc[j] = fmaf(wk[k], v[k], acc[j]);
-            }
-        }
-    }
-    float *dst = out + n * out_bs + pix;
-#pragma unroll
-    for (int j = 0; j < NN_COB; j++)
-        if (co0 + j < Co)
-            dst[(co0 + j) * HW] = (relu && acc[j] < 0.f) ? 0.f : acc[j];
+    uint4 u;
+    u.x = bf16_bits_rne(v[0]) | ((unsigned)bf16_bits_rne(v[1]) << 16);
+    u.y = bf16_bits_rne(v[2]) | ((unsigned)bf16_bits_rne(v[3]) << 16);
+    u.z = bf16_bits_rne(v[4]) | ((unsigned)bf16_bits_rne(v[5]) << 16);
+    u.w = bf16_bits_rne(v[6]) | ((unsigned)bf16_bits_rne(v[7]) << 16);
+    return __builtin_bit_cast(bf16x8, u);
 }
 
-// din[n][ci][p] = sum_co sum_k w[co][ci][k] dout[n][co][p - off_k]  (overwrites din)
-__global__ __launch_bounds__(256) void k_conv3_bwd_data(const float *__restrict__ dout, long dout_bs, int Co,
-                                                        const float *__restrict__ w, float *__restrict__ din,
-                                                        long din_bs, int Ci, int H, int W)
+// Convolution as a GEMM with rows = output channels r, columns = pixels q and
+// K = (c, k) over the Csrc source channels and the 9 taps k = 3 (dy + 1) + dx + 1:
+//   BWD = 0 (forward):       dst[r][q] = bias[r] + sum w[r][c][k] src[c][q + (dy, dx)], optional ReLU
+//   BWD = 1 (data gradient): dst[r][q] = sum w[c][r][k] src[c][q - (dy, dx)]
+// (src = dout, dst = din, the transposed convolution).  Taps outside the image
+// read zero.  Block: 4 waves, RT x 32 rows by 128 pixels of one image (blockIdx.z);
+// wave v takes pixels 32 v .. + 31 of the block for all RT row tiles.
+template <int BWD, int RT>
+__global__ __launch_bounds__(256) void k_conv3_gemm(const float *__restrict__ src, long src_bs, int Csrc,
+                                                    const float *__restrict__ w, const float *__restrict__ bias,
+                                                    float *__restrict__ dst, long dst_bs, int Cdst, int H, int W,
+                                                    int relu)
 {
-    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ci0 = blockIdx.y * NN_COB, n = blockIdx.z;
-    if (pix >= H * W)
-        return;
-    const int y = pix / W, x = pix % W;
-    const bool up = y > 0, dn = y < H - 1, lf = x > 0, rt = x < W - 1;
-    float acc[NN_COB];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 31, lh = lane >> 5;
+    const int HW = H * W, K = Csrc * 9, nks = (K + 15) / 16;
+    const int q = blockIdx.x * NN_PT + wave * 32 + lr;   // this lane's B column
+    const bool qv = q < HW;
+    const int y = qv ? q / W : 0, x = qv ? q - (q / W) * W : 0;
+    const int row0 = blockIdx.y * 32 * RT;
+    const float *s = src + (long)blockIdx.z * src_bs;
+    v16f acc[RT];
 #pragma unroll
-    for (int j = 0; j < NN_COB; j++)
-        acc[j] = 0.f;
-    const long HW = (long)H * W;
-    const float *src = dout + n * dout_bs + pix;
-    for (int co = 0; co < Co; co++) {
-        const float *p = src + co * HW;
-        // tap k = (ky, kx) takes dout at (y - ky + 1, x - kx + 1)
-        float v[9];
-        v[0] = (dn && rt) ? p[W + 1] : 0.f;
-        v[1] = dn ? p[W] : 0.f;
-        v[2] = (dn && lf) ? p[W - 1] : 0.f;
-        v[3] = rt ? p[1] : 0.f;
-        v[4] = p[0];
-        v[5] = lf ? p[-1] : 0.f;
-        v[6] = (up && rt) ? p[-W + 1] : 0.f;
-        v[7] = up ? p[-W] : 0.f;
-        v[8] = (up && lf) ? p[-W - 1] : 0.f;
+    for (int t = 0; t < RT; t++)
 #pragma unroll
-        for (int j = 0; j < NN_COB; j++) {
-            if (ci0 + j < Ci) {
-                const float *wk = w + ((long)co * Ci + ci0 + j) * 9;
+        for (int r = 0; r < 16; r++)
+            acc[t][r] = 0.f;
+    for (int ks = 0; ks < nks; ks++) {
+        const int kb = ks * 16 + 8 * lh;
+        float bv[8];
 #pragma unroll
-                for (int k = 0; k < 9; k++)
-                    acc[j] = fmaf(wk[k], v[k], acc[j]);
+        for (int j = 0; j < 8; j++) {
+            const int kk = kb + j;
+            float v = 0.f;
+            if (qv && kk < K) {
+                const int c = kk / 9, k = kk - 9 * c;
+                int dy = k / 3 - 1, dx = k - 3 * (k / 3) - 1;
+                if (BWD) {
+                    dy = -dy;
+                    dx = -dx;
+                }
+                const int yy = y + dy, xx = x + dx;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                    v = s[(long)c * HW + yy * W + xx];
             }
+            bv[j] = v;
+        }
+        const bf16x8 b = nn_pack8(bv);
+#pragma unroll
+        for (int t = 0; t < RT; t++) {
+            const int r = row0 + 32 * t + lr;
+            float av[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int kk = kb + j;
+                float v = 0.f;
+                if (r < Cdst && kk < K) {
+                    const int c = kk / 9, k = kk - 9 * c;
+                    v = BWD ? w[((long)c * Cdst + r) * 9 + k] : w[(long)r * K + kk];
+                }
+                av[j] = v;
+            }
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nn_pack8(av), b, acc[t], 0, 0, 0);
         }
     }
-    float *dst = din + n * din_bs + pix;
+    if (!qv)
+        return;
+    float *d = dst + (long)blockIdx.z * dst_bs + q;
 #pragma unroll
-    for (int j = 0; j < NN_COB; j++)
-        if (ci0 + j < Ci)
-            dst[(ci0 + j) * HW] = acc[j];
+    for (int t = 0; t < RT; t++)
+#pragma unroll
+        for (int rr = 0; rr < 16; rr++) {
+            const int r = row0 + 32 * t + 8 * (rr >> 2) + 4 * lh + (rr & 3);
+            if (r < Cdst) {
+                float v = acc[t][rr];
+                if (!BWD)
+                    v += bias[r];
+                if (relu && v < 0.f)
+                    v = 0.f;
+                d[(long)r * HW] = v;
+            }
+        }
 }
 
-// stage 1 of dw / db: block (ci, co, s) sums over the pixels [s*chunk, (s+1)*chunk)
-// of the flattened (n, p) range; part[((co*Ci + ci)*S + s)*10 + k], k = 9 the bias term
-__global__ __launch_bounds__(256) void k_conv3_wgrad_part(const float *__restrict__ in, long in_bs, int Ci,
-                                                          const float *__restrict__ dout, long dout_bs, int Co,
-                                                          int N, int H, int W, int chunk, float *__restrict__ part)
+// Weight and bias gradients as a GEMM with rows = output channels co, columns
+// = (ci, k) for the Ci * 9 weights of a row plus one column of ones (the bias),
+// and K = the pixels of the batch (n, p) flattened:
+//   part[s][co][ci 9 + k] = sum over (n, p) in [s chunk, (s + 1) chunk) of
+//                           dout[n][co][p] in[n][ci][p + (dy, dx)]
+//   part[s][co][Ci 9]     = the same sum of dout[n][co][p]
+// Block: 4 waves, 32 rows by 128 columns, split s = blockIdx.z (chunk a multiple of 16).
+__global__ __launch_bounds__(256) void k_conv3_wgrad(const float *__restrict__ in, long in_bs, int Ci,
+                                                     const float *__restrict__ dout, long dout_bs, int Co, int N,
+                                                     int H, int W, int chunk, float *__restrict__ part)
 {
-    const int ci = blockIdx.x, co = blockIdx.y, s = blockIdx.z, S = gridDim.z;
-    const int HW = H * W;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 31, lh = lane >> 5;
+    const int HW = H * W, K = Ci * 9, ncol = K + 1;
+    const int col = blockIdx.x * NN_PT + wave * 32 + lr;   // this lane's B column
+    const int co = blockIdx.y * 32 + lr;                   // this lane's A row
+    const int cc = col < K ? col / 9 : 0, k = col - 9 * cc;
+    const int dy = k / 3 - 1, dx = k - 3 * (k / 3) - 1;
     const long total = (long)N * HW;
-    const long beg = (long)s * chunk, end = beg + chunk < total ? beg + chunk : total;
-    float acc[10];
+    const long beg = (long)blockIdx.z * chunk, end = beg + chunk < total ? beg + chunk : total;
+    v16f acc;
 #pragma unroll
-    for (int k = 0; k < 10; k++)
-        acc[k] = 0.f;
-    for (long idx = beg + threadIdx.x; idx < end; idx += blockDim.x) {
-        const int n = (int)(idx / HW), pix = (int)(idx % HW);
-        const int y = pix / W, x = pix % W;
-        const bool up = y > 0, dn = y < H - 1, lf = x > 0, rt = x < W - 1;
-        const float g = dout[n * dout_bs + (long)co * HW + pix];
-        const float *p = in + n * in_bs + (long)ci * HW + pix;
-        acc[0] = fmaf(g, (up && lf) ? p[-W - 1] : 0.f, acc[0]);
-        acc[1] = fmaf(g, up ? p[-W] : 0.f, acc[1]);
-        acc[2] = fmaf(g, (up && rt) ? p[-W + 1] : 0.f, acc[2]);
-        acc[3] = fmaf(g, lf ? p[-1] : 0.f, acc[3]);
-        acc[4] = fmaf(g, p[0], acc[4]);
-        acc[5] = fmaf(g, rt ? p[1] : 0.f, acc[5]);
-        acc[6] = fmaf(g, (dn && lf) ? p[W - 1] : 0.f, acc[6]);
-        acc[7] = fmaf(g, dn ? p[W] : 0.f, acc[7]);
-        acc[8] = fmaf(g, (dn && rt) ? p[W + 1] : 0.f, acc[8]);
-        acc[9] += g;
-    }
-    __shared__ float sh[10][256];
+    for (int r = 0; r < 16; r++)
+        acc[r] = 0.f;
+    for (long i0 = beg; i0 < end; i0 += 16) {
+        float av[8], bv[8];
 #pragma unroll
-    for (int k = 0; k < 10; k++)
-        sh[k][threadIdx.x] = acc[k];
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) {
-#pragma unroll
-            for (int k = 0; k < 10; k++)
-                sh[k][threadIdx.x] += sh[k][threadIdx.x + st];
+        for (int j = 0; j < 8; j++) {
+            const long idx = i0 + 8 * lh + j;
+            float a = 0.f, b = 0.f;
+            if (idx < end) {
+                const int n = (int)(idx / HW), p = (int)(idx - (long)n * HW);
+                if (co < Co)
+                    a = dout[n * dout_bs + (long)co * HW + p];
+                if (col == K) {
+                    b = 1.f;
+                } else if (col < K) {
+                    const int py = p / W, px = p - (p / W) * W;
+                    const int yy = py + dy, xx = px + dx;
+                    if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                        b = in[n * in_bs + (long)cc * HW + yy * W + xx];
+                }
+            }
+            av[j] = a;
+            bv[j] = b;
         }
-        __syncthreads();
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nn_pack8(av), nn_pack8(bv), acc, 0, 0, 0);
     }
-    if (threadIdx.x < 10)
-        part[(((long)co * Ci + ci) * S + s) * 10 + threadIdx.x] = sh[threadIdx.x][0];
+    const int oc = blockIdx.x * NN_PT + wave * 32 + lr;
+    if (oc >= ncol)
+        return;
+    float *pp = part + (long)blockIdx.z * Co * ncol + oc;
+#pragma unroll
+    for (int rr = 0; rr < 16; rr++) {
+        const int r = blockIdx.y * 32 + 8 * (rr >> 2) + 4 * lh + (rr & 3);
+        if (r < Co)
+            pp[(long)r * ncol] = acc[rr];
+    }
 }
 
-// stage 2: dw[co][ci][k] = sum_s part, db[co] = sum_s part[ci = 0][9]; fixed order
+// dw[co][ci][k] = sum over s of part[s][co][ci 9 + k], db[co] = sum of part[s][co][Ci 9]; s in order
 __global__ void k_conv3_wgrad_finish(const float *__restrict__ part, int Ci, int Co, int S, float *__restrict__ dw,
                                      float *__restrict__ db)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long nw = (long)Co * Ci * 9;
-    const float *p;
-    if (i < nw) {
-        const long co = i / (Ci * 9), r = i % (Ci * 9), ci = r / 9, k = r % 9;
-        p = part + ((co * Ci + ci) * S) * 10 + k;
-    } else if (i < nw + Co) {
-        p = part + (((i - nw) * Ci) * S) * 10 + 9;
-    } else {
+    const int ncol = Ci * 9 + 1;
+    if (i >= (long)Co * ncol)
         return;
-    }
+    const long co = i / ncol, c = i - co * ncol;
     float s = 0.f;
     for (int j = 0; j < S; j++)
-        s += p[j * 10];
-    if (i < nw)
-        dw[i] = s;
+        s += part[(long)j * Co * ncol + i];
+    if (c < ncol - 1)
+        dw[co * (ncol - 1) + c] = s;
     else
-        db[i - nw] = s;
+        db[co] = s;
 }
 
 // dy[n][i] = 0 where y[n][i] <= 0 (ReLU backward through its output), i < len
@@ -311,23 +321,37 @@ static inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 static void conv_fwd(hipStream_t st, const float *in, long in_bs, int Ci, const float *w, const float *b, float *out,
                      long out_bs, int Co, int N, int H, int W, int relu)
 {
-    dim3 g(nblk((long)H * W), (Co + NN_COB - 1) / NN_COB, N);
-    k_conv3_fwd<<<g, 256, 0, st>>>(in, in_bs, Ci, w, b, out, out_bs, Co, H, W, relu);
+    const unsigned gx = (unsigned)((H * W + NN_PT - 1) / NN_PT);
+    if (Co > 32)
+        k_conv3_gemm<0, 2><<<dim3(gx, (Co + 63) / 64, N), 256, 0, st>>>(in, in_bs, Ci, w, b, out, out_bs, Co, H, W,
+                                                                        relu);
+    else
+        k_conv3_gemm<0, 1><<<dim3(gx, 1, N), 256, 0, st>>>(in, in_bs, Ci, w, b, out, out_bs, Co, H, W, relu);
 }
 
-static int wgrad_splits(int Ci, int Co, int N, int H, int W)
+// splits of the weight-gradient GEMM over the pixels of the batch, and their length
+// (a multiple of 16 pixels)
+static int wgrad_chunk(int Ci, int Co, int N, int H, int W, int *S_out)
 {
     const long total = (long)N * H * W;
-    long S = (total + NN_WG_CHUNK - 1) / NN_WG_CHUNK;
-    const long want = (4096 + (long)Ci * Co - 1) / ((long)Ci * Co);   // aim at >= 4096 blocks
-    if (S > want)
-        S = want;
-    return S < 1 ? 1 : (int)S;
+    const long blocks = (long)((Ci * 9 + 1 + NN_PT - 1) / NN_PT) * ((Co + 31) / 32);
+    long S = (NN_WG_TARGET + blocks - 1) / blocks;
+    const long smax = (total + 255) / 256;   // at least 256 pixels per split
+    if (S > smax)
+        S = smax;
+    if (S < 1)
+        S = 1;
+    long chunk = (total + S - 1) / S;
+    chunk = (chunk + 15) / 16 * 16;
+    *S_out = (int)((total + chunk - 1) / chunk);
+    return (int)chunk;
 }
 
 static long wgrad_part_size(int Ci, int Co, int N, int H, int W)
 {
-    return (long)Ci * Co * wgrad_splits(Ci, Co, N, H, W) * 10;
+    int S;
+    (void)wgrad_chunk(Ci, Co, N, H, W, &S);
+    return (long)S * Co * (Ci * 9 + 1);
 }
 
 // part holds wgrad_part_size(...) floats
@@ -336,14 +360,20 @@ static void conv_bwd(hipStream_t st, const float *in, long in_bs, int Ci, const 
                      float *part)
 {
     if (din) {
-        dim3 g(nblk((long)H * W), (Ci + NN_COB - 1) / NN_COB, N);
-        k_conv3_bwd_data<<<g, 256, 0, st>>>(dout, dout_bs, Co, w, din, din_bs, Ci, H, W);
+        const unsigned gx = (unsigned)((H * W + NN_PT - 1) / NN_PT);
+        if (Ci > 32)
+            k_conv3_gemm<1, 2><<<dim3(gx, (Ci + 63) / 64, N), 256, 0, st>>>(dout, dout_bs, Co, w, nullptr, din,
+                                                                            din_bs, Ci, H, W, 0);
+        else
+            k_conv3_gemm<1, 1><<<dim3(gx, 1, N), 256, 0, st>>>(dout, dout_bs, Co, w, nullptr, din, din_bs, Ci, H, W,
+                                                               0);
     }
-    const int S = wgrad_splits(Ci, Co, N, H, W);
-    const long total = (long)N * H * W;
-    const int chunk = (int)((total + S - 1) / S);
-    k_conv3_wgrad_part<<<dim3(Ci, Co, S), 256, 0, st>>>(in, in_bs, Ci, dout, dout_bs, Co, N, H, W, chunk, part);
-    k_conv3_wgrad_finish<<<nblk((long)Co * Ci * 9 + Co), 256, 0, st>>>(part, Ci, Co, S, dw, db);
+    int S;
+    const int chunk = wgrad_chunk(Ci, Co, N, H, W, &S);
+    const int ncol = Ci * 9 + 1;
+    k_conv3_wgrad<<<dim3((ncol + NN_PT - 1) / NN_PT, (Co + 31) / 32, S), 256, 0, st>>>(in, in_bs, Ci, dout, dout_bs,
+                                                                                        Co, N, H, W, chunk, part);
+    k_conv3_wgrad_finish<<<nblk((long)Co * ncol), 256, 0, st>>>(part, Ci, Co, S, dw, db);
 }
 
 // ---------------------------------------------------------------------------

@@ -7,20 +7,34 @@ upsampling, channel concatenation, and the U-Net built from them, forward and
 backward.  The flat parameter layout matches the library's: for each conv
 layer in order, weights [Co][Ci][3][3] then bias [Co].
 
+With bf16_ops the convolutions take their operands as the library's kernels do:
+weights, inputs and output gradients rounded from their FP32 values to BF16
+(round to nearest even), products summed exactly here (FP64), the bias FP32.
+
 Pinned by tests/test_nn_oracle.py: central finite differences of the loss with
 respect to inputs and every parameter tensor, closed forms (identity and shift
-kernels, constant pooling, pool(up(x)) = x) and the adjoint dot-product test of
-every linear operator.
+kernels, constant pooling, pool(up(x)) = x), the adjoint dot-product test of
+every linear operator, and the BF16 rounding against torch's bfloat16
+conversion and its tie cases.
 """
 from __future__ import annotations
 
 import numpy as np
 
 
+def bf16(x):
+    """x rounded to FP32, then to BF16 by round to nearest even (as float64)."""
+    u = np.ascontiguousarray(np.asarray(x, np.float32)).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
 # ---------------------------------------------------------------------------
 # layers  (x: [N][C][H][W])
 # ---------------------------------------------------------------------------
-def conv3_fwd(x, w, b):
+def conv3_fwd(x, w, b, bf16_ops=False):
+    if bf16_ops:
+        x, w = bf16(x), bf16(w)
     N, Ci, H, W = x.shape
     Co = w.shape[0]
     xp = np.zeros((N, Ci, H + 2, W + 2))
@@ -32,8 +46,10 @@ def conv3_fwd(x, w, b):
     return y
 
 
-def conv3_bwd(x, w, dy):
+def conv3_bwd(x, w, dy, bf16_ops=False):
     """returns dx, dw, db for y = conv3_fwd(x, w, b)."""
+    if bf16_ops:
+        x, w, dy = bf16(x), bf16(w), bf16(dy)
     N, Ci, H, W = x.shape
     xp = np.zeros((N, Ci, H + 2, W + 2))
     xp[:, :, 1:-1, 1:-1] = x
@@ -98,7 +114,7 @@ def unpack_params(flat, layers):
     return out
 
 
-def unet_forward(x, flat, in_ch, out_ch, base, levels):
+def unet_forward(x, flat, in_ch, out_ch, base, levels, bf16_ops=False):
     """returns (y, cache)."""
     layers = unet_layers(in_ch, out_ch, base, levels)
     P = unpack_params(np.asarray(flat, np.float64), layers)
@@ -109,31 +125,31 @@ def unet_forward(x, flat, in_ch, out_ch, base, levels):
         if l > 0:
             inp = pool2(cache["e"][l - 1])
         cache["x"].append(inp)
-        h = relu(conv3_fwd(inp, *P[k])); k += 1
-        e = relu(conv3_fwd(h, *P[k])); k += 1
+        h = relu(conv3_fwd(inp, *P[k], bf16_ops)); k += 1
+        e = relu(conv3_fwd(h, *P[k], bf16_ops)); k += 1
         cache["h"].append(h)
         cache["e"].append(e)
     d = cache["e"][levels - 1]
     cache["d"][levels - 1] = d
     for l in range(levels - 2, -1, -1):
         cat = np.concatenate([up2(d), cache["e"][l]], axis=1)
-        hd = relu(conv3_fwd(cat, *P[k])); k += 1
-        d = relu(conv3_fwd(hd, *P[k])); k += 1
+        hd = relu(conv3_fwd(cat, *P[k], bf16_ops)); k += 1
+        d = relu(conv3_fwd(hd, *P[k], bf16_ops)); k += 1
         cache["cat"].append(cat)
         cache["hd"].append(hd)
         cache["d"][l] = d
-    y = conv3_fwd(d, *P[k])
+    y = conv3_fwd(d, *P[k], bf16_ops)
     return y, cache
 
 
-def unet_backward(dy, flat, cache, in_ch, out_ch, base, levels):
+def unet_backward(dy, flat, cache, in_ch, out_ch, base, levels, bf16_ops=False):
     """returns (dx, dflat)."""
     layers = unet_layers(in_ch, out_ch, base, levels)
     P = unpack_params(np.asarray(flat, np.float64), layers)
     grads = [None] * len(layers)
     ch = [base << l for l in range(levels)]
     k = len(layers) - 1
-    dd, dw, db = conv3_bwd(cache["d"][0], P[k][0], dy)
+    dd, dw, db = conv3_bwd(cache["d"][0], P[k][0], dy, bf16_ops)
     grads[k] = (dw, db)
     de_skip = [None] * levels
     # decoder, shallowest first (reverse of the forward order)
@@ -142,10 +158,10 @@ def unet_backward(dy, flat, cache, in_ch, out_ch, base, levels):
         ka = 2 * levels + 2 * idx
         kb = ka + 1
         dd = dd * (cache["d"][l] > 0)
-        dhd, dw, db = conv3_bwd(cache["hd"][idx], P[kb][0], dd)
+        dhd, dw, db = conv3_bwd(cache["hd"][idx], P[kb][0], dd, bf16_ops)
         grads[kb] = (dw, db)
         dhd = dhd * (cache["hd"][idx] > 0)
-        dcat, dw, db = conv3_bwd(cache["cat"][idx], P[ka][0], dhd)
+        dcat, dw, db = conv3_bwd(cache["cat"][idx], P[ka][0], dhd, bf16_ops)
         grads[ka] = (dw, db)
         de_skip[l] = dcat[:, ch[l + 1]:]
         dd = up2_bwd(dcat[:, :ch[l + 1]])
@@ -157,10 +173,10 @@ def unet_backward(dy, flat, cache, in_ch, out_ch, base, levels):
             de = de_skip[l] + pool2_bwd(dx)
         ka, kb = 2 * l, 2 * l + 1
         de = de * (cache["e"][l] > 0)
-        dh, dw, db = conv3_bwd(cache["h"][l], P[kb][0], de)
+        dh, dw, db = conv3_bwd(cache["h"][l], P[kb][0], de, bf16_ops)
         grads[kb] = (dw, db)
         dh = dh * (cache["h"][l] > 0)
-        dx, dw, db = conv3_bwd(cache["x"][l], P[ka][0], dh)
+        dx, dw, db = conv3_bwd(cache["x"][l], P[ka][0], dh, bf16_ops)
         grads[ka] = (dw, db)
     dflat = np.concatenate([np.concatenate([g[0].ravel(), g[1].ravel()]) for g in grads])
     return dx, dflat

@@ -1,9 +1,13 @@
 """Network kernels (include/gpair_nn.h) against the FP64 network oracle (MI355X only).
 
-Tolerances (FP32 kernels against FP64): layer outputs and data gradients
-max|d| <= 1e-5 max|ref|; weight gradients (sums over the batch and pixels)
-max|d| <= 1e-5 max|ref|; whole U-Net output and every parameter gradient
-max|d| <= 1e-4 max|ref|.
+The convolutions multiply BF16 operands (rounded from FP32) and accumulate in
+FP32; the oracle with bf16_ops rounds the same operands and sums exactly.
+Tolerances: layer outputs, data, weight and bias gradients max|d| <= 1e-5
+max|ref| against the rounded-operand oracle (FP32 accumulation only), and
+<= 2e-2 against the exact convolution (BF16's rounding, 2^-8); whole U-Net
+output and every layer's parameter gradients against the rounded-operand
+oracle within the bounds stated per test (activations computed in FP32 and in
+FP64 round to BF16 differently once in a while, about 1e-3 of a layer's range).
 """
 import ctypes as C
 
@@ -46,18 +50,23 @@ def test_conv3_forward_and_backward(N, Ci, Co, H, W, pad_in, pad_out):
                                   W, relu)
         assert rc == 0
         y = d_y.download().reshape(N, Co + pad_out, H, W)[:, :Co]
-        ref = NN.conv3_fwd(x, w, b)
+        ref = NN.conv3_fwd(x.astype(np.float32), w.astype(np.float32), b.astype(np.float32), True)
+        exact = NN.conv3_fwd(x, w, b)
         if relu:
-            ref = np.maximum(ref, 0)
-        assert _err(y, ref) <= 1e-5
+            ref, exact = np.maximum(ref, 0), np.maximum(exact, 0)
+        print(f"  relu {relu}: y {_err(y, ref):.2e} (exact {_err(y, exact):.2e})", flush=True)
+        assert _err(y, ref) <= 1e-5 and _err(y, exact) <= 2e-2
     rc = L.gpair_nn_conv3_bwd(d_x.at(pad_in * HW), in_bs, Ci, d_w.at(0), d_dy.at(0), out_bs, Co, N, H, W,
                               d_dx.at(pad_in * HW), in_bs, dw_.at(0), db_.at(0))
     assert rc == 0
-    rdx, rdw, rdb = NN.conv3_bwd(x, w, dy)
+    rdx, rdw, rdb = NN.conv3_bwd(x.astype(np.float32), w.astype(np.float32), dy.astype(np.float32), True)
+    xdx, xdw, xdb = NN.conv3_bwd(x, w, dy)
     gdx = d_dx.download().reshape(N, Ci + pad_in, H, W)[:, pad_in:]
-    e = (_err(gdx, rdx), _err(dw_.download().reshape(rdw.shape), rdw), _err(db_.download(), rdb))
-    print(f"  dx {e[0]:.2e} dw {e[1]:.2e} db {e[2]:.2e}", flush=True)
-    assert max(e) <= 1e-5
+    gdw, gdb = dw_.download().reshape(rdw.shape), db_.download()
+    e = (_err(gdx, rdx), _err(gdw, rdw), _err(gdb, rdb))
+    ex = (_err(gdx, xdx), _err(gdw, xdw), _err(gdb, xdb))
+    print(f"  dx {e[0]:.2e} dw {e[1]:.2e} db {e[2]:.2e} (exact {max(ex):.2e})", flush=True)
+    assert max(e) <= 1e-5 and max(ex) <= 2e-2
     # invalid shapes are rejected before any launch
     assert L.gpair_nn_conv3_fwd(d_x.at(0), 1, Ci, d_w.at(0), d_b.at(0), d_y.at(0), out_bs, Co, N, H, W, 0) != 0
 
@@ -79,20 +88,56 @@ def test_pool_and_upsample():
     assert _err(u.download().reshape(x.shape), NN.up2_bwd(NN.up2(x))) <= 1e-6
 
 
-@pytest.mark.parametrize("W,base,levels,N", [(32, 4, 3, 2), (64, 16, 4, 2)])
-def test_unet_forward_backward_against_oracle(W, base, levels, N):
+@pytest.mark.parametrize("N,Ci,Co,H,W", [(2, 48, 16, 64, 64), (2, 128, 256, 8, 8), (1, 256, 128, 16, 16)])
+def test_conv3_gemm_tiles(N, Ci, Co, H, W):
+    """channel counts of the network's layers (two row tiles per wave above 32
+    channels, weight-gradient columns over several blocks, splits over pixels)"""
+    print(f"conv N={N} Ci={Ci} Co={Co} H={H} W={W}", flush=True)
+    L = _lib.load()
+    r = np.random.default_rng(Ci + Co)
+    HW = H * W
+    x = r.standard_normal((N, Ci, H, W)).astype(np.float32)
+    w = (r.standard_normal((Co, Ci, 3, 3)) / np.sqrt(9 * Ci)).astype(np.float32)
+    b = r.standard_normal(Co).astype(np.float32)
+    dy = r.standard_normal((N, Co, H, W)).astype(np.float32)
+    d_x, d_w, d_b = DeviceArray(x.size, x), DeviceArray(w.size, w), DeviceArray(Co, b)
+    d_y, d_dy, d_dx = DeviceArray(N * Co * HW), DeviceArray(dy.size, dy), DeviceArray(x.size)
+    dw_, db_ = DeviceArray(w.size), DeviceArray(Co)
+    assert L.gpair_nn_conv3_fwd(d_x.at(0), Ci * HW, Ci, d_w.at(0), d_b.at(0), d_y.at(0), Co * HW, Co, N, H, W,
+                                1) == 0
+    ref = np.maximum(NN.conv3_fwd(x, w, b, True), 0)
+    ey = _err(d_y.download().reshape(ref.shape), ref)
+    assert L.gpair_nn_conv3_bwd(d_x.at(0), Ci * HW, Ci, d_w.at(0), d_dy.at(0), Co * HW, Co, N, H, W, d_dx.at(0),
+                                Ci * HW, dw_.at(0), db_.at(0)) == 0
+    rdx, rdw, rdb = NN.conv3_bwd(x, w, dy, True)
+    e = (ey, _err(d_dx.download().reshape(rdx.shape), rdx), _err(dw_.download().reshape(rdw.shape), rdw),
+         _err(db_.download(), rdb))
+    print("  y {:.2e} dx {:.2e} dw {:.2e} db {:.2e}".format(*e), flush=True)
+    assert max(e) <= 1e-5
+
+
+# (y, all gradients, worst layer) bounds: the small net about 20x its measured
+# errors; the deep ones above the BF16 flip noise (about 1 % of activations
+# round to the neighbouring BF16 value on one side, each moving the next layer
+# by ~2^-8 / sqrt(K) relative, compounding over about 35 layers: measured y 2.3e-3,
+# gradients 1.5e-2, worst layer 2.3e-2 at 64^2, 4 levels); every layer shape
+# is checked exactly in test_conv3_gemm_tiles
+@pytest.mark.parametrize("W,base,levels,N,bounds", [(32, 4, 3, 2, (1e-6, 1e-3, 3e-3)),
+                                                    (64, 16, 4, 2, (1e-2, 6e-2, 1e-1)),
+                                                    (64, 16, 5, 1, (1e-2, 6e-2, 1e-1))])
+def test_unet_forward_backward_against_oracle(W, base, levels, N, bounds):
     print(f"unet W={W} base={base} levels={levels} N={N}", flush=True)
     net = UNet(W, base=base, levels=levels, max_batch=N, seed=3)
     r = np.random.default_rng(11)
     flat = net.params.astype(np.float64)
     x = r.standard_normal((N, 2, W, W)).astype(np.float32)
     y = net.forward(x)
-    ref, cache = NN.unet_forward(x.astype(np.float64), flat, 2, 2, base, levels)
+    ref, cache = NN.unet_forward(x.astype(np.float64), flat, 2, 2, base, levels, bf16_ops=True)
     ey = _err(y, ref)
     dy = r.standard_normal(ref.shape).astype(np.float32)
     net.backward(dy)
     g = net.grads()
-    _, rg = NN.unet_backward(dy.astype(np.float64), flat, cache, 2, 2, base, levels)
+    _, rg = NN.unet_backward(dy.astype(np.float64), flat, cache, 2, 2, base, levels, bf16_ops=True)
     eg = _err(g, rg)
     # per layer, so a wrong layer cannot hide behind a large one
     worst, o = 0.0, 0
@@ -101,7 +146,7 @@ def test_unet_forward_backward_against_oracle(W, base, levels, N):
         worst = max(worst, _err(g[o:o + k], rg[o:o + k]))
         o += k
     print(f"  y {ey:.2e} grads {eg:.2e} worst layer {worst:.2e} ({net.nparam} params)", flush=True)
-    assert ey <= 1e-4 and eg <= 1e-4 and worst <= 1e-4
+    assert ey <= bounds[0] and eg <= bounds[1] and worst <= bounds[2]
 
 
 def test_training_is_deterministic_and_reduces_the_loss():

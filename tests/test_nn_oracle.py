@@ -112,3 +112,40 @@ def test_planted_flip_in_data_gradient_fails():
     lhs = (NN.conv3_fwd(x, w, np.zeros(2)) * y).sum()
     wrong = NN.conv3_fwd(y, np.transpose(w, (1, 0, 2, 3)), np.zeros(2))   # no 180-degree flip
     assert abs(lhs - (x * wrong).sum()) > 1e-3 * abs(lhs)
+
+
+def test_bf16_rounding_against_torch_and_ties():
+    """the oracle's BF16 rounding: torch's bfloat16 conversion (round to nearest
+    even) on random values of every magnitude, and the tie cases by hand"""
+    torch = pytest.importorskip("torch")
+    r = _rng(21)
+    x = (r.standard_normal(200000) * np.exp(r.uniform(-60, 60, 200000))).astype(np.float32)
+    # exact ties: a BF16 value plus half a BF16 ulp
+    base = NN.bf16(r.standard_normal(1000)).astype(np.float32)
+    ulp = np.abs(base) * 2.0 ** -7
+    ties = (base + 0.5 * np.float32(2.0) ** np.floor(np.log2(ulp))).astype(np.float32)
+    for v in (x, ties):
+        want = torch.from_numpy(v).to(torch.bfloat16).to(torch.float64).numpy()
+        assert np.array_equal(NN.bf16(v), want)
+    # 1 + 2^-8 lies halfway between 1 and 1 + 2^-7: to the even mantissa, 1; 1 + 3 2^-8
+    # lies halfway between 1 + 2^-7 and 1 + 2^-6: to 1 + 2^-6
+    assert NN.bf16(np.float32(1 + 2.0 ** -8)) == 1.0
+    assert NN.bf16(np.float32(1 + 3 * 2.0 ** -8)) == 1 + 2.0 ** -6
+    assert NN.bf16(np.float32(1 + 2.0 ** -8 + 2.0 ** -20)) == 1 + 2.0 ** -7
+    assert NN.bf16(np.float32(-1.5)) == -1.5
+
+
+def test_bf16_convolution_is_the_convolution_of_rounded_operands():
+    """bf16_ops rounds exactly the operands the kernels round (input and weights
+    forward; output gradient with weights or input backward; bias never), and a
+    plant that leaves the weights unrounded is seen"""
+    r = _rng(22)
+    x, w, b = r.standard_normal((2, 3, 5, 6)), r.standard_normal((4, 3, 3, 3)), r.standard_normal(4)
+    dy = r.standard_normal((2, 4, 5, 6))
+    assert np.array_equal(NN.conv3_fwd(x, w, b, True), NN.conv3_fwd(NN.bf16(x), NN.bf16(w), b))
+    for got, want in zip(NN.conv3_bwd(x, w, dy, True), NN.conv3_bwd(NN.bf16(x), NN.bf16(w), NN.bf16(dy))):
+        assert np.array_equal(got, want)
+    assert not np.allclose(NN.conv3_fwd(x, w, b, True), NN.conv3_fwd(NN.bf16(x), w, b), rtol=1e-6, atol=0)
+    # and it stays within BF16's rounding of the exact convolution
+    e = np.abs(NN.conv3_fwd(x, w, b, True) - NN.conv3_fwd(x, w, b)).max()
+    assert 0 < e <= 2 * 2.0 ** -8 * (np.abs(x).max() * np.abs(w).sum(axis=(1, 2, 3)).max())
