@@ -11,9 +11,12 @@ timed region.  A step is one CG iteration.
   changes) over K timed iterations after W warm-up iterations.
 * to_target: wall time to chi2/ndof <= 1 from the prior for the accelerated
   loop in FP32, in BF16x3 and with FP16 DFTs (power-of-two scaling; convergence
-  confirmed on an exact BF16x3 evaluation), from the learned start (network
-  inference included, BF16x3), and for the plain loop with the reference's rules
-  (gpair_cg_run, FP32 DFT), the latter bounded by --plain-budget-s.
+  confirmed on an exact BF16x3 evaluation), and for the plain loop with the
+  reference's rules (gpair_cg_run, FP32 DFT), the latter bounded by
+  --plain-budget-s.
+* learned_cfg5: the learned start against the plain start on held-out problems
+  of the network's training family (256x256, 6-telescope array), network
+  inference included, BF16x3.
 * roofline: the forward DFT, timed with HIP events on the context stream, against
   the roofline that binds it.  FP32 reads the full X and Y tables (64 flop per
   compulsory byte at 256^2, above the FP32 balance of 20): MFMA, against 157.3
@@ -26,7 +29,8 @@ timed region.  A step is one CG iteration.
   same leg re-evaluates each final image's objective with the FP64 oracle
   (err_vs_fp64).
 * batch_cfg4: many independent reconstructions of uneven sizes, sequential
-  against gpair_batch_pcg_run.
+  against gpair_batch_pcg_run (one launch sequence per iteration for all
+  problems) and gpair_batch_pcg_run_streams (one stream per problem).
 * precision_cfg3: the 512x512 precision variant, FP32 / BF16x3 / FP16 with scaling
   against the FP64 oracle (visibilities, chi2, gradient, DFT times, and the
   objective after a bounded run).
@@ -70,7 +74,8 @@ def parse():
     ap.add_argument("--plain-budget-s", type=float, default=30.0)
     ap.add_argument("--no-target", action="store_true", help="skip the runs to chi2_r = 1")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-learned", action="store_true", help="skip the learned-start run")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-start workload")
+    ap.add_argument("--learned-problems", type=int, default=32, help="held-out problems of the learned-start leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch shapes")
     ap.add_argument("--batch", type=int, default=64, help="problems of the batching workload (0: skip)")
     ap.add_argument("--spread-seeds", default="2,3,4",
@@ -321,8 +326,10 @@ def precision_workload(a):
 def batch_workload(a):
     """cfg4: independent problems of uneven sizes (widths 64-256, 500-40,000 V2, T3 about
     2.3 V2), each run for a fixed number of accelerated-loop iterations (BF16x3): one
-    gpair_pcg_run after the other, then all of them in one gpair_batch_pcg_run (one stream
-    per problem, gradient phase replayed from hipGraphs).  Problem set-up is untimed."""
+    gpair_pcg_run after the other, then all of them in one gpair_batch_pcg_run (one launch
+    sequence per iteration for all problems, replayed from a hipGraph), and in
+    gpair_batch_pcg_run_streams (one stream per problem).  Problem set-up is untimed; the
+    batched runs must equal the sequential ones bit for bit."""
     from gpair_amd import synth
     from gpair_amd.api import GpairContext, batch_pcg_run
     t0 = time.perf_counter()
@@ -338,30 +345,65 @@ def batch_workload(a):
     alphas = [p.meta["alpha"] for p in probs]
     starts = [c.get("ci") for c in ctxs]
     K = a.batch_iters
+    key = lambda r: (r.iterations, r.J, r.chi2, r.entropy, r.forward_dfts, r.adjoint_dfts, r.ls_batches)  # noqa: E731
     t0 = time.perf_counter()
-    its_seq = 0
+    seq = []
     for c, al in zip(ctxs, alphas):
         r, _ = c.pcg_run(al, precond=1, max_iter=K, chi2r_target=0.0)
-        its_seq += r.iterations
+        seq.append(key(r))
     for c in ctxs:
         c.L.gpair_sync(c.h)
     t_seq = time.perf_counter() - t0
-    for c, s0 in zip(ctxs, starts):
-        c.gpu_copy_image(s0)
-        c.set_graphs(True)
-    t0 = time.perf_counter()
-    res = batch_pcg_run(ctxs, alphas, max_iter=K, chi2r_target=0.0)
-    t_batch = time.perf_counter() - t0
-    its_b = sum(r.iterations for r in res)
+    its = sum(s[0] for s in seq)
+    out = {"problems": len(probs), "iterations_each": K, "widths": sorted({p.W for p in probs}),
+           "v2_range": [min(p.npow for p in probs), max(p.npow for p in probs)],
+           "t3_range": [min(p.nbis for p in probs), max(p.nbis for p in probs)],
+           "sequential_s": t_seq, "iters_per_s_sequential": its / t_seq, "setup_generation_s": t_gen}
+    for mode in ("streams", "grouped"):
+        for c, s0 in zip(ctxs, starts):
+            c.gpu_copy_image(s0)
+            c.set_graphs(True)
+        for c in ctxs:
+            c.L.gpair_sync(c.h)
+        t0 = time.perf_counter()
+        res = batch_pcg_run(ctxs, alphas, max_iter=K, chi2r_target=0.0, streams=mode == "streams")
+        tb = time.perf_counter() - t0
+        out[mode] = {"batch_s": tb, "iters_per_s": sum(r.iterations for r in res) / tb, "speedup": t_seq / tb,
+                     "identical_to_sequential": int(sum(key(r) == s for r, s in zip(res, seq)))}
+        for c in ctxs:
+            c.set_graphs(False)
     for c in ctxs:
         c.close()
-    print(f"# batch cfg4: {len(probs)} problems x {K} its: sequential {t_seq:.2f} s, batch {t_batch:.2f} s",
+    out["batch_s"], out["speedup"] = out["grouped"]["batch_s"], out["grouped"]["speedup"]
+    print(f"# batch cfg4: {len(probs)} problems x {K} its: sequential {t_seq:.2f} s, one stream each "
+          f"{out['streams']['batch_s']:.2f} s, one launch sequence per iteration {out['grouped']['batch_s']:.2f} s "
+          f"({out['grouped']['identical_to_sequential']}/{len(probs)} bit-identical)", file=sys.stderr, flush=True)
+    return out
+
+
+def learned_workload(a):
+    """cfg5: the learned start (U-Net on the dirty image and the prior, BF16 weights)
+    against the plain start from the prior, on held-out problems of the training family
+    (256x256, 6-telescope array): per problem the plain run to chi2/ndof = 1 (the judge)
+    and the learned path (inference + the run from the learned start to the judge's
+    objective; a rejected start also pays for the plain run), BF16x3."""
+    from gpair_amd import learned
+    if not os.path.exists(learned.WEIGHTS):
+        return None
+    net = learned.UNet.load(learned.WEIGHTS)
+    probs = (p for _, p in learned.problems_ahead(range(learned.TEST_SEED0, learned.TEST_SEED0 + a.learned_problems)))
+    t0 = time.perf_counter()
+    rows = learned.evaluate(net, probs, log=lambda m: None)
+    summ = learned.summarize(rows)
+    summ.update({"params": net.nparam, "blend": net.blend, "seeds": [learned.TEST_SEED0,
+                                                                      learned.TEST_SEED0 + a.learned_problems - 1],
+                 "eval_s": time.perf_counter() - t0})
+    net.close()
+    print(f"# learned cfg5: {summ['plain_converged']} of {summ['problems']} held-out problems converge from the "
+          f"prior; median plain {summ['median_plain_s']:.3f} s ({summ['median_plain_its']:.0f} its), learned path "
+          f"{summ['median_learned_s']:.3f} s ({summ['median_path_its']:.0f} its), accepted {summ['accepted']}",
           file=sys.stderr, flush=True)
-    return {"problems": len(probs), "iterations_each": K, "widths": sorted({p.W for p in probs}),
-            "v2_range": [min(p.npow for p in probs), max(p.npow for p in probs)],
-            "t3_range": [min(p.nbis for p in probs), max(p.nbis for p in probs)],
-            "sequential_s": t_seq, "batch_s": t_batch, "iters_per_s_sequential": its_seq / t_seq,
-            "iters_per_s_batch": its_b / t_batch, "speedup": t_seq / t_batch, "setup_generation_s": t_gen}
+    return summ
 
 
 def pixel_workload(a, prob, prec, cpu=True):
@@ -598,35 +640,6 @@ def main():
                                               "median_wall_s": float(np.median(walls))}
             print(f"# accelerated_bf16x3 over seeds {[a.seed] + seeds}: {['%.2f' % w for w in walls]} s",
                   file=sys.stderr, flush=True)
-        # learned start (BF16x3): features at the prior, network inference, then the loop;
-        # the prior-start BF16x3 run above is the judge, a rejected start pays for both runs
-        from gpair_amd import learned
-        if not a.no_learned and os.path.exists(learned.WEIGHTS):
-            from types import SimpleNamespace
-            net = learned.UNet.load(learned.WEIGHTS)
-            c2 = GpairContext.from_problem(prob)
-            c2.set_precision(1)
-            tune(c2)
-            t0 = time.perf_counter()
-            start, pf = learned.predict(net, c2, prob)
-            t_inf = time.perf_counter() - t0
-            judge = SimpleNamespace(J=tt["accelerated_bf16x3"]["J"])
-            out_l = learned.run_with_start(c2, prob, alpha, start, pf, max_iter=a.max_iter, judge=judge)
-            c2.L.gpair_sync(c2.h)
-            wall = time.perf_counter() - t0
-            rl = out_l.learned_result
-            if not out_l.accepted:
-                wall += tt["accelerated_bf16x3"]["wall_s"]
-            tt["learned_bf16x3"] = {"wall_s": wall, "inference_s": t_inf, "iterations": rl.iterations,
-                                    "iters_per_s": rl.iterations / wall, "converged": bool(rl.converged),
-                                    "chi2r": rl.chi2r, "J": rl.J, "accepted": out_l.accepted,
-                                    "reason": out_l.reason, "forward_dfts": rl.forward_dfts,
-                                    "adjoint_dfts": rl.adjoint_dfts, "params": net.nparam}
-            finals["learned_bf16x3"] = (c2.get("ci"), rl.J)
-            print(f"# learned_bf16x3: {rl.iterations} its {wall:.2f} s chi2r {rl.chi2r:.6f} accepted "
-                  f"{out_l.accepted}", file=sys.stderr, flush=True)
-            c2.close()
-            net.close()
         # plain loop with the reference's rules, bounded by the budget (chunks of 500 iterations)
         c3 = GpairContext.from_problem(prob)
         t0 = time.perf_counter()
@@ -658,6 +671,8 @@ def main():
         out["pixel_moves"] = pixel_workload(a, prob, prec, cpu=not a.no_cpu)
     if a.batch > 0:
         out["batch_cfg4"] = batch_workload(a)
+    if not a.no_learned and a.learned_problems > 0:
+        out["learned_cfg5"] = learned_workload(a)
     if not a.no_cfg3:
         out["precision_cfg3"] = precision_workload(a)
     if not a.no_cpu:

@@ -35,7 +35,7 @@ class CgResult(C.Structure):
 
 class PcgParams(C.Structure):
     _fields_ = [("alpha", _d), ("precond", _i), ("max_iter", _i), ("chi2r_target", _d), ("minvalue", _d),
-                ("wolfe1", _d), ("verbose", _i)]
+                ("wolfe1", _d), ("verbose", _i), ("J_target", _d)]
 
 
 class PcgResult(C.Structure):
@@ -115,6 +115,7 @@ SIGNATURES = {
                                         C.c_char_p]),
     "gpair_set_snapshot": (_i, [_vp, C.c_char_p, _i, _f]),
     "gpair_batch_pcg_run": (_i, [_vp, _i, _vp, _vp]),
+    "gpair_batch_pcg_run_streams": (_i, [_vp, _i, _vp, _vp]),
     "gpair_pixel_begin": (_i, [_vp]),
     "gpu_update_vis_fluxchange": (_i, [_vp, _i, _i, _f, _i, _i, _i]),
     "gpair_update_vis_positionchange": (_i, [_vp, _i, _i, _i, _i, _f, _i, _i]),

@@ -306,12 +306,13 @@ class GpairContext:
         self._check(self.L.gpair_autotune(self.h, int(reps), _ptr(out), 4), "gpair_autotune")
         return tuple(int(v) for v in out)
 
-    def pcg_run(self, alpha, precond=1, max_iter=1000, chi2r_target=1.0, verbose=False):
-        """accelerated loop (preconditioned PR-CG with the batched line search)."""
+    def pcg_run(self, alpha, precond=1, max_iter=1000, chi2r_target=1.0, verbose=False, J_target=0.0):
+        """accelerated loop (preconditioned PR-CG with the batched line search);
+        J_target > 0: the target also needs an objective at or below it."""
         p = _lib.PcgParams()
         self.L.gpair_pcg_default_params(C.byref(p))
         p.alpha, p.precond, p.max_iter = float(alpha), int(precond), int(max_iter)
-        p.chi2r_target, p.verbose = float(chi2r_target), int(verbose)
+        p.chi2r_target, p.verbose, p.J_target = float(chi2r_target), int(verbose), float(J_target)
         res = _lib.PcgResult()
         trace = np.zeros((max(max_iter, 1), 8))
         rc = self.L.gpair_pcg_run(self.h, C.byref(p), C.byref(res), _ptr(trace), trace.shape[0])
@@ -351,9 +352,10 @@ class GpairContext:
         self._check(self.L.gpair_set_graphs(self.h, int(bool(on))), "gpair_set_graphs")
 
 
-def batch_pcg_run(ctxs, alphas, precond=1, max_iter=1000, chi2r_target=1.0):
-    """gpair_batch_pcg_run over several contexts (one stream each); returns the
-    per-context results."""
+def batch_pcg_run(ctxs, alphas, precond=1, max_iter=1000, chi2r_target=1.0, streams=False):
+    """gpair_batch_pcg_run over several contexts (one launch sequence per iteration
+    for all of them where they qualify; streams=True: gpair_batch_pcg_run_streams,
+    one stream each); returns the per-context results."""
     L = _lib.load()
     n = len(ctxs)
     params = (_lib.PcgParams * n)()
@@ -363,7 +365,7 @@ def batch_pcg_run(ctxs, alphas, precond=1, max_iter=1000, chi2r_target=1.0):
         params[i].chi2r_target = float(chi2r_target)
     results = (_lib.PcgResult * n)()
     handles = (C.c_void_p * n)(*[c.h.value for c in ctxs])
-    rc = L.gpair_batch_pcg_run(handles, n, params, results)
+    rc = (L.gpair_batch_pcg_run_streams if streams else L.gpair_batch_pcg_run)(handles, n, params, results)
     if rc != 0:
         bad = [i for i in range(n) if results[i].status]
         raise GpairError(f"gpair_batch_pcg_run: {L.gpair_strerror(rc).decode()} (contexts {bad})")

@@ -8,6 +8,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 #include <vector>
 
@@ -526,16 +527,35 @@ static inline int adj_wide(const gpair_ctx *c) { return ADJ_WIDE && c->W % AW_T 
 
 static inline int adj_factored(const gpair_ctx *c);
 
+// output tile of the split-precision adjoints on factored tables: the configured one
+// when it divides the width, else 128 x 128 (edge tiles partial when W % 128 != 0)
+static inline void adj_tile(const gpair_ctx *c, int *tr, int *tc)
+{
+    const int fit = c->W % c->adj_tr == 0 && c->W % c->adj_tc == 0;
+    *tr = fit ? c->adj_tr : 128;
+    *tc = fit ? c->adj_tc : 128;
+}
+
+static inline int adj_tiles(const gpair_ctx *c)
+{
+    int tr, tc;
+    adj_tile(c, &tr, &tc);
+    return ((c->W + tr - 1) / tr) * ((c->W + tc - 1) / tc);
+}
+
 static int set_adj_split(gpair_ctx *c, int b3, int groups)
 {
-    const int tiles = b3 ? (adj_factored(c) ? (c->W / c->adj_tr) * (c->W / c->adj_tc)
+    const int tiles = b3 ? (adj_factored(c) ? adj_tiles(c)
                             : adj_wide(c)   ? (c->W / AW_T) * (c->W / AW_T)
                                             : (c->W / AB3_YT) * (c->W / AB3_XT))
                          : (c->W / 64) * (c->W / 64);
-    const int gran = b3 ? AB3_PR : 8;
+    // split-precision splits take at least ADJ_MIN_PTS points (four rounds of 16):
+    // below that a workgroup's build and store of its tile outweigh its products,
+    // and the partial images to add up grow with the splits
+    const int gran = b3 ? AB3_PR : 8, minpts = b3 ? ADJ_MIN_PTS : 8;
     int nsplit = groups / tiles;
     if (nsplit < 1) nsplit = 1;
-    const int maxsplit = (c->nuv + gran - 1) / gran;
+    const int maxsplit = (c->nuv + minpts - 1) / minpts;
     if (nsplit > maxsplit) nsplit = maxsplit;
     int pper = (c->nuv + nsplit - 1) / nsplit;
     pper = (pper + gran - 1) / gran * gran;
@@ -589,7 +609,7 @@ extern "C" int gpair_set_precision(gpair_ctx *c, int precision)
 // BF16x3 operands: the split image buffers always; the split copies of the X table
 // only where a kernel reads them (tables copied in by gpu_copy_dft, or an adjoint
 // width without the factored quadrant kernel)
-static inline int adj_factored(const gpair_ctx *c) { return c->have_ft && adj_wide(c); }
+static inline int adj_factored(const gpair_ctx *c) { return c->have_ft && c->W % 64 == 0; }
 
 static int ensure_split_tables(gpair_ctx *c)
 {
@@ -611,21 +631,20 @@ static int ensure_split_tables(gpair_ctx *c)
         dfree(&c->Xh);
         dfree(&c->Xl);
     }
-    if (c->W % AB3_XT == 0) {   // the split-precision adjoint tiles the image in 64 x 128 or 256 x 256
-        if (!adj_factored(c)) {
-            c->nuvp = (c->nuv + AB3_PR - 1) / AB3_PR * AB3_PR;
-            const size_t tt = (size_t)c->nuvp * c->W;
-            if ((rc = dalloc(c, &c->Xth, 2 * tt)) || (rc = dalloc(c, &c->Xtl, 2 * tt)))
-                return rc;
-            k_split_tables_t<<<(unsigned)((tt + 255) / 256), 256, 0, c->stream>>>(c->X, c->nuv, c->nuvp, c->W,
-                                                                                 c->Xth, c->Xtl);
-            LAUNCHCHK(c);
-        } else {
-            dfree(&c->Xth);
-            dfree(&c->Xtl);
-        }
-        if (!c->adj_b3_nsplit &&
-            (rc = set_adj_split(c, 1, adj_factored(c) ? c->adj_groups : adj_wide(c) ? ADJ_WIDE_GROUPS : 512)))
+    if (adj_factored(c)) {   // tiles of the factored tables: any multiple of 64
+        dfree(&c->Xth);
+        dfree(&c->Xtl);
+        if (!c->adj_b3_nsplit && (rc = set_adj_split(c, 1, c->adj_groups)))
+            return rc;
+    } else if (c->W % AB3_XT == 0) {   // uploaded tables: 64 x 128 or 256 x 256 tiles of split copies
+        c->nuvp = (c->nuv + AB3_PR - 1) / AB3_PR * AB3_PR;
+        const size_t tt = (size_t)c->nuvp * c->W;
+        if ((rc = dalloc(c, &c->Xth, 2 * tt)) || (rc = dalloc(c, &c->Xtl, 2 * tt)))
+            return rc;
+        k_split_tables_t<<<(unsigned)((tt + 255) / 256), 256, 0, c->stream>>>(c->X, c->nuv, c->nuvp, c->W,
+                                                                             c->Xth, c->Xtl);
+        LAUNCHCHK(c);
+        if (!c->adj_b3_nsplit && (rc = set_adj_split(c, 1, adj_wide(c) ? ADJ_WIDE_GROUPS : 512)))
             return rc;
     }
     c->split_valid = 1;
@@ -918,32 +937,36 @@ static int launch_adjoint_dft(gpair_ctx *c, int *nsplit_out)
                                                                          2 * c->nuv, c->part, c->ticket,
                                                                          c->scal + SC_AMAXC);
         LAUNCHCHK(c);
-        const unsigned grid = (unsigned)((c->W / c->adj_tr) * (c->W / c->adj_tc) * nsplit);
+        int tr, tc;
+        adj_tile(c, &tr, &tc);
+        const unsigned grid = (unsigned)(adj_tiles(c) * nsplit);
 #define ADJ_H1(TR, TC)                                                                                              \
     TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,                                                                               \
                  (k_adj_h1wf<TR, TC><<<grid, 512, 0, c->stream>>>(c->coef, c->FX, c->FY, c->nuv, c->W,              \
                                                                   c->adj_b3_pper, nsplit, c->scal + SC_AMAXC,       \
                                                                   c->adj_part)))
-        if (c->adj_tr == 256 && c->adj_tc == 256)
+        if (tr == 256 && tc == 256)
             ADJ_H1(256, 256);
-        else if (c->adj_tr == 128 && c->adj_tc == 256)
+        else if (tr == 128 && tc == 256)
             ADJ_H1(128, 256);
         else
             ADJ_H1(128, 128);
 #undef ADJ_H1
-    } else if (c->precision == GPAIR_PREC_BF16X3 && c->W % AB3_XT == 0) {
+    } else if (c->precision == GPAIR_PREC_BF16X3 && (adj_factored(c) || c->W % AB3_XT == 0)) {
         if ((rc = ensure_split_tables(c)))
             return rc;
         nsplit = c->adj_b3_nsplit;
         if (adj_factored(c)) {
-            const unsigned grid = (unsigned)((c->W / c->adj_tr) * (c->W / c->adj_tc) * nsplit);
+            int tr, tc;
+            adj_tile(c, &tr, &tc);
+            const unsigned grid = (unsigned)(adj_tiles(c) * nsplit);
 #define ADJ_B3(TR, TC)                                                                                              \
     TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,                                                                               \
                  (k_adj_b3wf<TR, TC><<<grid, 512, 0, c->stream>>>(c->coef, c->FX, c->FY, c->nuv, c->W,              \
                                                                   c->adj_b3_pper, nsplit, c->adj_part)))
-            if (c->adj_tr == 256 && c->adj_tc == 256)
+            if (tr == 256 && tc == 256)
                 ADJ_B3(256, 256);
-            else if (c->adj_tr == 128 && c->adj_tc == 256)
+            else if (tr == 128 && tc == 256)
                 ADJ_B3(128, 256);
             else
                 ADJ_B3(128, 128);
@@ -1381,6 +1404,7 @@ extern "C" int gpair_snapshot_write(gpair_ctx *c)
 }
 
 #include "gpair_pcg.inc"
+#include "gpair_group.inc"
 #include "gpair_tune.inc"
 #include "gpair_pixel.inc"
 #include "gpair_noalloc.inc"

@@ -189,46 +189,71 @@ __device__ inline double block_fixed_sum(const double *p, int stride, int count)
     return total;
 }
 
-// Every block calls this once with its thread-partial values.  Block sums go to
-// part[blockIdx.x*NV + j]; the last block to arrive sums them in block order and
-// writes out[j] (and, for the flux slot, the float inverse).  The ticket is
-// reset by the last block so no memset is needed between launches.
+// red_sum_parts: out[j] = sum over b < count of part[b*NV + j], thread t taking b =
+// t, t + RED_THREADS, ... in order, then block_sum (and, for the flux slot, the
+// float inverse): the finish of the ticketed reductions below, and of their
+// ticket-free form in a kernel of its own (same order, same bits).
 template <int NV>
-__device__ inline void reduce_finish(double (&v)[NV], double *part, unsigned *ticket, double *out,
-                                     float *invflux_out)
+__device__ inline void red_sum_parts(const double *part, int count, double *out, float *invflux_out)
 {
-    __shared__ double sh[NV][RED_THREADS / 64];
-    __shared__ int is_last;
-    block_sum<NV>(v, sh);
-    const unsigned nblk = gridDim.x;
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int j = 0; j < NV; j++)
-            __hip_atomic_store(&part[blockIdx.x * NV + j], v[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (t == nblk - 1);
-    }
-    __syncthreads();
-    if (!is_last)
-        return;
+    __shared__ double shp[NV][RED_THREADS / 64];
     double acc[NV];
 #pragma unroll
     for (int j = 0; j < NV; j++)
         acc[j] = 0.0;
-    for (unsigned b = threadIdx.x; b < nblk; b += RED_THREADS) {
+    for (int b = threadIdx.x; b < count; b += RED_THREADS) {
 #pragma unroll
         for (int j = 0; j < NV; j++)
             acc[j] += __hip_atomic_load(&part[b * NV + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    block_sum<NV>(acc, sh);
+    block_sum<NV>(acc, shp);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int j = 0; j < NV; j++)
             out[j] = acc[j];
         if (invflux_out)
             *invflux_out = (float)(1.0 / acc[0]);
-        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+// Every block calls this once with its thread-partial values.  Block sums go to
+// part[bx*NV + j]; the last block to arrive sums them in block order and writes
+// out[j] (and, for the flux slot, the float inverse).  The ticket is reset by the
+// last block so no memset is needed between launches.  bx of nbx blocks is given
+// explicitly for remapped launches (reduce_finish: blockIdx.x of gridDim.x).  With
+// no ticket every block only stores its partial (red_sum_parts adds them up later).
+template <int NV>
+__device__ inline void reduce_finish_at(double (&v)[NV], double *part, unsigned *ticket, double *out,
+                                        float *invflux_out, int bx, int nbx)
+{
+    __shared__ double sh[NV][RED_THREADS / 64];
+    __shared__ int is_last;
+    block_sum<NV>(v, sh);
+    const unsigned nblk = (unsigned)nbx;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+            __hip_atomic_store(&part[bx * NV + j], v[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (ticket) {
+            unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            is_last = (t == nblk - 1);
+        }
+    }
+    if (!ticket)
+        return;
+    __syncthreads();
+    if (!is_last)
+        return;
+    red_sum_parts<NV>(part, nbx, out, invflux_out);
+    if (threadIdx.x == 0)
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int NV>
+__device__ inline void reduce_finish(double (&v)[NV], double *part, unsigned *ticket, double *out,
+                                     float *invflux_out)
+{
+    reduce_finish_at<NV>(v, part, ticket, out, invflux_out, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // Launch-shape independent reduction over items [0, n): the range is cut into
@@ -272,6 +297,8 @@ __device__ inline void partitioned_reduce_at(int n, int span_min, F item, double
                 __hip_atomic_store(&part[q * NV + j], v[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
+    if (!ticket)   // ticket-free: the partials only (red_sum_parts adds them up later)
+        return;
     if (threadIdx.x == 0) {
         const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         is_last = (t == (unsigned)nbx - 1);
@@ -279,24 +306,9 @@ __device__ inline void partitioned_reduce_at(int n, int span_min, F item, double
     __syncthreads();
     if (!is_last)
         return;
-    double acc[NV];
-#pragma unroll
-    for (int j = 0; j < NV; j++)
-        acc[j] = 0.0;
-    for (int q = threadIdx.x; q < nparts; q += RED_THREADS) {
-#pragma unroll
-        for (int j = 0; j < NV; j++)
-            acc[j] += __hip_atomic_load(&part[q * NV + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    block_sum<NV>(acc, sh);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int j = 0; j < NV; j++)
-            out[j] = acc[j];
-        if (invflux_out)
-            *invflux_out = (float)(1.0 / acc[0]);
+    red_sum_parts<NV>(part, nparts, out, invflux_out);
+    if (threadIdx.x == 0)
         __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
 }
 
 // partitioned_partials: the partition sums of partitioned_reduce_at alone, stored

@@ -53,16 +53,22 @@ __global__ __launch_bounds__(RED_THREADS) void k_flux(const float *__restrict__ 
 }
 
 // Gull-Skilling entropy (gpair1/trunk/kernel_entropy_gs.cl:3-26), per-pixel in FP64
+__device__ __forceinline__ void k_entropy_body(const int BX, const int GX, const float *__restrict__ img,
+                                                         const float *__restrict__ model, int n, double *part,
+                                                         unsigned *ticket, double *scal){
+    double v[1] = {0.0};
+    for (int i = BX * RED_THREADS + threadIdx.x; i < n; i += GX * RED_THREADS) {
+        double I = img[i], M = model[i];
+        v[0] += (I > 0.0 && M > 0.0) ? (I - M - I * log(I / M)) : -M;
+    }
+    reduce_finish_at<1>(v, part, ticket, scal + SC_ENT, nullptr, BX, GX);
+}
+
 __global__ __launch_bounds__(RED_THREADS) void k_entropy(const float *__restrict__ img,
                                                          const float *__restrict__ model, int n, double *part,
                                                          unsigned *ticket, double *scal)
 {
-    double v[1] = {0.0};
-    for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) {
-        double I = img[i], M = model[i];
-        v[0] += (I > 0.0 && M > 0.0) ? (I - M - I * log(I / M)) : -M;
-    }
-    reduce_finish<1>(v, part, ticket, scal + SC_ENT, nullptr);
+    k_entropy_body((int)blockIdx.x, (int)gridDim.x, img, model, n, part, ticket, scal);
 }
 
 // sum a*b (gpair1/trunk/kernel_scalarprod.cl:2-9 + reduction), FP64 accumulation
@@ -214,17 +220,16 @@ __device__ inline float2 leg_vis(const float2 *vis, int idx, int conj_bit)
     return conj_bit ? cconj(v) : v;
 }
 
-__global__ __launch_bounds__(RED_THREADS) void k_model_chi2(const float2 *__restrict__ vis,
+__device__ __forceinline__ void k_model_chi2_body(const int BX, const int GX, const float2 *__restrict__ vis,
                                                             const float *__restrict__ data,
                                                             const float *__restrict__ err,
                                                             const float2 *__restrict__ phasor,
                                                             const int4 *__restrict__ tri, int npow, int nbis,
                                                             float *__restrict__ mock, double *part,
-                                                            unsigned *ticket, double *scal)
-{
+                                                            unsigned *ticket, double *scal){
     double v[1] = {0.0};
     const int total = npow + nbis;
-    for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < total; i += gridDim.x * RED_THREADS) {
+    for (int i = BX * RED_THREADS + threadIdx.x; i < total; i += GX * RED_THREADS) {
         if (i < npow) {
             const float2 V = vis[i];
             const float m = V.x * V.x + V.y * V.y;
@@ -245,7 +250,18 @@ __global__ __launch_bounds__(RED_THREADS) void k_model_chi2(const float2 *__rest
             v[0] += r0 * r0 + r1 * r1;
         }
     }
-    reduce_finish<1>(v, part, ticket, scal + SC_CHI2, nullptr);
+    reduce_finish_at<1>(v, part, ticket, scal + SC_CHI2, nullptr, BX, GX);
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_model_chi2(const float2 *__restrict__ vis,
+                                                            const float *__restrict__ data,
+                                                            const float *__restrict__ err,
+                                                            const float2 *__restrict__ phasor,
+                                                            const int4 *__restrict__ tri, int npow, int nbis,
+                                                            float *__restrict__ mock, double *part,
+                                                            unsigned *ticket, double *scal)
+{
+    k_model_chi2_body((int)blockIdx.x, (int)gridDim.x, vis, data, err, phasor, tri, npow, nbis, mock, part, ticket, scal);
 }
 
 // ---------------------------------------------------------------------------
@@ -459,18 +475,17 @@ __global__ __launch_bounds__(256) void k_grad_finish(const float *__restrict__ p
 // when set): one launch instead of four in the accelerated loop.  Block b stores its
 // pixels' z.g and z.g_old in zpart[2b], zpart[2b + 1] (plain stores, no ticket);
 // k_pcg_dir adds them up.
-__global__ __launch_bounds__(256) void k_grad_finish_crit(const float *__restrict__ part, int nsplit, int n,
+__device__ __forceinline__ void k_grad_finish_crit_body(const int BX, const int GX, const float *__restrict__ part, int nsplit, int n,
                                                           const double *__restrict__ scal,
                                                           const float *__restrict__ img,
                                                           const float *__restrict__ model, float alpha,
                                                           float *__restrict__ gdata, float *__restrict__ gent,
                                                           float *__restrict__ g, const float *__restrict__ g_old,
                                                           const float *__restrict__ pfac, int precond,
-                                                          float *__restrict__ z, double *__restrict__ zpart)
-{
+                                                          float *__restrict__ z, double *__restrict__ zpart){
     __shared__ double sh[4][GF_PIX];
     const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = blockIdx.x * GF_PIX + lane;
+    const int i = BX * GF_PIX + lane;
     const int per = (nsplit + 3) / 4;
     const int k0 = q * per, k1 = min(nsplit, k0 + per);
     double s = 0.0;
@@ -516,9 +531,21 @@ __global__ __launch_bounds__(256) void k_grad_finish_crit(const float *__restric
     zg = wave_sum(zg);
     zgo = wave_sum(zgo);
     if (lane == 0) {
-        zpart[2 * blockIdx.x] = zg;
-        zpart[2 * blockIdx.x + 1] = zgo;
+        zpart[2 * BX] = zg;
+        zpart[2 * BX + 1] = zgo;
     }
+}
+
+__global__ __launch_bounds__(256) void k_grad_finish_crit(const float *__restrict__ part, int nsplit, int n,
+                                                          const double *__restrict__ scal,
+                                                          const float *__restrict__ img,
+                                                          const float *__restrict__ model, float alpha,
+                                                          float *__restrict__ gdata, float *__restrict__ gent,
+                                                          float *__restrict__ g, const float *__restrict__ g_old,
+                                                          const float *__restrict__ pfac, int precond,
+                                                          float *__restrict__ z, double *__restrict__ zpart)
+{
+    k_grad_finish_crit_body((int)blockIdx.x, (int)gridDim.x, part, nsplit, n, scal, img, model, alpha, gdata, gent, g, g_old, pfac, precond, z, zpart);
 }
 
 // ---------------------------------------------------------------------------
@@ -656,6 +683,9 @@ __global__ __launch_bounds__(256) void k_adj_b3(const float2 *__restrict__ coef,
 #endif
 #ifndef ADJ_WIDE_GROUPS
 #define ADJ_WIDE_GROUPS 256          // workgroups of k_adj_b3w (quadrants x uv splits)
+#endif
+#ifndef ADJ_MIN_PTS
+#define ADJ_MIN_PTS 64               // fewest uv points per split of the split-precision adjoints
 #endif
 #ifndef ADJ_TILE_R
 #define ADJ_TILE_R 256               // output tile of k_adj_b3wf / k_adj_h1wf (rows x columns)
@@ -848,13 +878,12 @@ __global__ void k_step(const float *img, const float *__restrict__ d, float s, f
 // Polak-Ribiere+ beta = max(0, (z.g - z.g_old) / zg_old) (kept in scal[SC_BETA]);
 // conj == 0: beta = 0 (-z).  With dh set, also the BF16 hi / lo split of d for the
 // forward DFT that follows.
-__global__ __launch_bounds__(RED_THREADS) void k_pcg_dir(const float *__restrict__ img, const float *__restrict__ z,
+__device__ __forceinline__ void k_pcg_dir_body(const int BX, const int GX, const float *__restrict__ img, const float *__restrict__ z,
                                                          const float *__restrict__ g, int conj, double zg_old,
                                                          float minval, int n, float *__restrict__ d, double *scal,
                                                          const double *__restrict__ zpart, int nzp,
                                                          double *part, unsigned *ticket, double *out,
-                                                         unsigned short *__restrict__ dh, unsigned short *__restrict__ dl)
-{
+                                                         unsigned short *__restrict__ dh, unsigned short *__restrict__ dl){
     __shared__ double shz[2][RED_THREADS / 64];
     __shared__ double zsum[2];
     if (zpart) {
@@ -867,7 +896,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_pcg_dir(const float *__restrict
         if (threadIdx.x == 0) {
             zsum[0] = zv[0];
             zsum[1] = zv[1];
-            if (blockIdx.x == 0) {
+            if (BX == 0) {
                 scal[SC_ZG] = zv[0];
                 scal[SC_ZGO] = zv[1];
             }
@@ -884,11 +913,11 @@ __global__ __launch_bounds__(RED_THREADS) void k_pcg_dir(const float *__restrict
         if (!(b > 0.0))
             b = 0.0;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    if (BX == 0 && threadIdx.x == 0)
         scal[SC_BETA] = b;
     const float beta = (float)b;
     double v[3] = {0.0, 0.0, 0.0};
-    for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) {
+    for (int i = BX * RED_THREADS + threadIdx.x; i < n; i += GX * RED_THREADS) {
         float di = -z[i] + beta * d[i];
         if (img[i] <= minval && di < 0.f)
             di = 0.f;
@@ -903,7 +932,17 @@ __global__ __launch_bounds__(RED_THREADS) void k_pcg_dir(const float *__restrict
         v[1] += (double)di;
         v[2] += fabs((double)di);
     }
-    reduce_finish<3>(v, part, ticket, out, nullptr);
+    reduce_finish_at<3>(v, part, ticket, out, nullptr, BX, GX);
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_pcg_dir(const float *__restrict__ img, const float *__restrict__ z,
+                                                         const float *__restrict__ g, int conj, double zg_old,
+                                                         float minval, int n, float *__restrict__ d, double *scal,
+                                                         const double *__restrict__ zpart, int nzp,
+                                                         double *part, unsigned *ticket, double *out,
+                                                         unsigned short *__restrict__ dh, unsigned short *__restrict__ dl)
+{
+    k_pcg_dir_body((int)blockIdx.x, (int)gridDim.x, img, z, g, conj, zg_old, minval, n, d, scal, zpart, nzp, part, ticket, out, dh, dl);
 }
 
 // candidate 0 is s = 0 (the linearised current image, the common reference)
@@ -925,11 +964,15 @@ __device__ inline double pix_entropy(double t, double M)
 }
 
 // Vd[p] = vis[p] (FP64 copy of freshly evaluated visibilities)
-__global__ void k_vis_to_double(const float2 *__restrict__ vis, int nuv, double2 *__restrict__ Vd)
-{
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_vis_to_double_body(const int BX, const int GX, const float2 *__restrict__ vis, int nuv, double2 *__restrict__ Vd){
+    const int p = BX * blockDim.x + threadIdx.x;
     if (p < nuv)
         Vd[p] = make_double2(vis[p].x, vis[p].y);
+}
+
+__global__ void k_vis_to_double(const float2 *__restrict__ vis, int nuv, double2 *__restrict__ Vd)
+{
+    k_vis_to_double_body((int)blockIdx.x, (int)gridDim.x, vis, nuv, Vd);
 }
 
 // Batched line search, one launch per candidate batch: block columns
@@ -1091,10 +1134,9 @@ __host__ inline int ls_nparts_ent(int n) { return red_nparts(n, LS_SPAN); }
 // minimum, lane 0 evaluates the parabola.
 #define LS_FIN_SUB 16
 #define LS_FIN_THREADS (2 * LS_K * LS_FIN_SUB)
-__global__ __launch_bounds__(LS_FIN_THREADS) void k_ls_choose(const double *__restrict__ part, int np_chi2,
+__device__ __forceinline__ void k_ls_choose_body(const int BX, const int GX, const double *__restrict__ part, int np_chi2,
                                                               int np_ent, double alpha, double s_ref,
-                                                              double *scal, double *__restrict__ ls)
-{
+                                                              double *scal, double *__restrict__ ls){
 #pragma clang fp contract(off)
     __shared__ double sh[2 * LS_K];
     {
@@ -1143,12 +1185,19 @@ __global__ __launch_bounds__(LS_FIN_THREADS) void k_ls_choose(const double *__re
     scal[SC_KBEST] = kbo;
 }
 
+__global__ __launch_bounds__(LS_FIN_THREADS) void k_ls_choose(const double *__restrict__ part, int np_chi2,
+                                                              int np_ent, double alpha, double s_ref,
+                                                              double *scal, double *__restrict__ ls)
+{
+    k_ls_choose_body((int)blockIdx.x, (int)gridDim.x, part, np_chi2, np_ent, alpha, s_ref, scal, ls);
+}
+
 // trial image I_t = max(I + s d, minval), the change delta = I_t - I, and per
 // block b: cs_part[3b] = sum delta, cs_part[3b + 1] = S(I_t) - S(I), cs_part[3b + 2]
 // = number of pixels where the floor replaced I + s d (plain stores, no ticket:
 // k_commit_dchi2 and the host add them up in block order); with th set, also the
 // BF16 hi / lo split of delta for the forward DFT that follows
-__global__ __launch_bounds__(RED_THREADS) void k_commit_step(const float *__restrict__ img,
+__device__ __forceinline__ void k_commit_step_body(const int BX, const int GX, const float *__restrict__ img,
                                                              const float *__restrict__ d,
                                                              const float *__restrict__ model, float s_arg,
                                                              const double *__restrict__ s_dev, float minval,
@@ -1157,15 +1206,14 @@ __global__ __launch_bounds__(RED_THREADS) void k_commit_step(const float *__rest
                                                              unsigned short *__restrict__ th,
                                                              unsigned short *__restrict__ tl,
                                                              int2 *__restrict__ clist, int *__restrict__ ccount,
-                                                             int cper, int W)
-{
+                                                             int cper, int W){
     __shared__ double sh[3][RED_THREADS / 64];
     __shared__ int wcnt[RED_THREADS / 64];
     const float s = s_dev ? (float)*s_dev : s_arg;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double v[3] = {0.0, 0.0, 0.0};
     int base = 0;   // clamped pixels listed by this block so far
-    for (int i0 = blockIdx.x * RED_THREADS; i0 < n; i0 += gridDim.x * RED_THREADS) {
+    for (int i0 = BX * RED_THREADS; i0 < n; i0 += GX * RED_THREADS) {
         const int i = i0 + (int)threadIdx.x;
         bool clamped = false;
         float corr = 0.f;
@@ -1203,7 +1251,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_commit_step(const float *__rest
             if (clamped) {
                 const int pos = off + __popcll(m & ((1ull << lane) - 1ull));
                 const int y = i / W, x = i - y * W;   // W <= 512: 16 bits each
-                clist[(size_t)blockIdx.x * cper + pos] = make_int2(x | (y << 16), __float_as_int(corr));
+                clist[(size_t)BX * cper + pos] = make_int2(x | (y << 16), __float_as_int(corr));
             }
             for (int w = 0; w < RED_THREADS / 64; w++)
                 base += wcnt[w];
@@ -1214,10 +1262,24 @@ __global__ __launch_bounds__(RED_THREADS) void k_commit_step(const float *__rest
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int j = 0; j < 3; j++)
-            cs_part[3 * blockIdx.x + j] = v[j];
+            cs_part[3 * BX + j] = v[j];
         if (ccount)
-            ccount[blockIdx.x] = base;
+            ccount[BX] = base;
     }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_commit_step(const float *__restrict__ img,
+                                                             const float *__restrict__ d,
+                                                             const float *__restrict__ model, float s_arg,
+                                                             const double *__restrict__ s_dev, float minval,
+                                                             int n, float *__restrict__ trial,
+                                                             float *__restrict__ delta, double *__restrict__ cs_part,
+                                                             unsigned short *__restrict__ th,
+                                                             unsigned short *__restrict__ tl,
+                                                             int2 *__restrict__ clist, int *__restrict__ ccount,
+                                                             int cper, int W)
+{
+    k_commit_step_body((int)blockIdx.x, (int)gridDim.x, img, d, model, s_arg, s_dev, minval, n, trial, delta, cs_part, th, tl, clist, ccount, cper, W);
 }
 
 // visibilities after the committed change: (F V + U_delta) / (F + F_delta)
@@ -1235,7 +1297,7 @@ __device__ __forceinline__ double2 vis_after(const double2 *__restrict__ Vd, con
 // k_commit_step's ncs partials, taken in the same order by every block.  The same
 // launch writes Vn (FP64) and vis_f (float) for every uv point; the T3 terms
 // rebuild their legs with the same expression.
-__global__ __launch_bounds__(RED_THREADS) void k_commit_dchi2(const double2 *__restrict__ Vd,
+__device__ __forceinline__ void k_commit_dchi2_body(const int BX, const int GX, const double2 *__restrict__ Vd,
                                                               const float2 *__restrict__ Udelta,
                                                               const double *__restrict__ scal,
                                                               const double *__restrict__ cs_part, int ncs, int nuv,
@@ -1245,18 +1307,17 @@ __global__ __launch_bounds__(RED_THREADS) void k_commit_dchi2(const double2 *__r
                                                               const float2 *__restrict__ phasor,
                                                               const int4 *__restrict__ tri, int npow, int nbis,
                                                               float *__restrict__ mock_new,
-                                                              double *__restrict__ dc_part)
-{
+                                                              double *__restrict__ dc_part){
     __shared__ double sh[1][RED_THREADS / 64];
     const double FI = scal[SC_FLUX], iv = 1.0 / (FI + block_fixed_sum(cs_part, 3, ncs));
-    for (int p = blockIdx.x * RED_THREADS + threadIdx.x; p < nuv; p += gridDim.x * RED_THREADS) {
+    for (int p = BX * RED_THREADS + threadIdx.x; p < nuv; p += GX * RED_THREADS) {
         const double2 r = vis_after(Vd, Udelta, FI, iv, p);
         Vn[p] = r;
         vis_f[p] = make_float2((float)r.x, (float)r.y);
     }
     double v[1] = {0.0};
     const int total = npow + nbis;
-    for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < total; i += gridDim.x * RED_THREADS) {
+    for (int i = BX * RED_THREADS + threadIdx.x; i < total; i += GX * RED_THREADS) {
         if (i < npow) {
             const double2 a = Vd[i], b = vis_after(Vd, Udelta, FI, iv, i);
             const double d0 = data[i], e = err[i];
@@ -1285,7 +1346,22 @@ __global__ __launch_bounds__(RED_THREADS) void k_commit_dchi2(const double2 *__r
     }
     block_sum<1>(v, sh);
     if (threadIdx.x == 0)
-        dc_part[blockIdx.x] = v[0];
+        dc_part[BX] = v[0];
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_commit_dchi2(const double2 *__restrict__ Vd,
+                                                              const float2 *__restrict__ Udelta,
+                                                              const double *__restrict__ scal,
+                                                              const double *__restrict__ cs_part, int ncs, int nuv,
+                                                              double2 *__restrict__ Vn, float2 *__restrict__ vis_f,
+                                                              const float *__restrict__ data,
+                                                              const float *__restrict__ err,
+                                                              const float2 *__restrict__ phasor,
+                                                              const int4 *__restrict__ tri, int npow, int nbis,
+                                                              float *__restrict__ mock_new,
+                                                              double *__restrict__ dc_part)
+{
+    k_commit_dchi2_body((int)blockIdx.x, (int)gridDim.x, Vd, Udelta, scal, cs_part, ncs, nuv, Vn, vis_f, data, err, phasor, tri, npow, nbis, mock_new, dc_part);
 }
 
 // ---------------------------------------------------------------------------
@@ -1315,16 +1391,21 @@ __global__ void k_split_tables(const float2 *__restrict__ X, int nuv, int W, uns
 }
 
 // image float [W*W] -> Ih, Il bf16 [W*W], same [y][x] layout
-__global__ void k_split_image(const float *__restrict__ img, int n, unsigned short *__restrict__ Ih,
-                              unsigned short *__restrict__ Il)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_split_image_body(const int BX, const int GX, const float *__restrict__ img, int n, unsigned short *__restrict__ Ih,
+                              unsigned short *__restrict__ Il){
+    const int i = BX * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     unsigned short h, l;
     split_bf16(img[i], h, l);
     Ih[i] = h;
     Il[i] = l;
+}
+
+__global__ void k_split_image(const float *__restrict__ img, int n, unsigned short *__restrict__ Ih,
+                              unsigned short *__restrict__ Il)
+{
+    k_split_image_body((int)blockIdx.x, (int)gridDim.x, img, n, Ih, Il);
 }
 
 // Forward DFT on v_mfma_f32_32x32x16_bf16 (gfx950).  Operand layouts: lane l
@@ -1557,15 +1638,14 @@ __global__ __launch_bounds__(256) void k_vis_fwd_b3(const unsigned short *__rest
 
 // the blocks split between two workgroups of k_vis_fwd_b3: one 64-thread block per
 // workgroup boundary, vis = (part 0 + part 1) * scale
-__global__ void k_vis_fwd_b3_combine(const float2 *__restrict__ vpart, int nuv, int W, int upw,
+__device__ __forceinline__ void k_vis_fwd_b3_combine_body(const int BX, const int GX, const float2 *__restrict__ vpart, int nuv, int W, int upw,
                                      const float *__restrict__ scale, float2 *__restrict__ vis,
-                                     const int *__restrict__ run_if)
-{
+                                     const int *__restrict__ run_if){
     if (run_if && *run_if == 0)
         return;
     const int nchunk = W / B3_KC;
     const int total = (nuv + B3_PW - 1) / B3_PW * nchunk;
-    const int ub = (int)(blockIdx.x + 1) * upw;
+    const int ub = (int)(BX + 1) * upw;
     if (ub >= total || ub % nchunk == 0)
         return;
     const int p = ub / nchunk * B3_PW + (int)threadIdx.x;
@@ -1574,6 +1654,13 @@ __global__ void k_vis_fwd_b3_combine(const float2 *__restrict__ vpart, int nuv, 
     const float f = scale ? *scale : 1.0f;
     const float2 a = vpart[p], b = vpart[nuv + p];
     vis[p] = make_float2((a.x + b.x) * f, (a.y + b.y) * f);
+}
+
+__global__ void k_vis_fwd_b3_combine(const float2 *__restrict__ vpart, int nuv, int W, int upw,
+                                     const float *__restrict__ scale, float2 *__restrict__ vis,
+                                     const int *__restrict__ run_if)
+{
+    k_vis_fwd_b3_combine_body((int)blockIdx.x, (int)gridDim.x, vpart, nuv, W, upw, scale, vis, run_if);
 }
 
 template __global__ void k_vis_fwd_b3<2>(const unsigned short *, const unsigned short *, const unsigned short *,
@@ -1703,15 +1790,14 @@ __device__ __forceinline__ void cs_offsets(const int *__restrict__ ccount, int n
 // dynamic LDS: the 64 points' X and Y factor rows, padded (ft + 1 entries each)
 __host__ __device__ inline size_t cs_lds_bytes(int W) { return 2 * sizeof(float2) * CS_PTS * (size_t)(ft_len(W) + 1); }
 
-__global__ __launch_bounds__(64 * CS_WAVES) void k_commit_sparse(const float2 *__restrict__ Ud,
+__device__ __forceinline__ void k_commit_sparse_body(const int BX, const int GX, const float2 *__restrict__ Ud,
                                                                 const double *__restrict__ s_dev, float s_arg,
                                                                 const int2 *__restrict__ clist,
                                                                 const int *__restrict__ ccount, int ncb, int cper,
                                                                 const float2 *__restrict__ FX,
                                                                 const float2 *__restrict__ FY, int nuv, int W,
                                                                 int cap, float2 *__restrict__ Udel,
-                                                                int *__restrict__ dense)
-{
+                                                                int *__restrict__ dense){
     extern __shared__ float2 cs_rows[];
     __shared__ int soff[RED_GRID_MAX + 1];
     __shared__ float2 sacc[CS_WAVES - 1][CS_PTS];
@@ -1721,13 +1807,13 @@ __global__ __launch_bounds__(64 * CS_WAVES) void k_commit_sparse(const float2 *_
         cs_offsets(ccount, ncb, soff, lane);
     __syncthreads();
     const int total = soff[ncb];
-    if (blockIdx.x == 0 && tid == 0)
+    if (BX == 0 && tid == 0)
         *dense = total > cap;   // read by the full DFT launched behind this kernel
     if (total > cap)
         return;
     const int ft = ft_len(W), nco = W / 16, st = ft + 1;   // padded rows: lanes hit different banks
     float2 *sX = cs_rows, *sY = cs_rows + CS_PTS * st;
-    const int p0 = blockIdx.x * CS_PTS;
+    const int p0 = BX * CS_PTS;
     {
         // CS_WAVES threads per row of 64, consecutive factors per thread group
         const int q = tid / CS_WAVES, e0 = tid % CS_WAVES, p = p0 + q;
@@ -1806,6 +1892,18 @@ __global__ __launch_bounds__(64 * CS_WAVES) void k_commit_sparse(const float2 *_
     }
 }
 
+__global__ __launch_bounds__(64 * CS_WAVES) void k_commit_sparse(const float2 *__restrict__ Ud,
+                                                                const double *__restrict__ s_dev, float s_arg,
+                                                                const int2 *__restrict__ clist,
+                                                                const int *__restrict__ ccount, int ncb, int cper,
+                                                                const float2 *__restrict__ FX,
+                                                                const float2 *__restrict__ FY, int nuv, int W,
+                                                                int cap, float2 *__restrict__ Udel,
+                                                                int *__restrict__ dense)
+{
+    k_commit_sparse_body((int)blockIdx.x, (int)gridDim.x, Ud, s_dev, s_arg, clist, ccount, ncb, cper, FX, FY, nuv, W, cap, Udel, dense);
+}
+
 
 
 struct FyStage {
@@ -1837,13 +1935,12 @@ __device__ __forceinline__ void fy_store(const FyStage &S, float2 *sY, int ft, i
 }
 
 template <int NBK>
-__global__ __launch_bounds__(256) void k_vis_fwd_b3f(const unsigned short *__restrict__ Ih,
+__device__ __forceinline__ void k_vis_fwd_b3f_body(const int BX, const int GX, const unsigned short *__restrict__ Ih,
                                                         const unsigned short *__restrict__ Il,
                                                         const float2 *__restrict__ FX,
                                                         const float2 *__restrict__ FY, int nuv, int W, int upw,
                                                         const float *__restrict__ scale, float2 *__restrict__ vis,
-                                                        float2 *__restrict__ vpart, const int *__restrict__ run_if)
-{
+                                                        float2 *__restrict__ vpart, const int *__restrict__ run_if){
     if (run_if && *run_if == 0)   // a commit whose change DFT the sparse correction already gave
         return;
     constexpr int ROWS = NBK * 32;
@@ -1859,8 +1956,8 @@ __global__ __launch_bounds__(256) void k_vis_fwd_b3f(const unsigned short *__res
     const int nchunk = W / B3_KC;
     const int nblk = (nuv + B3_PW - 1) / B3_PW;
     const int total = nblk * nchunk;
-    const int u_end = min(total, (int)(blockIdx.x + 1) * upw);
-    for (int u = (int)blockIdx.x * upw; u < u_end;) {
+    const int u_end = min(total, (int)(BX + 1) * upw);
+    for (int u = (int)BX * upw; u < u_end;) {
         const int blk = u / nchunk, c0 = u % nchunk;
         const int c1 = min(nchunk, c0 + (u_end - u));
         u += c1 - c0;
@@ -2003,6 +2100,17 @@ __global__ __launch_bounds__(256) void k_vis_fwd_b3f(const unsigned short *__res
     }
 }
 
+template <int NBK>
+__global__ __launch_bounds__(256) void k_vis_fwd_b3f(const unsigned short *__restrict__ Ih,
+                                                        const unsigned short *__restrict__ Il,
+                                                        const float2 *__restrict__ FX,
+                                                        const float2 *__restrict__ FY, int nuv, int W, int upw,
+                                                        const float *__restrict__ scale, float2 *__restrict__ vis,
+                                                        float2 *__restrict__ vpart, const int *__restrict__ run_if)
+{
+    k_vis_fwd_b3f_body<NBK>((int)blockIdx.x, (int)gridDim.x, Ih, Il, FX, FY, nuv, W, upw, scale, vis, vpart, run_if);
+}
+
 template __global__ void k_vis_fwd_b3f<2>(const unsigned short *, const unsigned short *, const float2 *,
                                           const float2 *, int, int, int, const float *, float2 *, float2 *,
                                           const int *);
@@ -2029,12 +2137,14 @@ template __global__ void k_vis_fwd_b3f<8>(const unsigned short *, const unsigned
 // h < TC / 128 (B): the four entries of a run share one coarse factor and the runs
 // the same four fine factors.  In a wave the 16 points are consecutive 32-bit
 // words and the four runs start 80 words apart (bank offsets 0, 16, 32, 48), so
-// the builders' LDS writes are free of bank conflicts.
+// the builders' LDS writes are free of bank conflicts.  W is any multiple of 64:
+// rows and columns of an edge tile past W are built from the entries that follow
+// in the point's factor row (coarse index < (W + TR) / 16 <= the row's W / 16 +
+// FT_FINE entries, as TR, TC <= 256) and are not stored.
 template <int TR, int TC>
-__global__ __launch_bounds__(512) void k_adj_b3wf(const float2 *__restrict__ coef, const float2 *__restrict__ FX,
+__device__ __forceinline__ void k_adj_b3wf_body(const int BX, const int GX, const float2 *__restrict__ coef, const float2 *__restrict__ FX,
                                                   const float2 *__restrict__ FY, int nuv, int W, int p_per_split,
-                                                  int nsplit, float *__restrict__ part)
-{
+                                                  int nsplit, float *__restrict__ part){
     static_assert((TR == 128 || TR == 256) && (TC == 128 || TC == 256), "tile");
     constexpr int HA = TR / 128, HB = TC / 128, MT = TR / 128, NT = TC / 64;
     __shared__ __attribute__((aligned(16))) unsigned short sA[2][2][TR * AW_STRIDE];   // [buf][hi/lo]
@@ -2043,8 +2153,8 @@ __global__ __launch_bounds__(512) void k_adj_b3wf(const float2 *__restrict__ coe
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lr = lane & 31, lh = lane >> 5;
     const int ft = ft_len(W), nco = W / 16;
-    const int split = blockIdx.x % nsplit, tile = blockIdx.x / nsplit;
-    const int qx = W / TC;
+    const int split = BX % nsplit, tile = BX / nsplit;
+    const int qx = (W + TC - 1) / TC;   // edge tiles of a width that is not a multiple of TC are partial
     const int y0 = (tile / qx) * TR, x0 = (tile % qx) * TC;
     const int pstart = split * p_per_split;
     const int pend = min(pstart + p_per_split, nuv);
@@ -2172,9 +2282,18 @@ __global__ __launch_bounds__(512) void k_adj_b3wf(const float2 *__restrict__ coe
         for (int n = 0; n < NT; n++)
 #pragma unroll
             for (int r = 0; r < 16; r++) {
-                const int y = y0 + wy + 32 * m + 8 * (r >> 2) + 4 * lh + (r & 3);
-                out[(size_t)y * W + x0 + wx + 32 * n + lr] = acc[m][n][r];
+                const int y = y0 + wy + 32 * m + 8 * (r >> 2) + 4 * lh + (r & 3), x = x0 + wx + 32 * n + lr;
+                if (y < W && x < W)
+                    out[(size_t)y * W + x] = acc[m][n][r];
             }
+}
+
+template <int TR, int TC>
+__global__ __launch_bounds__(512) void k_adj_b3wf(const float2 *__restrict__ coef, const float2 *__restrict__ FX,
+                                                  const float2 *__restrict__ FY, int nuv, int W, int p_per_split,
+                                                  int nsplit, float *__restrict__ part)
+{
+    k_adj_b3wf_body<TR, TC>((int)blockIdx.x, (int)gridDim.x, coef, FX, FY, nuv, W, p_per_split, nsplit, part);
 }
 
 #define ADJ_TILE_INST(K, TR, TC)                                                                                   \
@@ -2452,7 +2571,7 @@ __global__ __launch_bounds__(512) void k_adj_h1wf(const float2 *__restrict__ coe
     const int lr = lane & 31, lh = lane >> 5;
     const int ft = ft_len(W), nco = W / 16;
     const int split = blockIdx.x % nsplit, tile = blockIdx.x / nsplit;
-    const int qx = W / TC;
+    const int qx = (W + TC - 1) / TC;   // edge tiles of a width that is not a multiple of TC are partial
     const int y0 = (tile / qx) * TR, x0 = (tile % qx) * TC;
     const int pstart = split * p_per_split;
     const int pend = min(pstart + p_per_split, nuv);
@@ -2550,8 +2669,9 @@ __global__ __launch_bounds__(512) void k_adj_h1wf(const float2 *__restrict__ coe
         for (int n = 0; n < NT; n++)
 #pragma unroll
             for (int r = 0; r < 16; r++) {
-                const int y = y0 + wy + 32 * m + 8 * (r >> 2) + 4 * lh + (r & 3);
-                out[(size_t)y * W + x0 + wx + 32 * n + lr] = acc[m][n][r] * unscale;
+                const int y = y0 + wy + 32 * m + 8 * (r >> 2) + 4 * lh + (r & 3), x = x0 + wx + 32 * n + lr;
+                if (y < W && x < W)
+                    out[(size_t)y * W + x] = acc[m][n][r] * unscale;
             }
 }
 

@@ -8,23 +8,25 @@ A U-Net, trained and run by the library's own HIP kernels, maps two channels
 
 to two channels
 
-  0  log of the start image over the prior mean, log(I0 / mean M)
+  0  the log ratio of the start image to the prior, log(I0 / M)
   1  log of a per-pixel preconditioner factor (gpair_set_pfactor)
 
 The targets come from reconstructions made by the MEM-CG loop itself (the
 image I* at chi2/ndof <= 1 from the prior start, or within 1 % of it when the
-loop's iteration cap comes first): log(I*/mean M) floored at
-FLOOR_LOG, and log pf = GAMMA log(I*/mean M), so that the loop's image metric
-z = I pf g becomes the power metric I^(1 + GAMMA) at the solution (GAMMA = 1/2
-measured best on the headline problem; the plain image metric is GAMMA = 0).
+loop's iteration cap comes first): log(I*/M) clipped to +-RCLIP (zero wherever
+the reconstruction keeps the prior), and log pf = GAMMA log(I*/M), so that the
+loop's image metric z = I pf g becomes the power metric I^(1 + GAMMA) M^-GAMMA at
+the solution (the plain image metric is GAMMA = 0).
 The plain loop stays the fallback and the judge (run_with_start).
 
-Training (python -m gpair_amd.learned train ...) generates the problems
-(sky family learn_mix on the headline array, synth config cfg5), reconstructs
-each from the prior with the accelerated loop, and fits the network with Adam
-on the library's kernels; augmentation is the 180-degree rotation about the
-prior's centre pixel, under which V2 is unchanged and closure phases change
-sign, so the whole problem maps onto itself.
+Training (python -m gpair_amd.learned all ...) generates the problems (sky
+family learn_mix on the 256x256 grid sampled by the 6-telescope array, synth
+config cfg5), reconstructs each from the prior with the accelerated loop, many
+problems per gpair_batch_pcg_run, and fits the network with Adam on the
+library's kernels (BF16 operands, FP32 accumulation); augmentation is the
+180-degree rotation about the prior's centre pixel, under which V2 is unchanged
+and closure phases change sign, so the whole problem maps onto itself.  The
+stored parameters are BF16, as the convolutions use them.
 """
 from __future__ import annotations
 
@@ -38,14 +40,14 @@ import numpy as np
 from . import _lib
 
 IN_CH, OUT_CH = 2, 2
-FLOOR_LOG = -5.0          # log of the darkest target pixel relative to the prior mean (about 0.7 %)
+RCLIP = 8.0               # |log(I*/M)| <= RCLIP in the targets and the start
 ACCEPT_CHI2R = 1.01       # training targets: reconstructions within 1 % of the target chi2
 GAMMA = 0.5               # power of the learned metric factor
 PF_CLIP = 4.0             # |log pf| <= 4
-# measured on held-out problems (seeds 900004-900013 of the family and the headline
-# problem): the prediction blended 3:1 with the prior (geometric) is the start that
-# converges most often and fastest; the factor channel slows every converging case
-# (e.g. 1,699 instead of 467 iterations), so it is not applied by default
+# start = prior^(1 - BLEND) * prediction^BLEND (geometric); a trained network stores
+# the blend chosen on validation problems, BLEND is the default for one that does not;
+# the factor channel slowed every converging case of the headline array (e.g. 1,699
+# instead of 467 iterations), so it is not applied by default
 BLEND = 0.75
 USE_PF = False
 BUDGET = 6000             # iterations granted to a learned start before the fallback
@@ -168,16 +170,15 @@ class UNet:
                     "train_step")
         return loss.value
 
-    def save(self, path, **meta):
-        np.savez(path, params=self.params, **{k: np.int64(v) for k, v in self.arch.items()},
-                 **{k: np.asarray(v) for k, v in meta.items()})
-
     @classmethod
     def load(cls, path, max_batch=1):
+        """a network saved by main(): BF16 parameters (params_bf16) or FP32 ones
+        (params); the start blend chosen on validation problems when stored."""
         z = np.load(path, allow_pickle=False)
         net = cls(int(z["width"]), base=int(z["base"]), levels=int(z["levels"]), in_ch=int(z["in_ch"]),
                   out_ch=int(z["out_ch"]), max_batch=max_batch)
-        net.params = z["params"]
+        net.params = from_bf16_bits(z["params_bf16"]) if "params_bf16" in z.files else z["params"]
+        net.blend = float(z["blend"]) if "blend" in z.files else BLEND
         return net
 
 
@@ -197,22 +198,26 @@ def features(ctx, prob):
     return np.stack([d, lp]).reshape(2, W, W).astype(np.float32)
 
 
+def log_ratio(prob, final):
+    """log(I*/M) per pixel, clipped to +-RCLIP: 0 wherever the reconstruction keeps the
+    prior (the background of a MEM image), so the network learns only the departures."""
+    M = np.maximum(prob.model.astype(np.float64), 1e-30)
+    return np.clip(np.log(np.maximum(final.astype(np.float64), 1e-30) / M), -RCLIP, RCLIP)
+
+
 def targets(prob, final):
-    """training target [2][W][W]: log(I*/mean M) floored, and GAMMA times it (log pf)."""
-    W = prob.W
-    m = float(prob.model.astype(np.float64).mean())
-    t0 = np.maximum(np.log(np.maximum(final.astype(np.float64), 1e-30) / m), FLOOR_LOG)
+    """training target [2][W][W]: log(I*/M) clipped, and GAMMA times it (log pf)."""
+    t0 = log_ratio(prob, final)
     t1 = np.clip(GAMMA * t0, -PF_CLIP, PF_CLIP)
-    return np.stack([t0, t1]).reshape(2, W, W).astype(np.float32)
+    return np.stack([t0, t1]).reshape(2, prob.W, prob.W).astype(np.float32)
 
 
 def start_from_output(prob, y, blend=BLEND, use_pf=USE_PF):
     """start image and preconditioner factor (None unless use_pf) from the network
-    output [2][W][W]: the start is the geometric blend prior^(1 - blend) * pred^blend."""
+    output [2][W][W]: the start is M exp(blend * predicted log(I*/M)), the geometric
+    blend prior^(1 - blend) * prediction^blend."""
     M = prob.model.astype(np.float64)
-    m = float(M.mean())
-    logI = np.log(m) + np.clip(y[0].ravel().astype(np.float64), FLOOR_LOG - 2.0, 12.0)
-    I0 = np.exp(blend * logI + (1.0 - blend) * np.log(np.maximum(M, 1e-30)))
+    I0 = M * np.exp(blend * np.clip(y[0].ravel().astype(np.float64), -RCLIP, RCLIP))
     pf = np.exp(np.clip(y[1].ravel().astype(np.float64), -PF_CLIP, PF_CLIP)).astype(np.float32) if use_pf else None
     return np.maximum(I0, 1e-8).astype(np.float32), pf
 
@@ -233,17 +238,21 @@ def run_with_start(ctx, prob, alpha, start, pfactor=None, max_iter=30000, chi2r_
     no factor) is the fallback and the judge.
 
     Acceptance: the learned run reaches chi2/ndof <= chi2r_target within `budget`
-    iterations, and, when a judge result (the plain run's gpair_pcg_result) is
-    given, its objective is no worse than the judge's by more than rel_tol
-    relative.  Otherwise the plain run is made (or reused from the judge) and
-    returned."""
+    iterations and, when a judge result (the plain run's gpair_pcg_result) is
+    given, an objective within rel_tol of the judge's or below it: the run goes
+    on past chi2/ndof = 1 until J <= (1 + rel_tol) J_judge (gpair_pcg_params
+    J_target), so an accepted start costs what it takes to reach a solution as
+    good as the plain run's.  Otherwise the plain run is made (or reused from the
+    judge) and returned."""
     t0 = time.perf_counter()
     ctx.gpu_copy_image(start)
     ctx.set_pfactor(pfactor)
-    r, _ = ctx.pcg_run(alpha, precond=1, max_iter=min(max_iter, budget), chi2r_target=chi2r_target)
+    jt = judge.J + rel_tol * abs(judge.J) if judge is not None else 0.0
+    r, _ = ctx.pcg_run(alpha, precond=1, max_iter=min(max_iter, budget), chi2r_target=chi2r_target,
+                       J_target=max(jt, 0.0))
     ok = bool(r.converged) and r.chi2r <= chi2r_target
     reason = "converged" if ok else "learned start did not reach the target"
-    if ok and judge is not None and r.J > judge.J + rel_tol * abs(judge.J):
+    if ok and judge is not None and r.J > jt:
         ok = False
         reason = f"objective {r.J:.8e} above the plain run's {judge.J:.8e} by more than {rel_tol:g}"
     if ok:
@@ -259,24 +268,30 @@ def run_with_start(ctx, prob, alpha, start, pfactor=None, max_iter=30000, chi2r_
 # ---------------------------------------------------------------------------
 # data generation, training, evaluation
 # ---------------------------------------------------------------------------
+CFG = "cfg5"              # 256x256 grid, 6-telescope array (synth.CONFIGS)
+TRAIN_SEED0 = 100000      # training problems: seeds from here
+VAL_SEED0 = 800000        # validation problems (loss curve, choice of the blend)
+TEST_SEED0 = 900000       # held-out problems (evaluation only)
+
+
 def family_problem(seed):
     from . import synth
-    return synth.make_problem("cfg5", seed=seed)
+    return synth.make_problem(CFG, seed=seed)
 
 
-def reconstruct(prob, precision=1, max_iter=15000):
-    """features at the prior, and the accelerated loop's reconstruction from the prior."""
-    from .api import GpairContext
-    alpha = prob.meta["alpha"]
-    with GpairContext.from_problem(prob) as ctx:
-        ctx.set_precision(precision)
-        x = features(ctx, prob)
-        ctx.gpu_copy_image(prob.start)
-        t0 = time.perf_counter()
-        r, _ = ctx.pcg_run(alpha, precond=1, max_iter=max_iter, chi2r_target=1.0)
-        wall = time.perf_counter() - t0
-        final = ctx.get("ci")
-    return x, final, r, wall
+def problems_ahead(seeds, workers=6):
+    """(seed, problem) in seed order, built on host threads ahead of use (the FP64
+    numpy DFTs release the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    seeds = list(seeds)
+    ahead = 4 * workers
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        futs = {k: pool.submit(family_problem, seeds[k]) for k in range(min(ahead, len(seeds)))}
+        for i, s in enumerate(seeds):
+            prob = futs.pop(i).result()
+            if i + ahead < len(seeds):
+                futs[i + ahead] = pool.submit(family_problem, seeds[i + ahead])
+            yield s, prob
 
 
 def rot180(a):
@@ -284,161 +299,250 @@ def rot180(a):
     return np.roll(a[..., ::-1, ::-1], 1, axis=(-2, -1))
 
 
-def generate(seeds, precision=1, max_iter=12000, log=print):
-    """problems are built on host threads (FP64 numpy DFTs) ahead of the GPU loop."""
-    from concurrent.futures import ThreadPoolExecutor
-    X, F, info = [], [], []
-    seeds = list(seeds)
-    pool = ThreadPoolExecutor(max_workers=3)
-    futs = [pool.submit(family_problem, s) for s in seeds[:3]]
-    for i, s in enumerate(seeds):
-        prob = futs[i].result()
-        if i + 3 < len(seeds):
-            futs.append(pool.submit(family_problem, seeds[i + 3]))
-        x, final, r, wall = reconstruct(prob, precision, max_iter)
-        log(f"sample {i} seed {s}: sky {prob.meta['components'][0]['kind']} its {r.iterations} "
-            f"{wall:.2f}s chi2r {r.chi2r:.5f} converged {bool(r.converged)}")
-        if not (r.chi2r <= ACCEPT_CHI2R):
-            continue
-        X.append(x)
-        F.append(final)
-        info.append((s, r.iterations, wall, r.J))
-    pool.shutdown()
-    return np.stack(X), np.stack(F), info
+def to_bf16_bits(a):
+    """float32 -> bfloat16 bit patterns, rounded to nearest even (finite values)."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
 
 
-def train(net, X, T, epochs=40, batch=8, lr=1e-3, seed=0, log=print):
-    """Adam on random batches with the rotation augmentation; cosine learning rate."""
+def from_bf16_bits(b):
+    return (np.asarray(b, np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+@dataclass
+class Dataset:
+    dirty: np.ndarray       # [n][W][W] float16: the dirty-image channel
+    logR: np.ndarray        # [n][W][W] float16: log(I*/M) clipped to +-RCLIP
+    logprior: np.ndarray    # [W][W] float32: the prior channel (one prior for the family)
+    info: np.ndarray        # [n][4]: seed, iterations, J, chi2/ndof of the reconstruction
+
+    def __len__(self):
+        return self.dirty.shape[0]
+
+    def inputs(self, idx):
+        d = self.dirty[idx].astype(np.float32)
+        return np.stack([d, np.broadcast_to(self.logprior, d.shape)], axis=1)
+
+    def targets(self, idx):
+        t0 = self.logR[idx].astype(np.float32)
+        return np.stack([t0, np.clip(GAMMA * t0, -PF_CLIP, PF_CLIP)], axis=1)
+
+
+def generate(seeds, precision=1, max_iter=4000, chunk=16, log=print):
+    """Training pairs from reconstructions made by the plain method (the accelerated
+    loop from the prior to chi2/ndof <= 1), `chunk` problems per gpair_batch_pcg_run.
+    Problems that end above ACCEPT_CHI2R (the loop's cap came first) are dropped."""
+    import itertools
+    from .api import GpairContext, batch_pcg_run
+    dirty, logR, info = [], [], []
+    logprior = None
+    t0 = time.perf_counter()
+    seen = 0
+    src = problems_ahead(seeds)
+    while True:
+        group = list(itertools.islice(src, chunk))
+        if not group:
+            break
+        ctxs = []
+        try:
+            feats = []
+            for _, prob in group:
+                ctx = GpairContext.from_problem(prob)
+                ctxs.append(ctx)
+                ctx.set_precision(precision)
+                feats.append(features(ctx, prob))
+                ctx.gpu_copy_image(prob.start)
+            res = batch_pcg_run(ctxs, [p.meta["alpha"] for _, p in group], max_iter=max_iter, chi2r_target=1.0)
+            for (s, prob), ctx, x, r in zip(group, ctxs, feats, res):
+                if logprior is None:
+                    logprior = x[1].copy()
+                if not r.chi2r <= ACCEPT_CHI2R:
+                    continue
+                dirty.append(x[0].astype(np.float16))
+                logR.append(log_ratio(prob, ctx.get("ci")).reshape(prob.W, prob.W).astype(np.float16))
+                info.append((s, r.iterations, r.J, r.chi2r))
+        finally:
+            for c in ctxs:
+                c.close()
+        seen += len(group)
+        if seen % (16 * chunk) == 0 or seen == len(seeds):
+            log(f"generated {seen}/{len(seeds)} problems, kept {len(dirty)}, {time.perf_counter() - t0:.0f} s")
+    return Dataset(np.stack(dirty), np.stack(logR), logprior, np.array(info, np.float64))
+
+
+def batch_mse(net, data, idx, batch=16):
+    """mean squared error of the network on data[idx] (forward only)."""
+    tot, cnt = 0.0, 0
+    for a in range(0, len(idx), batch):
+        j = idx[a:a + batch]
+        y = net.forward(data.inputs(j)).astype(np.float64)
+        t = data.targets(j).astype(np.float64)
+        tot += float(((y - t) ** 2).sum())
+        cnt += y.size
+    return tot / max(cnt, 1)
+
+
+def train(net, data, val=None, epochs=20, batch=16, lr=1e-3, seed=0, log=print):
+    """Adam on shuffled batches with the rotation augmentation and a cosine learning
+    rate; returns the learning curve rows (epoch, steps, mean training loss of the
+    epoch, validation loss)."""
     rng = np.random.default_rng(seed)
-    n = X.shape[0]
-    W = X.shape[-1]
+    n = len(data)
+    W = data.dirty.shape[-1]
     xs = DeviceArray(batch * IN_CH * W * W)
     ts = DeviceArray(batch * OUT_CH * W * W)
-    steps = max(1, epochs * n // batch)
-    hist = []
-    for k in range(steps):
-        idx = rng.choice(n, size=batch, replace=n < batch)
-        xb, tb = X[idx].copy(), T[idx].copy()
-        flip = rng.random(batch) < 0.5
-        xb[flip], tb[flip] = rot180(xb[flip]), rot180(tb[flip])
-        xs.upload(xb)
-        ts.upload(tb)
-        lr_k = lr * 0.5 * (1.0 + np.cos(np.pi * k / steps))
-        loss = net.train_step(xs.at(0), ts.at(0), batch, max(lr_k, lr * 1e-3))
-        hist.append(loss)
-        if k % 100 == 0 or k == steps - 1:
-            log(f"step {k}/{steps} loss {np.mean(hist[-100:]):.5f} lr {lr_k:.2e}")
-    xs.close()
-    ts.close()
-    return hist
+    per = max(1, n // batch)
+    total = epochs * per
+    curve = []
+    k = 0
+    vidx = np.arange(len(val)) if val is not None else None
+    t0 = time.perf_counter()
+    try:
+        for ep in range(epochs):
+            perm = rng.permutation(n)
+            losses = []
+            for b in range(per):
+                idx = np.sort(perm[b * batch:(b + 1) * batch])
+                xb, tb = data.inputs(idx), data.targets(idx)
+                flip = rng.random(len(idx)) < 0.5
+                xb[flip], tb[flip] = rot180(xb[flip]), rot180(tb[flip])
+                xs.upload(xb)
+                ts.upload(tb)
+                lr_k = lr * 0.5 * (1.0 + np.cos(np.pi * k / total))
+                losses.append(net.train_step(xs.at(0), ts.at(0), len(idx), max(lr_k, lr * 1e-3)))
+                k += 1
+            vl = batch_mse(net, val, vidx, batch) if val is not None else float("nan")
+            curve.append((ep + 1, k, float(np.mean(losses)), vl))
+            log(f"epoch {ep + 1}/{epochs}: {k} steps, training loss {np.mean(losses):.5f}, validation loss "
+                f"{vl:.5f} ({time.perf_counter() - t0:.0f} s)")
+    finally:
+        xs.close()
+        ts.close()
+    return curve
 
 
-def predict(net, ctx, prob):
+def predict(net, ctx, prob, blend=None):
     """start image and preconditioner factor for a problem held by ctx."""
     x = features(ctx, prob)
     y = net.forward(x[None])[0]
-    return start_from_output(prob, y)
+    return start_from_output(prob, y, blend=getattr(net, "blend", BLEND) if blend is None else blend)
 
 
-def evaluate(net, seeds, precision=1, max_iter=15000, log=print, problems=None, autotune=True):
-    """plain start (the judge) against the learned start, per problem."""
+def evaluate(net, problems, precision=1, max_iter=4000, blend=None, log=print):
+    """Per problem: the plain start (the judge) and the learned start, each timed to
+    chi2/ndof <= 1 after an untimed one-iteration run that sets up the context (split
+    tables, graphs).  The learned path's cost is inference plus its run, plus the plain
+    run when the start is rejected."""
     from .api import GpairContext
     rows = []
-    probs = problems if problems is not None else [family_problem(s) for s in seeds]
-    for prob in probs:
+    for prob in problems:
         alpha = prob.meta["alpha"]
         with GpairContext.from_problem(prob) as ctx:
             ctx.set_precision(precision)
-            if autotune:
-                ctx.autotune()
+            ctx.pcg_run(alpha, precond=1, max_iter=1, chi2r_target=0.0)
+            ctx.gpu_copy_image(prob.start)
             t0 = time.perf_counter()
             rp, _ = ctx.pcg_run(alpha, precond=1, max_iter=max_iter, chi2r_target=1.0)
             ctx.L.gpair_sync(ctx.h)
             wp = time.perf_counter() - t0
             t0 = time.perf_counter()
-            start, pf = predict(net, ctx, prob)
+            start, pf = predict(net, ctx, prob, blend)
             t_inf = time.perf_counter() - t0
             out = run_with_start(ctx, prob, alpha, start, pf, max_iter=max_iter, judge=rp)
             ctx.set_pfactor(None)
         lr_ = out.learned_result
         cost = t_inf + out.wall_s + (0.0 if out.accepted else wp)
-        row = dict(cfg=prob.meta["cfg"], seed=int(prob.meta["seed"]), plain_its=rp.iterations, plain_s=wp,
-                   plain_J=rp.J, plain_conv=bool(rp.converged), learned_its=lr_.iterations, learned_s=cost,
-                   learned_J=lr_.J, learned_conv=bool(lr_.converged), accepted=out.accepted, infer_s=t_inf,
-                   reason=out.reason)
-        log(f"{row['cfg']} seed {row['seed']}: plain {rp.iterations} its {wp:.2f} s J {rp.J:.2f} conv "
-            f"{bool(rp.converged)} | learned {lr_.iterations} its, path {cost:.2f} s, J {lr_.J:.2f}, accepted "
-            f"{out.accepted} ({out.reason})")
-        rows.append(row)
+        rows.append(dict(cfg=prob.meta["cfg"], seed=int(prob.meta["seed"]), plain_its=rp.iterations, plain_s=wp,
+                         plain_J=rp.J, plain_conv=bool(rp.converged), learned_its=lr_.iterations,
+                         path_its=lr_.iterations + (0 if out.accepted else rp.iterations), learned_s=cost,
+                         learned_J=lr_.J, learned_conv=bool(lr_.converged), accepted=out.accepted, infer_s=t_inf,
+                         reason=out.reason))
+        r = rows[-1]
+        if len(rows) <= 8 or len(rows) % 64 == 0:
+            log(f"{r['cfg']} seed {r['seed']}: plain {r['plain_its']} its {wp:.3f} s | learned {r['learned_its']} "
+                f"its, path {cost:.3f} s, accepted {out.accepted} ({out.reason})")
     return rows
 
 
+def summarize(rows):
+    """medians and totals of an evaluation; objective differences of the converged
+    learned runs relative to the judge's."""
+    conv = [r for r in rows if r["plain_conv"]]
+    dJ = np.array([(r["learned_J"] - r["plain_J"]) / abs(r["plain_J"]) for r in conv if r["learned_conv"]])
+    med = lambda k: float(np.median([r[k] for r in conv])) if conv else float("nan")  # noqa: E731
+    return dict(problems=len(rows), plain_converged=len(conv), accepted=int(sum(r["accepted"] for r in conv)),
+                median_plain_s=med("plain_s"), median_learned_s=med("learned_s"),
+                median_plain_its=med("plain_its"), median_path_its=med("path_its"),
+                total_plain_s=float(sum(r["plain_s"] for r in conv)),
+                total_learned_s=float(sum(r["learned_s"] for r in conv)),
+                dJ_rel_quantiles=[float(q) for q in np.quantile(dJ, [0.0, 0.1, 0.5, 0.9, 1.0])] if dJ.size else [],
+                dJ_within_1e4=int(np.sum(np.abs(dJ) <= 1e-4)), dJ_below=int(np.sum(dJ < 0.0)))
+
+
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "unet_cfg5.npz")
-
-
-def save_data(path, X, F, info):
-    """dirty-image channel and floored log reconstruction as float16 (the prior
-    channel is the same for every problem of the family and is stored once)."""
-    np.savez_compressed(path, dirty=X[:, 0].astype(np.float16), logprior=X[0, 1].astype(np.float32),
-                        logF=np.maximum(np.log(np.maximum(F.astype(np.float64), 1e-30)), FLOOR_LOG)
-                        .astype(np.float16), info=np.array(info, np.float64))
-
-
-def load_data(path):
-    z = np.load(path, allow_pickle=False)
-    d = z["dirty"].astype(np.float32)
-    n, W = d.shape[0], d.shape[-1]
-    X = np.stack([d, np.broadcast_to(z["logprior"], d.shape)], axis=1).astype(np.float32)
-    lf = z["logF"].astype(np.float32).reshape(n, W, W)
-    T = np.stack([lf, np.clip(GAMMA * lf, -PF_CLIP, PF_CLIP)], axis=1).astype(np.float32)
-    return X, T, z["info"]
 
 
 def main(argv=None):
     import argparse
     import json
     ap = argparse.ArgumentParser(prog="python -m gpair_amd.learned")
-    ap.add_argument("mode", choices=["generate", "train", "eval"])
-    ap.add_argument("--n", type=int, default=240, help="training problems")
-    ap.add_argument("--seed0", type=int, default=100000, help="first training seed")
-    ap.add_argument("--data", default="gpurun_out/learn_data.npz")
+    ap.add_argument("mode", choices=["all", "eval"])
+    ap.add_argument("--n", type=int, default=4000, help="training problems")
+    ap.add_argument("--n-val", type=int, default=128, help="validation problems")
+    ap.add_argument("--n-test", type=int, default=512, help="held-out problems")
     ap.add_argument("--out", default=WEIGHTS)
-    ap.add_argument("--epochs", type=int, default=100)
-    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--base", type=int, default=16)
-    ap.add_argument("--levels", type=int, default=4)
-    ap.add_argument("--eval-seeds", default="900000:900008")
+    ap.add_argument("--levels", type=int, default=5)
+    ap.add_argument("--blends", default="0.5,0.75,1.0", help="start blends tried on validation problems")
+    ap.add_argument("--n-blend", type=int, default=32, help="validation problems per blend trial")
     ap.add_argument("--json", default=None)
     a = ap.parse_args(argv)
 
     def log(m):
         print(m, flush=True)
 
-    if a.mode in ("generate", "train"):
-        if a.mode == "generate" or not os.path.exists(a.data):
-            t0 = time.perf_counter()
-            X, F, info = generate(range(a.seed0, a.seed0 + a.n), log=log)
-            save_data(a.data, X, F, info)
-            log(f"saved {X.shape[0]} samples to {a.data} ({time.perf_counter() - t0:.0f} s)")
-        if a.mode == "train":
-            X, T, info = load_data(a.data)
-            W = X.shape[-1]
-            net = UNet(W, base=a.base, levels=a.levels, max_batch=a.batch, seed=1)
-            log(f"training {net.nparam} parameters on {X.shape[0]} samples")
-            t0 = time.perf_counter()
-            hist = train(net, X, T, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
-            log(f"trained in {time.perf_counter() - t0:.1f} s, final loss {np.mean(hist[-50:]):.5f}")
-            os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
-            net.save(a.out, n_train=X.shape[0], epochs=a.epochs, final_loss=float(np.mean(hist[-50:])))
-            log(f"saved {a.out}")
-    if a.mode == "eval":
-        lo, hi = (int(v) for v in a.eval_seeds.split(":"))
-        net = UNet.load(a.out)
-        rows = evaluate(net, range(lo, hi), log=log)
-        if a.json:
-            with open(a.json, "w") as f:
-                json.dump(rows, f)
+    report = {}
+    if a.mode == "all":
+        t0 = time.perf_counter()
+        data = generate(range(TRAIN_SEED0, TRAIN_SEED0 + a.n), log=log)
+        val = generate(range(VAL_SEED0, VAL_SEED0 + a.n_val), log=log)
+        t_gen = time.perf_counter() - t0
+        log(f"data: {len(data)} training and {len(val)} validation reconstructions in {t_gen:.0f} s")
+        W = data.dirty.shape[-1]
+        net = UNet(W, base=a.base, levels=a.levels, max_batch=a.batch, seed=1)
+        log(f"training {net.nparam} parameters")
+        t0 = time.perf_counter()
+        curve = train(net, data, val, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
+        t_train = time.perf_counter() - t0
+        # the stored network is the one the kernels run: BF16 parameters
+        net.params = from_bf16_bits(to_bf16_bits(net.params))
+        trials = {}
+        for b in (float(v) for v in a.blends.split(",")):
+            probs = [family_problem(s) for s in range(VAL_SEED0, VAL_SEED0 + a.n_blend)]
+            trials[b] = summarize(evaluate(net, probs, blend=b, log=lambda m: None))
+            log(f"blend {b}: {trials[b]}")
+        blend = min(trials, key=lambda b: (trials[b]["median_learned_s"], -trials[b]["accepted"]))
+        os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+        np.savez(a.out, params_bf16=to_bf16_bits(net.params), **{k: np.int64(v) for k, v in net.arch.items()},
+                 blend=np.float64(blend), n_train=np.int64(len(data)), n_val=np.int64(len(val)),
+                 epochs=np.int64(a.epochs), batch=np.int64(a.batch), lr=np.float64(a.lr),
+                 curve=np.array(curve, np.float64), train_info=data.info, gen_s=np.float64(t_gen),
+                 train_s=np.float64(t_train))
+        log(f"saved {a.out} (blend {blend}, {t_train:.0f} s of training)")
+        report.update(n_train=len(data), n_val=len(val), gen_s=t_gen, train_s=t_train, curve=curve,
+                      blend_trials={str(k): v for k, v in trials.items()}, blend=blend, params=net.nparam)
+        net.close()
+    net = UNet.load(a.out, max_batch=1)
+    probs = problems_ahead(range(TEST_SEED0, TEST_SEED0 + a.n_test))
+    rows = evaluate(net, (p for _, p in probs), log=log)
+    report["test"] = summarize(rows)
+    log(f"held-out: {report['test']}")
+    if a.json:
+        with open(a.json, "w") as f:
+            json.dump(dict(report, rows=rows), f)
 
 
 if __name__ == "__main__":

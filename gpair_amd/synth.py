@@ -336,11 +336,13 @@ CONFIGS = {
     "cfg2": dict(W=256, pix=0.05, ntel=9, nch=64, nsnap=16, uvmax=3e8, sky="ld_disk_spots_companion", alpha=0.3,
                  prior=("gauss", 5.0)),
     # precision variant: 512x512, 256 channels
-    # learned start: skies from the learn_mix family on the headline array and grid
-    # (closure triangles with |T3| < 1e-3 are not observed: smooth skies have closure
-    # amplitudes far below any instrument's reach, and the reference weights the
-    # closure phase by 1 / |observed amplitude|)
-    "cfg5": dict(W=256, pix=0.05, ntel=9, nch=64, nsnap=16, uvmax=3e8, sky="learn_mix", alpha=0.3,
+    # learned start: skies from the learn_mix family on the headline grid (256x256 @
+    # 0.05 mas) sampled by the 6-telescope array of cfg1 (480 V2, 640 closure triangles
+    # before the cut below); alpha 0.03 for this sparse sampling (0.3 leaves chi2/ndof
+    # near 2 on some skies); closure triangles with |T3| < 1e-3 are not observed: smooth
+    # skies have closure amplitudes far below any instrument's reach, and the reference
+    # weights the closure phase by 1 / |observed amplitude|
+    "cfg5": dict(W=256, pix=0.05, ntel=6, nch=8, nsnap=4, uvmax=3e8, sky="learn_mix", alpha=0.03,
                  prior=("gauss", 5.0), t3_min=1e-3),
     # batching: 9 telescopes (T3 = 84/36 of V2), widths and V2 counts per problem
     # drawn by batch_problems

@@ -210,6 +210,8 @@ typedef struct {
     double minvalue;       /* 1e-8 */
     double wolfe1;         /* Armijo constant, 1e-4 */
     int verbose;
+    double J_target;       /* > 0: the target also needs J <= J_target (a run judged
+                              against another run's objective); 0: chi2/ndof only */
 } gpair_pcg_params;
 
 typedef struct {
@@ -238,8 +240,9 @@ int gpair_pcg_run(gpair_ctx *ctx, const gpair_pcg_params *params, gpair_pcg_resu
  * where the correction costs as much as the full DFT; 0: always the full DFT. */
 int gpair_set_sparse_commit(gpair_ctx *ctx, int max_pixels);
 /* No reference counterpart.  Output tile of the split-precision adjoint on factored
- * tables (W a multiple of 256): rows x cols = 256 x 256, 128 x 256 or 128 x 128 per
- * workgroup, with the uv points split so that about `workgroups` workgroups run
+ * tables: rows x cols = 256 x 256, 128 x 256 or 128 x 128 per workgroup where the
+ * tile divides W (otherwise 128 x 128, with partial edge tiles when W is not a
+ * multiple of 128), with the uv points split so that about `workgroups` workgroups run
  * (0: 256).  Smaller tiles mean fewer uv splits for as many workgroups, and so
  * fewer partial images to add up, at more operand building per product.  The
  * split fixes the summation order: results are reproducible for a given setting,
@@ -252,14 +255,25 @@ int gpair_set_pfactor(gpair_ctx *ctx, const float *pfactor, int size);
  * from a hipGraph captured at the start of the run; results are bit-identical
  * to the launched path.  Off by default; ignored while DFT timing is on. */
 int gpair_set_graphs(gpair_ctx *ctx, int on);
-/* Batch (no reference counterpart): runs the accelerated loop on `count`
- * independent contexts (any sizes), each on its own stream, with params[i] and
- * results[i] per context.  The host resumes whichever problem has finished its
- * read-back, so the GPU works on the others meanwhile; every problem follows
- * exactly the steps gpair_pcg_run would take (bit-identical results).  Returns
- * the first non-zero status, GPAIR_OK when all succeed. */
+/* Batch (no reference counterpart; it replaces the one-trial-step-at-a-time loop
+ * of gpair1/trunk/gpair.c:808-906 run once per problem): runs the accelerated
+ * loop on `count` independent contexts with params[i] and results[i] per
+ * context.  When every context is BF16x3 on tables set from uv (gpair_set_uv),
+ * of a width that is a multiple of 64 up to 512, without snapshots or DFT
+ * timing, and count <= 1024, one launch sequence per iteration serves all
+ * problems: each kernel takes the blocks of every problem in one launch, the
+ * loop's decisions (step choice, Armijo test, restarts, re-evaluations) run on
+ * the device, and the sequence is replayed from one hipGraph.  Otherwise each
+ * context runs on its own stream (gpair_batch_pcg_run_streams).  Either way every
+ * problem follows exactly the steps gpair_pcg_run would take (bit-identical
+ * results) and the contexts end as gpair_pcg_run leaves them.  Returns the first
+ * non-zero status, GPAIR_OK when all succeed. */
 int gpair_batch_pcg_run(gpair_ctx *const *ctxs, int count, const gpair_pcg_params *params,
                         gpair_pcg_result *results);
+/* The same with one stream per context: the host resumes whichever problem has
+ * finished its read-back, so the GPU works on the others meanwhile. */
+int gpair_batch_pcg_run_streams(gpair_ctx *const *ctxs, int count, const gpair_pcg_params *params,
+                                gpair_pcg_result *results);
 
 /* No reference counterpart (the reference restarts from scratch): the
  * accelerated loop with a checkpoint.  resume_from (NULL: start from the current

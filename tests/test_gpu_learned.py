@@ -1,10 +1,11 @@
 """Learned start (gpair_amd/learned.py with the committed network
-gpair_amd/weights/unet_cfg5.npz) against the plain start from the prior
-(MI355X only).
+gpair_amd/weights/unet_cfg5.npz) against the plain start from the prior, on
+held-out problems of the network's training family (MI355X only).
 
 The plain MEM-CG run is the judge: a learned run is accepted when it reaches
-chi2/ndof <= 1 with an objective no worse than the plain run's by more than
-1e-4 relative; a rejected start costs the learned run plus the plain run.
+chi2/ndof <= 1 with an objective within 1e-4 relative of the plain run's or
+below it (the learned run goes on past chi2/ndof = 1 until it gets there); a
+rejected start costs the learned run plus the plain run.
 """
 from types import SimpleNamespace
 
@@ -16,7 +17,7 @@ from gpair_amd.api import GpairContext
 
 pytestmark = pytest.mark.gpu
 
-HELD_OUT = [900000, 900001, 900002, 900003]
+HELD_OUT = range(learned.TEST_SEED0, learned.TEST_SEED0 + 64)
 
 
 def test_acceptance_rule_falls_back_to_the_plain_run():
@@ -33,29 +34,21 @@ def test_acceptance_rule_falls_back_to_the_plain_run():
 
 
 def test_learned_start_against_the_plain_start():
-    """what the network gains: per problem, the plain run from the prior (the judge)
-    against the learned start (inference + learned run, plus the plain run when the
-    start is rejected).  Every accepted start meets the acceptance rule; on the
-    held-out problems of the training family most starts are accepted and the
-    learned path costs no more than the plain one; the headline problem's outcome
-    is printed (its time to chi2/ndof = 1 varies with the trajectory: 2,760 to
-    4,100 iterations from the learned start, 5,200 to 6,000 from the prior)."""
+    """What the network gains on 64 held-out skies: per problem the plain run from the
+    prior (the judge) against the learned path (inference, the run from the learned
+    start to the judge's objective, plus the plain run when the start is rejected).
+    Every accepted start meets the acceptance rule, and the learned path needs fewer
+    iterations than the plain start at the median over the problems the plain start
+    solves (iterations: deterministic; the wall-clock medians are printed)."""
     net = learned.UNet.load(learned.WEIGHTS)
-    probs = [learned.family_problem(s) for s in HELD_OUT] + [synth.make_problem("cfg2", seed=1)]
-    rows = learned.evaluate(net, None, problems=probs)
+    assert net.nparam > 1_900_000
+    rows = learned.evaluate(net, (p for _, p in learned.problems_ahead(HELD_OUT)))
     for r in rows:
         if r["accepted"]:
             assert r["learned_conv"] and r["learned_J"] <= r["plain_J"] + 1e-4 * abs(r["plain_J"])
         else:
             assert r["reason"]
-    fam = [r for r in rows if r["cfg"] == "cfg5" and r["plain_conv"]]
-    plain = sum(r["plain_s"] for r in fam)
-    cost = sum(r["learned_s"] for r in fam)
-    dJ = [1.0 - r["learned_J"] / r["plain_J"] for r in fam if r["accepted"]]
-    head = rows[-1]
-    print(f"family: {len(fam)} problems, plain {plain:.2f} s, learned path {cost:.2f} s, accepted "
-          f"{sum(r['accepted'] for r in fam)}, objective lower by {np.round(dJ, 4).tolist()}; headline: plain "
-          f"{head['plain_its']} iterations, learned {head['learned_its']} accepted {head['accepted']}", flush=True)
-    assert len(fam) >= 3
-    assert sum(r["accepted"] for r in fam) >= len(fam) - 1
-    assert cost <= 1.1 * plain
+    s = learned.summarize(rows)
+    print(f"held-out: {s}", flush=True)
+    assert s["plain_converged"] >= 32
+    assert s["median_path_its"] < s["median_plain_its"]

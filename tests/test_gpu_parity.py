@@ -295,9 +295,21 @@ def test_gradient_graphs_are_bit_identical():
     assert np.array_equal(outs[0][1], outs[1][1]) and outs[0][2] == outs[1][2]
 
 
-def test_batch_matches_single_runs():
-    """gpair_batch_pcg_run over problems of different widths and uv counts takes, for
-    each problem, exactly the steps of its own gpair_pcg_run."""
+RESULT_FIELDS = ("iterations", "forward_dfts", "adjoint_dfts", "ls_batches", "J", "chi2", "entropy", "chi2r",
+                 "converged", "stalled", "status", "resyncs", "sparse_commits")
+
+
+def result_tuple(r):
+    return tuple(getattr(r, f) for f in RESULT_FIELDS)
+
+
+@pytest.mark.parametrize("streams", [False, True], ids=["grouped", "streams"])
+def test_batch_matches_single_runs(streams):
+    """gpair_batch_pcg_run over problems of different widths and uv counts (one launch
+    sequence per iteration for all of them; streams: one stream per problem) takes,
+    for each problem, exactly the steps of its own gpair_pcg_run: every counter, the
+    objective and the final image agree bit for bit.  The runs to chi2/ndof = 1
+    include exact re-evaluations (resyncs) decided on the device."""
     from gpair_amd.api import batch_pcg_run
     probs = [synth.make_problem("cfg1", seed=1), synth.make_problem("cfg1", seed=2, W=128, ntel=7, nch=6, nsnap=5)]
     probs += synth.batch_problems(3, seed=7, v2_range=(500, 4000))
@@ -307,16 +319,20 @@ def test_batch_matches_single_runs():
         with GpairContext.from_problem(p) as ctx:
             ctx.set_precision(1)
             r, _ = ctx.pcg_run(p.meta["alpha"], precond=1, max_iter=150, chi2r_target=1.0)
-            single.append((r.iterations, r.J, r.converged, ctx.get("ci")))
+            single.append((result_tuple(r), ctx.get("ci")))
     ctxs = [GpairContext.from_problem(p) for p in probs]
-    for c in ctxs:
-        c.set_precision(1)
-        c.set_graphs(True)
-    res = batch_pcg_run(ctxs, [p.meta["alpha"] for p in probs], max_iter=150, chi2r_target=1.0)
-    for (its, J, conv, img), r, c in zip(single, res, ctxs):
-        assert r.iterations == its and r.J == J and bool(r.converged) == bool(conv)
-        assert np.array_equal(c.get("ci"), img)
-        c.close()
+    try:
+        for c in ctxs:
+            c.set_precision(1)
+            c.set_graphs(True)
+        res = batch_pcg_run(ctxs, [p.meta["alpha"] for p in probs], max_iter=150, chi2r_target=1.0,
+                            streams=streams)
+        for (rs, img), r, c in zip(single, res, ctxs):
+            assert result_tuple(r) == rs
+            assert np.array_equal(c.get("ci"), img)
+    finally:
+        for c in ctxs:
+            c.close()
 
 
 def _ft_cases():
