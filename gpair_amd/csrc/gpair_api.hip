@@ -528,12 +528,13 @@ static inline int adj_wide(const gpair_ctx *c) { return ADJ_WIDE && c->W % AW_T 
 static inline int adj_factored(const gpair_ctx *c);
 
 // output tile of the split-precision adjoints on factored tables: the configured one
-// when it divides the width, else 128 x 128 (edge tiles partial when W % 128 != 0)
+// when it divides the width, else 128 x 128 when that does, else 64 x 64
 static inline void adj_tile(const gpair_ctx *c, int *tr, int *tc)
 {
     const int fit = c->W % c->adj_tr == 0 && c->W % c->adj_tc == 0;
-    *tr = fit ? c->adj_tr : 128;
-    *tc = fit ? c->adj_tc : 128;
+    const int t = c->W % 128 == 0 ? 128 : 64;
+    *tr = fit ? c->adj_tr : t;
+    *tc = fit ? c->adj_tc : t;
 }
 
 static inline int adj_tiles(const gpair_ctx *c)
@@ -942,15 +943,17 @@ static int launch_adjoint_dft(gpair_ctx *c, int *nsplit_out)
         const unsigned grid = (unsigned)(adj_tiles(c) * nsplit);
 #define ADJ_H1(TR, TC)                                                                                              \
     TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,                                                                               \
-                 (k_adj_h1wf<TR, TC><<<grid, 512, 0, c->stream>>>(c->coef, c->FX, c->FY, c->nuv, c->W,              \
-                                                                  c->adj_b3_pper, nsplit, c->scal + SC_AMAXC,       \
-                                                                  c->adj_part)))
+                 (k_adj_h1wf<TR, TC><<<grid, ADJ_NTH(TR), 0, c->stream>>>(c->coef, c->FX, c->FY, c->nuv, c->W,      \
+                                                                          c->adj_b3_pper, nsplit,                   \
+                                                                          c->scal + SC_AMAXC, c->adj_part)))
         if (tr == 256 && tc == 256)
             ADJ_H1(256, 256);
         else if (tr == 128 && tc == 256)
             ADJ_H1(128, 256);
-        else
+        else if (tr == 128)
             ADJ_H1(128, 128);
+        else
+            ADJ_H1(64, 64);
 #undef ADJ_H1
     } else if (c->precision == GPAIR_PREC_BF16X3 && (adj_factored(c) || c->W % AB3_XT == 0)) {
         if ((rc = ensure_split_tables(c)))
@@ -962,14 +965,16 @@ static int launch_adjoint_dft(gpair_ctx *c, int *nsplit_out)
             const unsigned grid = (unsigned)(adj_tiles(c) * nsplit);
 #define ADJ_B3(TR, TC)                                                                                              \
     TIMED_LAUNCH(c, GPAIR_TK_ADJOINT,                                                                               \
-                 (k_adj_b3wf<TR, TC><<<grid, 512, 0, c->stream>>>(c->coef, c->FX, c->FY, c->nuv, c->W,              \
-                                                                  c->adj_b3_pper, nsplit, c->adj_part)))
+                 (k_adj_b3wf<TR, TC><<<grid, ADJ_NTH(TR), 0, c->stream>>>(c->coef, c->FX, c->FY, c->nuv, c->W,      \
+                                                                          c->adj_b3_pper, nsplit, c->adj_part)))
             if (tr == 256 && tc == 256)
                 ADJ_B3(256, 256);
             else if (tr == 128 && tc == 256)
                 ADJ_B3(128, 256);
-            else
+            else if (tr == 128)
                 ADJ_B3(128, 128);
+            else
+                ADJ_B3(64, 64);
 #undef ADJ_B3
         } else if (adj_wide(c)) {
             dim3 grid((c->W / AW_T) * (c->W / AW_T), nsplit);

@@ -684,6 +684,8 @@ __global__ __launch_bounds__(256) void k_adj_b3(const float2 *__restrict__ coef,
 #ifndef ADJ_WIDE_GROUPS
 #define ADJ_WIDE_GROUPS 256          // workgroups of k_adj_b3w (quadrants x uv splits)
 #endif
+// threads of the factored adjoints: 8 waves, 4 for the 64 x 64 tile
+#define ADJ_NTH(TR) ((TR) == 64 ? 256 : 512)
 #ifndef ADJ_MIN_PTS
 #define ADJ_MIN_PTS 64               // fewest uv points per split of the split-precision adjoints
 #endif
@@ -2145,8 +2147,11 @@ template <int TR, int TC>
 __device__ __forceinline__ void k_adj_b3wf_body(const int BX, const int GX, const float2 *__restrict__ coef, const float2 *__restrict__ FX,
                                                   const float2 *__restrict__ FY, int nuv, int W, int p_per_split,
                                                   int nsplit, float *__restrict__ part){
-    static_assert((TR == 128 || TR == 256) && (TC == 128 || TC == 256), "tile");
-    constexpr int HA = TR / 128, HB = TC / 128, MT = TR / 128, NT = TC / 64;
+    static_assert(((TR == 128 || TR == 256) && (TC == 128 || TC == 256)) || (TR == 64 && TC == 64), "tile");
+    // ADJ_NTH(TR) threads; builder runs cover RUN rows (columns) per pass; waves in two
+    // columns, each wave TR * 4 / NTH row and TC / 64 column tiles of 32
+    constexpr int NTH = ADJ_NTH(TR), RUN = NTH / 4;
+    constexpr int HA = TR / RUN, HB = TC / RUN, MT = TR * 4 / NTH, NT = TC / 64;
     __shared__ __attribute__((aligned(16))) unsigned short sA[2][2][TR * AW_STRIDE];   // [buf][hi/lo]
     __shared__ __attribute__((aligned(16))) unsigned short sB[2][2][TC * AW_STRIDE];
     constexpr int WROW = AW_STRIDE / 2;   // 32-bit words per LDS row
@@ -2178,10 +2183,10 @@ __device__ __forceinline__ void k_adj_b3wf_body(const int BX, const int GX, cons
         fx23 = fxf[1];
 #pragma unroll
         for (int h = 0; h < HA; h++)
-            cy[h] = fy[(y0 + o4 + 128 * h) >> 4];
+            cy[h] = fy[(y0 + o4 + RUN * h) >> 4];
 #pragma unroll
         for (int h = 0; h < HB; h++)
-            cx[h] = fx[(x0 + o4 + 128 * h) >> 4];
+            cx[h] = fx[(x0 + o4 + RUN * h) >> 4];
     };
     auto store = [&](int buf) {
         unsigned *ah = reinterpret_cast<unsigned *>(sA[buf][0]), *al = reinterpret_cast<unsigned *>(sA[buf][1]);
@@ -2199,7 +2204,7 @@ __device__ __forceinline__ void k_adj_b3wf_body(const int BX, const int GX, cons
                 const float2 w = cmul(acy, fyv[j]);
                 unsigned hh, ll;
                 split2_bf16(w.x, -w.y, hh, ll);
-                const int o = (o4 + 128 * h + j) * WROW + q;
+                const int o = (o4 + RUN * h + j) * WROW + q;
                 ah[o] = hh;
                 al[o] = ll;
             }
@@ -2212,13 +2217,13 @@ __device__ __forceinline__ void k_adj_b3wf_body(const int BX, const int GX, cons
                 const float2 w = cmul(cx[h], fxv[j]);
                 unsigned hh, ll;
                 split2_bf16(w.x, w.y, hh, ll);
-                const int o = (o4 + 128 * h + j) * WROW + q;
+                const int o = (o4 + RUN * h + j) * WROW + q;
                 bh[o] = hh;
                 bl[o] = ll;
             }
         }
     };
-    const int wy = (wave >> 1) * (TR / 4), wx = (wave & 1) * (TC / 2);
+    const int wy = (wave >> 1) * (TR * 128 / NTH), wx = (wave & 1) * (TC / 2);
     v16f acc[MT][NT];
 #pragma unroll
     for (int m = 0; m < MT; m++)
@@ -2289,7 +2294,7 @@ __device__ __forceinline__ void k_adj_b3wf_body(const int BX, const int GX, cons
 }
 
 template <int TR, int TC>
-__global__ __launch_bounds__(512) void k_adj_b3wf(const float2 *__restrict__ coef, const float2 *__restrict__ FX,
+__global__ __launch_bounds__(ADJ_NTH(TR)) void k_adj_b3wf(const float2 *__restrict__ coef, const float2 *__restrict__ FX,
                                                   const float2 *__restrict__ FY, int nuv, int W, int p_per_split,
                                                   int nsplit, float *__restrict__ part)
 {
@@ -2301,6 +2306,7 @@ __global__ __launch_bounds__(512) void k_adj_b3wf(const float2 *__restrict__ coe
 ADJ_TILE_INST(k_adj_b3wf, 256, 256)
 ADJ_TILE_INST(k_adj_b3wf, 128, 256)
 ADJ_TILE_INST(k_adj_b3wf, 128, 128)
+ADJ_TILE_INST(k_adj_b3wf, 64, 64)
 #undef ADJ_TILE_INST
 
 // ---------------------------------------------------------------------------
@@ -2557,13 +2563,16 @@ template __global__ void k_vis_fwd_h1f<8>(const unsigned short *, const float2 *
 // per tile and one MFMA per k-step and tile; the coefficients scaled by
 // f16_scale(*amax) (max over their Re and Im parts) and the partials scaled back
 template <int TR, int TC>
-__global__ __launch_bounds__(512) void k_adj_h1wf(const float2 *__restrict__ coef, const float2 *__restrict__ FX,
+__global__ __launch_bounds__(ADJ_NTH(TR)) void k_adj_h1wf(const float2 *__restrict__ coef, const float2 *__restrict__ FX,
                                                   const float2 *__restrict__ FY, int nuv, int W, int p_per_split,
                                                   int nsplit, const double *__restrict__ amax,
                                                   float *__restrict__ part)
 {
-    static_assert((TR == 128 || TR == 256) && (TC == 128 || TC == 256), "tile");
-    constexpr int HA = TR / 128, HB = TC / 128, MT = TR / 128, NT = TC / 64;
+    static_assert(((TR == 128 || TR == 256) && (TC == 128 || TC == 256)) || (TR == 64 && TC == 64), "tile");
+    // ADJ_NTH(TR) threads; builder runs cover RUN rows (columns) per pass; waves in two
+    // columns, each wave TR * 4 / NTH row and TC / 64 column tiles of 32
+    constexpr int NTH = ADJ_NTH(TR), RUN = NTH / 4;
+    constexpr int HA = TR / RUN, HB = TC / RUN, MT = TR * 4 / NTH, NT = TC / 64;
     __shared__ __attribute__((aligned(16))) unsigned short sA[2][TR * AW_STRIDE];
     __shared__ __attribute__((aligned(16))) unsigned short sB[2][TC * AW_STRIDE];
     constexpr int WROW = AW_STRIDE / 2;
@@ -2594,10 +2603,10 @@ __global__ __launch_bounds__(512) void k_adj_h1wf(const float2 *__restrict__ coe
         fx23 = fxf[1];
 #pragma unroll
         for (int h = 0; h < HA; h++)
-            cy[h] = fy[(y0 + o4 + 128 * h) >> 4];
+            cy[h] = fy[(y0 + o4 + RUN * h) >> 4];
 #pragma unroll
         for (int h = 0; h < HB; h++)
-            cx[h] = fx[(x0 + o4 + 128 * h) >> 4];
+            cx[h] = fx[(x0 + o4 + RUN * h) >> 4];
     };
     auto store = [&](int buf) {
         unsigned *aw = reinterpret_cast<unsigned *>(sA[buf]), *bw = reinterpret_cast<unsigned *>(sB[buf]);
@@ -2611,7 +2620,7 @@ __global__ __launch_bounds__(512) void k_adj_h1wf(const float2 *__restrict__ coe
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const float2 w = cmul(acy, fyv[j]);
-                aw[(o4 + 128 * h + j) * WROW + q] = pack_f16(w.x, -w.y);
+                aw[(o4 + RUN * h + j) * WROW + q] = pack_f16(w.x, -w.y);
             }
         }
 #pragma unroll
@@ -2619,11 +2628,11 @@ __global__ __launch_bounds__(512) void k_adj_h1wf(const float2 *__restrict__ coe
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const float2 w = cmul(cx[h], fxv[j]);
-                bw[(o4 + 128 * h + j) * WROW + q] = pack_f16(w.x, w.y);
+                bw[(o4 + RUN * h + j) * WROW + q] = pack_f16(w.x, w.y);
             }
         }
     };
-    const int wy = (wave >> 1) * (TR / 4), wx = (wave & 1) * (TC / 2);
+    const int wy = (wave >> 1) * (TR * 128 / NTH), wx = (wave & 1) * (TC / 2);
     v16f acc[MT][NT];
 #pragma unroll
     for (int m = 0; m < MT; m++)
@@ -2680,4 +2689,6 @@ template __global__ void k_adj_h1wf<256, 256>(const float2 *, const float2 *, co
 template __global__ void k_adj_h1wf<128, 256>(const float2 *, const float2 *, const float2 *, int, int, int, int,
                                               const double *, float *);
 template __global__ void k_adj_h1wf<128, 128>(const float2 *, const float2 *, const float2 *, int, int, int, int,
+                                              const double *, float *);
+template __global__ void k_adj_h1wf<64, 64>(const float2 *, const float2 *, const float2 *, int, int, int, int,
                                               const double *, float *);

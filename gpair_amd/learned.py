@@ -50,7 +50,7 @@ PF_CLIP = 4.0             # |log pf| <= 4
 # instead of 467 iterations), so it is not applied by default
 BLEND = 0.75
 USE_PF = False
-BUDGET = 6000             # iterations granted to a learned start before the fallback
+BUDGET = 500              # iterations granted to a learned start before the fallback
 
 
 def _p(a):

@@ -241,8 +241,8 @@ int gpair_pcg_run(gpair_ctx *ctx, const gpair_pcg_params *params, gpair_pcg_resu
 int gpair_set_sparse_commit(gpair_ctx *ctx, int max_pixels);
 /* No reference counterpart.  Output tile of the split-precision adjoint on factored
  * tables: rows x cols = 256 x 256, 128 x 256 or 128 x 128 per workgroup where the
- * tile divides W (otherwise 128 x 128, with partial edge tiles when W is not a
- * multiple of 128), with the uv points split so that about `workgroups` workgroups run
+ * tile divides W (otherwise 128 x 128 when that divides W, else 64 x 64), with the
+ * uv points split (at least 64 points per split) so that about `workgroups` workgroups run
  * (0: 256).  Smaller tiles mean fewer uv splits for as many workgroups, and so
  * fewer partial images to add up, at more operand building per product.  The
  * split fixes the summation order: results are reproducible for a given setting,
