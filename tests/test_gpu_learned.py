@@ -26,7 +26,7 @@ def test_acceptance_rule_falls_back_to_the_plain_run():
     with GpairContext.from_problem(prob) as ctx:
         # a judge that no start can beat: the learned result is rejected, the judge returned
         judge = SimpleNamespace(J=-1e30)
-        out = learned.run_with_start(ctx, prob, alpha, prob.start, None, max_iter=3000, judge=judge)
+        out = learned.run_with_start(ctx, prob, alpha, prob.start, None, max_iter=3000, judge=judge, budget=3000)
         assert not out.accepted and out.result is judge and "objective" in out.reason
         # without a judge the fallback is a fresh plain run when the start cannot converge
         out = learned.run_with_start(ctx, prob, alpha, prob.start, None, max_iter=2, judge=None)

@@ -203,6 +203,44 @@ def test_pcg_reaches_target_and_objective_is_exact(orc):
     assert chi2_o / prob.ndof <= 1.0 + 1e-5
 
 
+@pytest.mark.parametrize("grouped", [False, True], ids=["single", "batch"])
+def test_objective_target_goes_on_past_chi2_target(orc, grouped):
+    """J_target (the learned start's acceptance): the loop does not stop at
+    chi2/ndof <= 1 while J is above the target; it goes on (same trajectory, more
+    iterations) until both hold, on an exact evaluation.  The grouped batch path
+    applies the same rule on the device and stops at the same iteration."""
+    from gpair_amd.api import batch_pcg_run
+    prob = synth.make_problem("cfg1", seed=1)
+    alpha = prob.meta["alpha"]
+    with GpairContext.from_problem(prob) as ctx:
+        ctx.set_precision(1)
+        r1, _ = ctx.pcg_run(alpha, precond=1, max_iter=3000, chi2r_target=1.0)
+        ctx.gpu_copy_image(prob.start)
+        target = r1.J * (1.0 - 2e-3)
+        r2, _ = ctx.pcg_run(alpha, precond=1, max_iter=3000, chi2r_target=1.0, J_target=target)
+    assert r1.converged and r2.converged and r2.chi2r <= 1.0 and r2.J <= target
+    assert r2.iterations > r1.iterations
+    if grouped:
+        ctxs = [GpairContext.from_problem(prob) for _ in range(2)]
+        try:
+            L = ctxs[0].L
+            from gpair_amd import _lib
+            import ctypes as C
+            ps = (_lib.PcgParams * 2)()
+            for k, jt in enumerate((0.0, target)):
+                L.gpair_pcg_default_params(C.byref(ps[k]))
+                ps[k].alpha, ps[k].max_iter, ps[k].chi2r_target, ps[k].J_target = alpha, 3000, 1.0, jt
+                ctxs[k].set_precision(1)
+            rs = (_lib.PcgResult * 2)()
+            hs = (C.c_void_p * 2)(*[c.h.value for c in ctxs])
+            assert L.gpair_batch_pcg_run(hs, 2, ps, rs) == 0
+            assert (rs[0].iterations, rs[0].J) == (r1.iterations, r1.J)
+            assert (rs[1].iterations, rs[1].J) == (r2.iterations, r2.J)
+        finally:
+            for c in ctxs:
+                c.close()
+
+
 @pytest.mark.parametrize("name,kw", _cases(), ids=[c[0] for c in _cases()])
 def test_bf16x3_forward(name, kw, orc):
     """split-precision forward DFT (3 BF16 MFMA products, FP32 accumulation)
