@@ -75,13 +75,16 @@ def parse():
     ap.add_argument("--no-target", action="store_true", help="skip the runs to chi2_r = 1")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-start workload")
-    ap.add_argument("--learned-problems", type=int, default=32, help="held-out problems of the learned-start leg")
+    ap.add_argument("--learned-problems", type=int, default=128, help="held-out problems of the learned-start leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch shapes")
     ap.add_argument("--batch", type=int, default=64, help="problems of the batching workload (0: skip)")
     ap.add_argument("--spread-seeds", default="2,3,4",
                     help="further seeds of the headline configuration run to the target with BF16x3 "
                          "(the iteration count to chi2_r = 1 varies with the trajectory)")
     ap.add_argument("--batch-iters", type=int, default=100)
+    ap.add_argument("--batch-modes", default="streams,grouped",
+                    help="batched runs after the sequential one: streams (one stream per problem), grouped "
+                         "(one launch sequence per iteration)")
     ap.add_argument("--moves", type=int, default=4096, help="single-pixel moves scored per call (0: skip)")
     ap.add_argument("--no-cfg3", action="store_true", help="skip the 512x512 precision workload")
     ap.add_argument("--cfg3-iters", type=int, default=300, help="accelerated iterations per precision at 512x512")
@@ -359,7 +362,8 @@ def batch_workload(a):
            "v2_range": [min(p.npow for p in probs), max(p.npow for p in probs)],
            "t3_range": [min(p.nbis for p in probs), max(p.nbis for p in probs)],
            "sequential_s": t_seq, "iters_per_s_sequential": its / t_seq, "setup_generation_s": t_gen}
-    for mode in ("streams", "grouped"):
+    modes = [m for m in a.batch_modes.split(",") if m in ("streams", "grouped")]
+    for mode in modes:
         for c, s0 in zip(ctxs, starts):
             c.gpu_copy_image(s0)
             c.set_graphs(True)
@@ -374,10 +378,12 @@ def batch_workload(a):
             c.set_graphs(False)
     for c in ctxs:
         c.close()
-    out["batch_s"], out["speedup"] = out["grouped"]["batch_s"], out["grouped"]["speedup"]
-    print(f"# batch cfg4: {len(probs)} problems x {K} its: sequential {t_seq:.2f} s, one stream each "
-          f"{out['streams']['batch_s']:.2f} s, one launch sequence per iteration {out['grouped']['batch_s']:.2f} s "
-          f"({out['grouped']['identical_to_sequential']}/{len(probs)} bit-identical)", file=sys.stderr, flush=True)
+    main_mode = "grouped" if "grouped" in out else modes[-1] if modes else None
+    if main_mode:
+        out["batch_s"], out["speedup"] = out[main_mode]["batch_s"], out[main_mode]["speedup"]
+    print(f"# batch cfg4: {len(probs)} problems x {K} its: sequential {t_seq:.2f} s, "
+          + ", ".join(f"{m} {out[m]['batch_s']:.2f} s ({out[m]['identical_to_sequential']}/{len(probs)} "
+                      "bit-identical)" for m in modes), file=sys.stderr, flush=True)
     return out
 
 
@@ -526,9 +532,11 @@ def main():
     if a.warmup > 0:
         ctx.pcg_run(alpha, precond=1, max_iter=a.warmup, chi2r_target=0.0)
     L.gpair_sync(ctx.h)
-    # HIP events around every 8th DFT launch of the timed region: events around every
-    # launch cost the loop about 7 % (4,250 against 4,568 iterations/s at 256²)
-    L.gpair_timing_enable(ctx.h, TIMING_EVERY)
+    # HIP events around every 8th DFT launch of the timed region (every 2nd in a run of
+    # fewer than 160 steps, for at least about ten samples): events around every launch
+    # cost the loop about 7 % (4,250 against 4,568 iterations/s at 256²)
+    every = TIMING_EVERY if a.steps >= 160 else 2
+    L.gpair_timing_enable(ctx.h, every)
     t0 = time.perf_counter()
     res, tr = ctx.pcg_run(alpha, precond=1, max_iter=a.steps, chi2r_target=0.0)
     L.gpair_sync(ctx.h)
@@ -562,7 +570,7 @@ def main():
         roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK[a.precision], "unit": "TFLOP/s",
                     "frac": achieved / PEAK[a.precision]}
     roofline.update({"traffic": traffic, "kernel": f"{kname} (forward DFT)", "avg_us": avg_fwd * 1e6,
-                     "launches": fwd_n, "timed_every": TIMING_EVERY, "flop_per_launch": flop,
+                     "launches": fwd_n, "timed_every": every, "flop_per_launch": flop,
                      "bytes_per_launch": nbytes, "flop_per_byte": intensity,
                      "mfma_pipe_tflops": achieved * PRODUCTS[a.precision],
                      "mfma_pipe_frac": achieved * PRODUCTS[a.precision] / PEAK[a.precision]})
@@ -589,7 +597,7 @@ def main():
                    "prior": list(prob.meta["prior"]), "seed": a.seed, "precision": a.precision},
         "roofline": roofline,
         "adjoint": {"kernel": aname, "avg_us": adj_ms / max(adj_n, 1) * 1e3, "launches": adj_n,
-                    "timed_every": TIMING_EVERY,
+                    "timed_every": every,
                     "achieved_tflops": flop / (adj_ms / max(adj_n, 1) * 1e-3) / 1e12 if adj_n else None,
                     # HBM bytes per launch from the committed rocprofv3 --pmc summary: the adjoint
                     # (its split-K partials included) and the kernel that adds the partials up
