@@ -987,12 +987,17 @@ __global__ void k_vis_to_double(const float2 *__restrict__ vis, int nuv, double2
 #define LS_SPAN (2 * RED_THREADS)
 #define LS_HALF ((LS_K / LS_G) * RED_PARTS * LS_G)   // doubles of partials per half
 
-// chi2(s_k) - chi2(0) from V = U / F in FP64 and the direction's DFT U_d:
-// V(s) = (U + s U_d) / (F + s F_d).  Per item the numerator is a polynomial in s
-// (|U + s U_d|^2 for a V2 point, the product of three such legs times the phasor
-// for a T3), expanded once and evaluated per candidate by Horner's rule; the
-// candidates' s_k and powers of 1 / (F + s_k F_d) are wave-uniform (lane k computes
-// candidate k's, the others read them from scalar registers).
+// chi2(s_k) - chi2(0) from V0 = U / F in FP64 and the direction's DFT U_d.  With
+// V(s) = (U + s U_d) / (F + s F_d) = V0 + sigma D, sigma = s / (F + s F_d) and
+// D = U_d - F_d V0, every model term moves by a polynomial in sigma without a
+// constant term: e (|V(s)|^2 - |V0|^2) = sigma (a1 + sigma a2) for a V2 point, and
+// the change of a T3 (the product of three legs V0_j + sigma D_j, conjugated where
+// the triangle says, times the phasor) is sigma (c1 + sigma (c2 + sigma c3)).  A
+// residual r0 then changes by delta, and r^2 - r0^2 = delta (delta + 2 r0): the
+// change itself is evaluated, not a difference of two squares (whose rounding would
+// swamp the small changes of the short steps).  Coefficients once per item, Horner
+// per candidate; the candidates' sigma_k are wave-uniform (lane k computes
+// candidate k's, the others read it from scalar registers).
 __device__ __forceinline__ void ls_chi2_part(const double2 *__restrict__ Vd, const float2 *__restrict__ Ud,
                                              const double *__restrict__ scal, double s_ref,
                                              const float *__restrict__ data, const float *__restrict__ err,
@@ -1002,66 +1007,62 @@ __device__ __forceinline__ void ls_chi2_part(const double2 *__restrict__ Vd, con
     const int g0 = grp * LS_G;
     const double FI = scal[SC_FLUX], Fd = scal[SC_FD];
     const double s_l = ls_step(s_ref, g0 + (int)(threadIdx.x & (LS_G - 1)));
-    const double iv_l = 1.0 / (FI + s_l * Fd);
-    const double iv2_l = iv_l * iv_l, iv3_l = iv2_l * iv_l;
-    double sk[LS_G], iv2[LS_G], iv3[LS_G];
+    const double sg_l = s_l / (FI + s_l * Fd);
+    double sg[LS_G];
 #pragma unroll
-    for (int k = 0; k < LS_G; k++) {
-        sk[k] = readlane_d(s_l, k);
-        iv2[k] = readlane_d(iv2_l, k);
-        iv3[k] = readlane_d(iv3_l, k);
-    }
-    const double iF = 1.0 / FI, iF3 = iF * iF * iF;
+    for (int k = 0; k < LS_G; k++)
+        sg[k] = readlane_d(sg_l, k);
     auto item = [&](int i, double (&v)[LS_G]) {
         if (i < npow) {
             const double2 a = Vd[i];
-            const double bx = Ud[i].x, by = Ud[i].y, d0 = data[i], e = err[i];
-            const double r0 = (a.x * a.x + a.y * a.y - d0) * e;
-            const double fx = FI * a.x, fy = FI * a.y;
-            // |U + s U_d|^2 = c0 + s (c1 + s c2)
-            const double c0 = fx * fx + fy * fy, c1 = 2.0 * (fx * bx + fy * by), c2 = bx * bx + by * by;
+            const float2 u = Ud[i];
+            const double d0 = data[i], e = err[i];
+            const double dx = (double)u.x - Fd * a.x, dy = (double)u.y - Fd * a.y;
+            const double t0 = 2.0 * (a.x * a.x + a.y * a.y - d0) * e;   // 2 r0
+            const double a1 = 2.0 * e * (a.x * dx + a.y * dy), a2 = e * (dx * dx + dy * dy);
 #pragma unroll
             for (int k = 0; k < LS_G; k++) {
-                const double q = fma(fma(c2, sk[k], c1), sk[k], c0);
-                const double r = (q * iv2[k] - d0) * e;
-                v[k] += r * r - r0 * r0;
+                const double dl = sg[k] * fma(a2, sg[k], a1);
+                v[k] = fma(dl, dl + t0, v[k]);
             }
         } else {
             const int t = i - npow;
             const int4 tr = tri[t];
             const int idx[3] = {tr.x, tr.y, tr.z};
-            double2 A[3], B[3];   // legs A_j + s B_j (conjugated where the triangle says)
+            double2 A[3], B[3];   // legs A_j + sigma B_j
 #pragma unroll
             for (int j = 0; j < 3; j++) {
                 const double2 a = Vd[idx[j]];
                 const float2 u = Ud[idx[j]];
+                const double dx = (double)u.x - Fd * a.x, dy = (double)u.y - Fd * a.y;
                 const bool cj = (tr.w >> j) & 1;
-                A[j] = make_double2(FI * a.x, cj ? -(FI * a.y) : FI * a.y);
-                B[j] = make_double2((double)u.x, cj ? -(double)u.y : (double)u.y);
+                A[j] = make_double2(a.x, cj ? -a.y : a.y);
+                B[j] = make_double2(dx, cj ? -dy : dy);
             }
             const double2 rho = make_double2(phasor[t].x, phasor[t].y);
             A[2] = dcmul(A[2], rho);
             B[2] = dcmul(B[2], rho);
-            // (A0 + s B0)(A1 + s B1) = p0 + s p1 + s^2 p2, times (A2 + s B2) = c0 + s c1 + s^2 c2 + s^3 c3
+            // (A0 + sigma B0)(A1 + sigma B1) = p0 + sigma p1 + sigma^2 p2, times (A2 + sigma B2)
             const double2 p0 = dcmul(A[0], A[1]), p2 = dcmul(B[0], B[1]);
             const double2 p1a = dcmul(A[0], B[1]), p1b = dcmul(B[0], A[1]);
             const double2 p1 = make_double2(p1a.x + p1b.x, p1a.y + p1b.y);
             const double2 c0 = dcmul(p0, A[2]), c3 = dcmul(p2, B[2]);
             const double2 c1a = dcmul(p0, B[2]), c1b = dcmul(p1, A[2]);
             const double2 c2a = dcmul(p1, B[2]), c2b = dcmul(p2, A[2]);
-            const double2 c1 = make_double2(c1a.x + c1b.x, c1a.y + c1b.y);
-            const double2 c2 = make_double2(c2a.x + c2b.x, c2a.y + c2b.y);
             const int kk = npow + 2 * t;
-            const double dr = data[kk], di = data[kk + 1], er = err[kk], ei = err[kk + 1];
-            const double q0x = (c0.x * iF3 - dr) * er, q0y = (c0.y * iF3 - di) * ei;
-            const double q0 = q0x * q0x + q0y * q0y;
+            const double er = err[kk], ei = err[kk + 1];
+            // coefficients of the residuals' changes (weights folded in)
+            const double c1x = (c1a.x + c1b.x) * er, c1y = (c1a.y + c1b.y) * ei;
+            const double c2x = (c2a.x + c2b.x) * er, c2y = (c2a.y + c2b.y) * ei;
+            const double c3x = c3.x * er, c3y = c3.y * ei;
+            const double t0x = 2.0 * (c0.x - (double)data[kk]) * er, t0y = 2.0 * (c0.y - (double)data[kk + 1]) * ei;
 #pragma unroll
             for (int k = 0; k < LS_G; k++) {
-                const double s = sk[k];
-                const double tx = fma(fma(fma(c3.x, s, c2.x), s, c1.x), s, c0.x);
-                const double ty = fma(fma(fma(c3.y, s, c2.y), s, c1.y), s, c0.y);
-                const double rx = (tx * iv3[k] - dr) * er, ry = (ty * iv3[k] - di) * ei;
-                v[k] += rx * rx + ry * ry - q0;
+                const double s = sg[k];
+                const double hx = fma(fma(c3x, s, c2x), s, c1x) * s;
+                const double hy = fma(fma(c3y, s, c2y), s, c1y) * s;
+                v[k] = fma(hx, hx + t0x, v[k]);
+                v[k] = fma(hy, hy + t0y, v[k]);
             }
         }
     };
@@ -1093,9 +1094,15 @@ __device__ __forceinline__ void ls_entropy_part(const float *__restrict__ img, c
             if (t < (float)minval)
                 t = (float)minval;
             const float dl = t - I, q = dl * invI;
-            // ln(t/I): log1p while the change is small, log of the ratio once t is far
-            // below I (there dl rounds to -I and log1p(-1) would be -inf)
-            const float lr = q > -0.5f ? log1pf(q) : logf(t * invI);
+            // ln(t/I) = log1p(q).  While the change is small, with u = 1 + q rounded,
+            // log(u) q / (u - 1): u - 1 is exact, and the quotient takes the rounding of
+            // u out to first order (within 4 ulp of log1p, at a tenth of its
+            // instructions).  Once t is far below I (q <= -0.5, where dl rounds to -I)
+            // the log of the ratio itself.
+            const float u = 1.0f + q;
+            const bool near = q > -0.5f;
+            const float lx = __logf(near ? u : t * invI);
+            const float lr = !near ? lx : (u == 1.0f ? q : lx * (q * __builtin_amdgcn_rcpf(u - 1.0f)));
             v[k] += (double)(dl * (1.0f - L) - t * lr);
         }
     };
