@@ -37,6 +37,7 @@ struct gpair_ctx {
     float2 *phasor;
     int4 *tri;
     int *csr_ptr, *csr_ent;
+    float2 *t3c;   // per T3 leg: its term of the adjoint coefficient of its uv point
     float2 *X, *Y;
     float2 *FX, *FY;   // factored tables (coarse then fine factors per point), from gpair_set_uv
     int have_ft;
@@ -268,6 +269,7 @@ extern "C" int gpu_cleanup(gpair_ctx *c)
     free_images(c);
     px_free(c);
     dfree(&c->data); dfree(&c->err); dfree(&c->phasor); dfree(&c->tri); dfree(&c->csr_ptr); dfree(&c->csr_ent);
+    dfree(&c->t3c);
     dfree(&c->X); dfree(&c->Y); dfree(&c->FX); dfree(&c->FY); dfree(&c->vis); dfree(&c->coef); dfree(&c->mock);
     dfree(&c->scal); dfree(&c->invflux); dfree(&c->part); dfree(&c->ticket); dfree(&c->adj_part);
     dfree(&c->Ud); dfree(&c->Udel); dfree(&c->pfac);
@@ -391,6 +393,7 @@ extern "C" int gpu_copy_data(gpair_ctx *c, const float *data, const float *data_
     if ((rc = dalloc(c, &c->data, ndof)) || (rc = dalloc(c, &c->err, ndof)) || (rc = dalloc(c, &c->mock, ndof)) ||
         (rc = dalloc(c, &c->phasor, nbis)) || (rc = dalloc(c, &c->tri, nbis)) ||
         (rc = dalloc(c, &c->csr_ptr, nuv + 1)) || (rc = dalloc(c, &c->csr_ent, 3 * (size_t)nbis)) ||
+        (rc = dalloc(c, &c->t3c, 3 * (size_t)nbis)) ||
         (rc = dalloc(c, &c->vis, nuv)) || (rc = dalloc(c, &c->coef, nuv)) || (rc = dalloc(c, &c->Ud, nuv)) ||
         (rc = dalloc(c, &c->Udel, nuv)) || (rc = dalloc(c, &c->vis2, nuv)) || (rc = dalloc(c, &c->Vd, nuv)) || (rc = dalloc(c, &c->Vn, nuv)) ||
         (rc = dalloc(c, &c->mock2, ndof)))
@@ -907,12 +910,25 @@ static int launch_adjoint_dft(gpair_ctx *c, int *nsplit_out);
 
 // per-uv adjoint coefficients from the visibilities and mock data of the last chi2
 // (k_coef) and the flux of img (as launch_flux) in one launch, with the same results
+static int launch_t3_contrib(gpair_ctx *c)
+{
+    if (c->nbis > 0) {
+        k_t3_contrib<<<ew_blocks(c->nbis), 256, 0, c->stream>>>(c->vis, c->mock, c->data, c->err, c->phasor, c->tri,
+                                                               c->npow, c->nbis, c->t3c);
+        LAUNCHCHK(c);
+    }
+    return GPAIR_OK;
+}
+
 static int launch_coef_flux(gpair_ctx *c, const float *img)
 {
+    int rc;
+    if ((rc = launch_t3_contrib(c)))
+        return rc;
     const int nbc = tuned_blocks(c->coef_blocks, red_nparts(c->nuv, COEF_SPAN)), nbf = red_blocks(c->n);
-    k_coef_flux<<<nbc + nbf, RED_THREADS, 0, c->stream>>>(c->vis, c->mock, c->data, c->err, c->phasor, c->tri,
-                                                           c->csr_ptr, c->csr_ent, c->npow, c->nuv, c->coef, nbc, img,
-                                                           c->n, c->part, c->ticket, c->scal, c->invflux);
+    k_coef_flux<<<nbc + nbf, RED_THREADS, 0, c->stream>>>(c->vis, c->mock, c->data, c->err, c->t3c, c->csr_ptr,
+                                                           c->csr_ent, c->npow, c->nuv, c->coef, nbc, img, c->n,
+                                                           c->part, c->ticket, c->scal, c->invflux);
     LAUNCHCHK(c);
     return GPAIR_OK;
 }

@@ -271,17 +271,51 @@ __global__ __launch_bounds__(RED_THREADS) void k_model_chi2(const float2 *__rest
 //                  a_p += b, or conj(b) for a conjugated leg
 // c0 = Re sum_p a_p V_p
 // ---------------------------------------------------------------------------
-// one uv point per thread and partition (partitions of 512 points, two per thread,
-// took 18.8 µs against 15.4 µs for the fused launch at 256²; four lanes per point,
-// each with a quarter of its legs, 16.6 µs)
+// Two launches: k_t3_contrib, one thread per T3, writes the three legs' terms b
+// (contrib[3 t + j]), each gathering its three visibilities once; then coef_part,
+// one uv point per thread and partition, adds its V2 term and its legs' b in CSR
+// order.  The loads of a point's entries are independent of each other, where the
+// single gather pass waited on csr -> tri -> vis for every leg in turn.
 #define COEF_SPAN RED_THREADS
+
+__device__ __forceinline__ void k_t3_contrib_body(const int BX, const int GX, const float2 *__restrict__ vis,
+                                                  const float *__restrict__ mock, const float *__restrict__ data,
+                                                  const float *__restrict__ err, const float2 *__restrict__ phasor,
+                                                  const int4 *__restrict__ tri, int npow, int nbis,
+                                                  float2 *__restrict__ contrib)
+{
+    for (int t = BX * 256 + (int)threadIdx.x; t < nbis; t += GX * 256) {
+        const int4 tr = tri[t];
+        const float2 V[3] = {leg_vis(vis, tr.x, tr.w & 1), leg_vis(vis, tr.y, tr.w & 2), leg_vis(vis, tr.z, tr.w & 4)};
+        const int k = npow + 2 * t;
+        const float er = err[k], ei = err[k + 1];
+        const float2 ct = make_float2(2.0f * er * er * (mock[k] - data[k]), -2.0f * ei * ei * mock[k + 1]);
+        const float2 cr = cmul(ct, phasor[t]);
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            float2 b = cmul(cr, V[(j + 1) % 3]);
+            b = cmul(b, V[(j + 2) % 3]);
+            if ((tr.w >> j) & 1)
+                b = cconj(b);
+            contrib[3 * t + j] = b;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_t3_contrib(const float2 *__restrict__ vis, const float *__restrict__ mock,
+                                                    const float *__restrict__ data, const float *__restrict__ err,
+                                                    const float2 *__restrict__ phasor, const int4 *__restrict__ tri,
+                                                    int npow, int nbis, float2 *__restrict__ contrib)
+{
+    k_t3_contrib_body((int)blockIdx.x, (int)gridDim.x, vis, mock, data, err, phasor, tri, npow, nbis, contrib);
+}
 
 __device__ __forceinline__ void coef_part(const float2 *__restrict__ vis, const float *__restrict__ mock,
                                           const float *__restrict__ data, const float *__restrict__ err,
-                                          const float2 *__restrict__ phasor, const int4 *__restrict__ tri,
-                                          const int *__restrict__ csr_ptr, const int *__restrict__ csr_ent, int npow,
-                                          int nuv, float2 *__restrict__ coef, double *part, unsigned *ticket,
-                                          double *scal, int bx, int nbx)
+                                          const float2 *__restrict__ contrib, const int *__restrict__ csr_ptr,
+                                          const int *__restrict__ csr_ent, int npow, int nuv,
+                                          float2 *__restrict__ coef, double *part, unsigned *ticket, double *scal,
+                                          int bx, int nbx)
 {
     auto item = [&](int p, double (&v)[1]) {
         const float2 Vp = vis[p];
@@ -291,20 +325,10 @@ __device__ __forceinline__ void coef_part(const float2 *__restrict__ vis, const 
             const float w = 4.0f * e * e * (mock[p] - data[p]);
             a = make_float2(w * Vp.x, -w * Vp.y);
         }
-        for (int q = csr_ptr[p]; q < csr_ptr[p + 1]; q++) {
-            const int ent = csr_ent[q];
-            const int t = ent / 3, j = ent - 3 * t;
-            const int4 tr = tri[t];
-            const int idx[3] = {tr.x, tr.y, tr.z};
-            const int k = npow + 2 * t;
-            const float er = err[k], ei = err[k + 1];
-            const float2 ct = make_float2(2.0f * er * er * (mock[k] - data[k]), -2.0f * ei * ei * mock[k + 1]);
-            const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
-            float2 b = cmul(ct, phasor[t]);
-            b = cmul(b, leg_vis(vis, idx[j1], (tr.w >> j1) & 1));
-            b = cmul(b, leg_vis(vis, idx[j2], (tr.w >> j2) & 1));
-            if ((tr.w >> j) & 1)
-                b = cconj(b);
+        const int q0 = csr_ptr[p], q1 = csr_ptr[p + 1];
+#pragma unroll 4
+        for (int q = q0; q < q1; q++) {   // loads in flight four at a time, added in order
+            const float2 b = contrib[csr_ent[q]];
             a.x += b.x;
             a.y += b.y;
         }
@@ -318,14 +342,13 @@ __global__ __launch_bounds__(RED_THREADS) void k_coef(const float2 *__restrict__
                                                       const float *__restrict__ mock,
                                                       const float *__restrict__ data,
                                                       const float *__restrict__ err,
-                                                      const float2 *__restrict__ phasor,
-                                                      const int4 *__restrict__ tri,
+                                                      const float2 *__restrict__ contrib,
                                                       const int *__restrict__ csr_ptr,
                                                       const int *__restrict__ csr_ent, int npow, int nuv,
                                                       float2 *__restrict__ coef, double *part, unsigned *ticket,
                                                       double *scal)
 {
-    coef_part(vis, mock, data, err, phasor, tri, csr_ptr, csr_ent, npow, nuv, coef, part, ticket, scal,
+    coef_part(vis, mock, data, err, contrib, csr_ptr, csr_ent, npow, nuv, coef, part, ticket, scal,
               (int)blockIdx.x, (int)gridDim.x);
 }
 
@@ -336,8 +359,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_coef_flux(const float2 *__restr
                                                            const float *__restrict__ mock,
                                                            const float *__restrict__ data,
                                                            const float *__restrict__ err,
-                                                           const float2 *__restrict__ phasor,
-                                                           const int4 *__restrict__ tri,
+                                                           const float2 *__restrict__ contrib,
                                                            const int *__restrict__ csr_ptr,
                                                            const int *__restrict__ csr_ent, int npow, int nuv,
                                                            float2 *__restrict__ coef, int nb_coef,
@@ -346,7 +368,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_coef_flux(const float2 *__restr
 {
     const int bx = (int)blockIdx.x;
     if (bx < nb_coef)
-        coef_part(vis, mock, data, err, phasor, tri, csr_ptr, csr_ent, npow, nuv, coef, part, ticket, scal, bx,
+        coef_part(vis, mock, data, err, contrib, csr_ptr, csr_ent, npow, nuv, coef, part, ticket, scal, bx,
                   nb_coef);
     else
         flux_part(img, n, bx - nb_coef, (int)gridDim.x - nb_coef, part + RED_PARTS, ticket + 1, scal, invflux);
@@ -1154,7 +1176,8 @@ __device__ __forceinline__ void k_ls_choose_body(const int BX, const int GX, con
         const double *p = part + (size_t)half * LS_HALF + (size_t)(c / LS_G) * RED_PARTS * LS_G + c % LS_G;
         const int np = half ? np_ent : np_chi2;
         double acc = 0.0;
-        for (int q = sub; q < np; q += LS_FIN_SUB)
+#pragma unroll 4
+        for (int q = sub; q < np; q += LS_FIN_SUB)   // loads in flight four at a time
             acc += p[(size_t)q * LS_G];
 #pragma unroll
         for (int o = LS_FIN_SUB / 2; o > 0; o >>= 1)
@@ -1766,12 +1789,13 @@ struct B3Stage {
 // use); a block of this kernel owns 64 uv points (lane = point) with their factor
 // rows staged in LDS, and each of its CS_WAVES waves walks a contiguous share of the list
 // (one pixel at a time, the same pixel across the wave; entries hold x | y << 16 and
-// the correction; the list is read 64 entries
-// per load and broadcast lane by lane), rebuilding X and Y as coarse x fine factors
-// and Y only when the row changes.  The waves' sums are added in wave order: the
-// result depends on the list only.  With more than cap pixels the kernel writes
-// nothing but *dense = 1, and the full DFT of the change launched behind it runs
-// (it exits at once when *dense = 0).
+// the correction; the list is read 64 entries per load, the next 64 in flight while
+// these are summed, and broadcast lane by lane).  X and Y are coarse x fine factors:
+// a run of pixels of one row under one coarse X factor sums its corrections times
+// the fine factors and multiplies by the coarse factor once, a row by Y once.  The
+// waves' sums are added in wave order: the result depends on the list only.  With
+// more than cap pixels the kernel writes nothing but *dense = 1, and the full DFT of
+// the change launched behind it runs (it exits at once when *dense = 0).
 #define CS_PTS 64
 // soff[b] = sum of ccount[0 .. b-1] for b <= ncb (<= 128), by one wave: an inclusive
 // scan of two counts per lane (a serial loop would wait on 128 dependent loads)
@@ -1837,9 +1861,8 @@ __device__ __forceinline__ void k_commit_sparse_body(const int BX, const int GX,
     const int j0 = (int)((long long)total * wave / CS_WAVES), j1 = (int)((long long)total * (wave + 1) / CS_WAVES);
     float2 acc = make_float2(0.f, 0.f), row = make_float2(0.f, 0.f), yv = make_float2(0.f, 0.f);
     int prev_y = -1;
-    for (int jb = j0; jb < j1; jb += 64) {
-        // this lane's entry of the next 64: its block by a search over the offsets
-        const int j = jb + lane;
+    // the list entry at position j (this lane's): its block by a search over the offsets
+    auto fetch = [&](int j) -> int2 {
         int2 en = make_int2(0, 0);
         if (j < j1) {
             int lo = 0, hi = ncb;   // soff[lo] <= j < soff[hi]
@@ -1852,34 +1875,45 @@ __device__ __forceinline__ void k_commit_sparse_body(const int BX, const int GX,
             }
             en = clist[(size_t)lo * cper + (j - soff[lo])];
         }
+        return en;
+    };
+    // consecutive entries of one row that share a coarse X factor form a run: the run
+    // sums c_j times the fine factor and takes the coarse factor once at its end;
+    // rows take their Y factor once
+    float2 bacc = make_float2(0.f, 0.f), xc = make_float2(0.f, 0.f);
+    int prev_key = -1;   // y << 8 | coarse block of the current run
+    int2 en = fetch(j0 + lane);
+    for (int jb = j0; jb < j1; jb += 64) {
+        const int2 en_next = fetch(jb + 64 + lane);   // in flight while this chunk is summed
         const int cnt = min(64, j1 - jb);
-        // the factors of pixel k are read one pixel ahead (each wave's LDS latency
-        // overlaps the previous pixel's arithmetic); Y only when the row changes
-        int xy = __builtin_amdgcn_readlane(en.x, 0);
-        float2 fxc = rx[(xy & 0xffff) >> 4], fxf = rx[nco + (xy & 15)];
         for (int k = 0; k < cnt; k++) {
-            const int y = xy >> 16;
+            const int xy = __builtin_amdgcn_readlane(en.x, k);
             const float cj = __int_as_float(__builtin_amdgcn_readlane(en.y, k));
-            const float2 xc = fxc, xf = fxf;
-            if (k + 1 < cnt) {
-                xy = __builtin_amdgcn_readlane(en.x, k + 1);
-                fxc = rx[(xy & 0xffff) >> 4];
-                fxf = rx[nco + (xy & 15)];
-            }
-            if (y != prev_y) {
-                if (prev_y >= 0) {
-                    acc.x += row.x * yv.x - row.y * yv.y;
-                    acc.y += row.x * yv.y + row.y * yv.x;
+            const int y = xy >> 16, x = xy & 0xffff, key = (y << 8) | (x >> 4);
+            if (key != prev_key) {
+                row.x += bacc.x * xc.x - bacc.y * xc.y;
+                row.y += bacc.x * xc.y + bacc.y * xc.x;
+                bacc = make_float2(0.f, 0.f);
+                if (y != prev_y) {
+                    if (prev_y >= 0) {
+                        acc.x += row.x * yv.x - row.y * yv.y;
+                        acc.y += row.x * yv.y + row.y * yv.x;
+                    }
+                    yv = cmul(ry[y >> 4], ry[nco + (y & 15)]);
+                    row = make_float2(0.f, 0.f);
+                    prev_y = y;
                 }
-                yv = cmul(ry[y >> 4], ry[nco + (y & 15)]);
-                row = make_float2(0.f, 0.f);
-                prev_y = y;
+                xc = rx[x >> 4];
+                prev_key = key;
             }
-            const float2 xv = cmul(xc, xf);
-            row.x += cj * xv.x;
-            row.y += cj * xv.y;
+            const float2 xf = rx[nco + (x & 15)];
+            bacc.x += cj * xf.x;
+            bacc.y += cj * xf.y;
         }
+        en = en_next;
     }
+    row.x += bacc.x * xc.x - bacc.y * xc.y;
+    row.y += bacc.x * xc.y + bacc.y * xc.x;
     if (prev_y >= 0) {
         acc.x += row.x * yv.x - row.y * yv.y;
         acc.y += row.x * yv.y + row.y * yv.x;
