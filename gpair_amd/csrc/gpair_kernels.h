@@ -46,7 +46,7 @@ enum {
 // batched line search: LS_K candidate steps, s_0 = 0 and s_k = s_ref * 2^((k - LS_TOP) / 2)
 #define LS_K 32
 #define LS_TOP 26
-#define LS_G 8   // candidates per block row of the line-search kernels (grid.y = LS_K / LS_G)
+#define LS_G 8   // line-search candidates per wave of the chi2 half and per block row of the entropy half
 
 // The accelerated loop's read-back region, one device allocation read back in one
 // copy: the scalars, the line-search sums, then the commit kernels' per-block

@@ -999,13 +999,13 @@ __global__ void k_vis_to_double(const float2 *__restrict__ vis, int nuv, double2
     k_vis_to_double_body((int)blockIdx.x, (int)gridDim.x, vis, nuv, Vd);
 }
 
-// Batched line search, one launch per candidate batch: block columns
-// [0, nb_chi2) take the chi2 change of every candidate, columns [nb_chi2,
-// nb_chi2 + nb_ent) the entropy change; block row y takes candidates
-// LS_G y .. LS_G y + LS_G - 1 (fewer live registers per lane).  Both halves store
-// the sums of fixed partitions of LS_SPAN items (part: [2][LS_K / LS_G][RED_PARTS][LS_G],
-// chi2 half first), so that neither depends on nb_chi2 / nb_ent; k_ls_choose adds
-// them up in a fixed order.
+// Batched line search, one launch per candidate batch: blocks [0, nb_chi2) take the
+// chi2 change of every candidate, one partition at a time; then LS_K / LS_G rows of
+// nb_ent blocks the entropy change, row y candidates LS_G y .. LS_G y + LS_G - 1
+// (fewer live registers per lane).  Both halves store the sums of fixed partitions
+// of LS_SPAN items (part: [2][LS_K / LS_G][RED_PARTS][LS_G], chi2 half first), so
+// that neither depends on nb_chi2 / nb_ent; k_ls_choose adds them up in a fixed
+// order.
 #define LS_SPAN (2 * RED_THREADS)
 #define LS_HALF ((LS_K / LS_G) * RED_PARTS * LS_G)   // doubles of partials per half
 
@@ -1017,78 +1017,116 @@ __global__ void k_vis_to_double(const float2 *__restrict__ vis, int nuv, double2
 // the triangle says, times the phasor) is sigma (c1 + sigma (c2 + sigma c3)).  A
 // residual r0 then changes by delta, and r^2 - r0^2 = delta (delta + 2 r0): the
 // change itself is evaluated, not a difference of two squares (whose rounding would
-// swamp the small changes of the short steps).  Coefficients once per item, Horner
-// per candidate; the candidates' sigma_k are wave-uniform (lane k computes
-// candidate k's, the others read it from scalar registers).
+// swamp the small changes of the short steps).  One block per partition and all 32
+// candidates: the block builds each item's coefficients once (weights folded in)
+// into LDS, LS_SPAN items at a time, then wave w evaluates candidates LS_G w ..
+// LS_G w + LS_G - 1 over them by Horner's rule, its lanes taking every 64th item
+// in order; the candidates' sigma_k are wave-uniform (lane k computes candidate k's,
+// the others read it from scalar registers).  Each wave's sums are its group's
+// partials: no block reduction.
 __device__ __forceinline__ void ls_chi2_part(const double2 *__restrict__ Vd, const float2 *__restrict__ Ud,
                                              const double *__restrict__ scal, double s_ref,
                                              const float *__restrict__ data, const float *__restrict__ err,
                                              const float2 *__restrict__ phasor, const int4 *__restrict__ tri,
-                                             int npow, int nbis, int grp, int bx, int nbx, double *part)
+                                             int npow, int nbis, int bx, int nbx, double *part)
 {
-    const int g0 = grp * LS_G;
+    static_assert(RED_THREADS / 64 * LS_G == LS_K, "one candidate group per wave");
+    // per item (c1x, c1y, c2x, c2y), (c3x, c3y, 2 r0x, 2 r0y); a V2 point (a1, a2, 2 r0, 0)
+    __shared__ double4 cf[LS_SPAN][2];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const double FI = scal[SC_FLUX], Fd = scal[SC_FD];
-    const double s_l = ls_step(s_ref, g0 + (int)(threadIdx.x & (LS_G - 1)));
+    const double s_l = ls_step(s_ref, wave * LS_G + (lane & (LS_G - 1)));
     const double sg_l = s_l / (FI + s_l * Fd);
     double sg[LS_G];
 #pragma unroll
     for (int k = 0; k < LS_G; k++)
         sg[k] = readlane_d(sg_l, k);
-    auto item = [&](int i, double (&v)[LS_G]) {
-        if (i < npow) {
-            const double2 a = Vd[i];
-            const float2 u = Ud[i];
-            const double d0 = data[i], e = err[i];
-            const double dx = (double)u.x - Fd * a.x, dy = (double)u.y - Fd * a.y;
-            const double t0 = 2.0 * (a.x * a.x + a.y * a.y - d0) * e;   // 2 r0
-            const double a1 = 2.0 * e * (a.x * dx + a.y * dy), a2 = e * (dx * dx + dy * dy);
+    const int n = npow + nbis;
+    const int nparts = red_nparts(n, LS_SPAN);
+    const int span = (n + nparts - 1) / nparts;
+    for (int q = bx; q < nparts; q += nbx) {
+        double v[LS_G];
 #pragma unroll
-            for (int k = 0; k < LS_G; k++) {
-                const double dl = sg[k] * fma(a2, sg[k], a1);
-                v[k] = fma(dl, dl + t0, v[k]);
-            }
-        } else {
-            const int t = i - npow;
-            const int4 tr = tri[t];
-            const int idx[3] = {tr.x, tr.y, tr.z};
-            double2 A[3], B[3];   // legs A_j + sigma B_j
+        for (int k = 0; k < LS_G; k++)
+            v[k] = 0.0;
+        const int pend = (q + 1) * span < n ? (q + 1) * span : n;
+        for (int beg = q * span; beg < pend; beg += LS_SPAN) {
+            const int end = beg + LS_SPAN < pend ? beg + LS_SPAN : pend;
+            for (int i = beg + tid; i < end; i += RED_THREADS) {
+                double4 x0, x1 = make_double4(0.0, 0.0, 0.0, 0.0);
+                if (i < npow) {
+                    const double2 a = Vd[i];
+                    const float2 u = Ud[i];
+                    const double d0 = data[i], e = err[i];
+                    const double dx = (double)u.x - Fd * a.x, dy = (double)u.y - Fd * a.y;
+                    x0 = make_double4(2.0 * e * (a.x * dx + a.y * dy), e * (dx * dx + dy * dy),
+                                      2.0 * (a.x * a.x + a.y * a.y - d0) * e, 0.0);
+                } else {
+                    const int t = i - npow;
+                    const int4 tr = tri[t];
+                    const int idx[3] = {tr.x, tr.y, tr.z};
+                    double2 A[3], B[3];   // legs A_j + sigma B_j
 #pragma unroll
-            for (int j = 0; j < 3; j++) {
-                const double2 a = Vd[idx[j]];
-                const float2 u = Ud[idx[j]];
-                const double dx = (double)u.x - Fd * a.x, dy = (double)u.y - Fd * a.y;
-                const bool cj = (tr.w >> j) & 1;
-                A[j] = make_double2(a.x, cj ? -a.y : a.y);
-                B[j] = make_double2(dx, cj ? -dy : dy);
+                    for (int j = 0; j < 3; j++) {
+                        const double2 a = Vd[idx[j]];
+                        const float2 u = Ud[idx[j]];
+                        const double dx = (double)u.x - Fd * a.x, dy = (double)u.y - Fd * a.y;
+                        const bool cj = (tr.w >> j) & 1;
+                        A[j] = make_double2(a.x, cj ? -a.y : a.y);
+                        B[j] = make_double2(dx, cj ? -dy : dy);
+                    }
+                    const double2 rho = make_double2(phasor[t].x, phasor[t].y);
+                    A[2] = dcmul(A[2], rho);
+                    B[2] = dcmul(B[2], rho);
+                    // (A0 + sigma B0)(A1 + sigma B1) = p0 + sigma p1 + sigma^2 p2, times (A2 + sigma B2)
+                    const double2 p0 = dcmul(A[0], A[1]), p2 = dcmul(B[0], B[1]);
+                    const double2 p1a = dcmul(A[0], B[1]), p1b = dcmul(B[0], A[1]);
+                    const double2 p1 = make_double2(p1a.x + p1b.x, p1a.y + p1b.y);
+                    const double2 c0 = dcmul(p0, A[2]), c3 = dcmul(p2, B[2]);
+                    const double2 c1a = dcmul(p0, B[2]), c1b = dcmul(p1, A[2]);
+                    const double2 c2a = dcmul(p1, B[2]), c2b = dcmul(p2, A[2]);
+                    const int kk = npow + 2 * t;
+                    const double er = err[kk], ei = err[kk + 1];
+                    x0 = make_double4((c1a.x + c1b.x) * er, (c1a.y + c1b.y) * ei, (c2a.x + c2b.x) * er,
+                                      (c2a.y + c2b.y) * ei);
+                    x1 = make_double4(c3.x * er, c3.y * ei, 2.0 * (c0.x - (double)data[kk]) * er,
+                                      2.0 * (c0.y - (double)data[kk + 1]) * ei);
+                }
+                cf[i - beg][0] = x0;
+                cf[i - beg][1] = x1;
             }
-            const double2 rho = make_double2(phasor[t].x, phasor[t].y);
-            A[2] = dcmul(A[2], rho);
-            B[2] = dcmul(B[2], rho);
-            // (A0 + sigma B0)(A1 + sigma B1) = p0 + sigma p1 + sigma^2 p2, times (A2 + sigma B2)
-            const double2 p0 = dcmul(A[0], A[1]), p2 = dcmul(B[0], B[1]);
-            const double2 p1a = dcmul(A[0], B[1]), p1b = dcmul(B[0], A[1]);
-            const double2 p1 = make_double2(p1a.x + p1b.x, p1a.y + p1b.y);
-            const double2 c0 = dcmul(p0, A[2]), c3 = dcmul(p2, B[2]);
-            const double2 c1a = dcmul(p0, B[2]), c1b = dcmul(p1, A[2]);
-            const double2 c2a = dcmul(p1, B[2]), c2b = dcmul(p2, A[2]);
-            const int kk = npow + 2 * t;
-            const double er = err[kk], ei = err[kk + 1];
-            // coefficients of the residuals' changes (weights folded in)
-            const double c1x = (c1a.x + c1b.x) * er, c1y = (c1a.y + c1b.y) * ei;
-            const double c2x = (c2a.x + c2b.x) * er, c2y = (c2a.y + c2b.y) * ei;
-            const double c3x = c3.x * er, c3y = c3.y * ei;
-            const double t0x = 2.0 * (c0.x - (double)data[kk]) * er, t0y = 2.0 * (c0.y - (double)data[kk + 1]) * ei;
+            __syncthreads();
+            const int cnt = end - beg, nv = npow - beg;   // items [0, nv) of the chunk are V2 points
+            for (int j = lane; j < cnt; j += 64) {
+                const double4 x0 = cf[j][0];
+                if (j < nv) {
 #pragma unroll
-            for (int k = 0; k < LS_G; k++) {
-                const double s = sg[k];
-                const double hx = fma(fma(c3x, s, c2x), s, c1x) * s;
-                const double hy = fma(fma(c3y, s, c2y), s, c1y) * s;
-                v[k] = fma(hx, hx + t0x, v[k]);
-                v[k] = fma(hy, hy + t0y, v[k]);
+                    for (int k = 0; k < LS_G; k++) {
+                        const double dl = sg[k] * fma(x0.y, sg[k], x0.x);
+                        v[k] = fma(dl, dl + x0.z, v[k]);
+                    }
+                } else {
+                    const double4 x1 = cf[j][1];
+#pragma unroll
+                    for (int k = 0; k < LS_G; k++) {
+                        const double s = sg[k];
+                        const double hx = fma(fma(x1.x, s, x0.z), s, x0.x) * s;
+                        const double hy = fma(fma(x1.y, s, x0.w), s, x0.y) * s;
+                        v[k] = fma(hx, hx + x1.z, v[k]);
+                        v[k] = fma(hy, hy + x1.w, v[k]);
+                    }
+                }
             }
+            __syncthreads();   // the next chunk overwrites cf
         }
-    };
-    partitioned_partials<LS_G>(npow + nbis, LS_SPAN, item, part + (size_t)grp * RED_PARTS * LS_G, bx, nbx);
+#pragma unroll
+        for (int k = 0; k < LS_G; k++) {
+            const double t = wave_sum(v[k]);
+            if (lane == 0)
+                part[((size_t)wave * RED_PARTS + q) * LS_G + k] = t;
+        }
+    }
 }
 
 // S(max(I + s_k d, minval)) - S(I), summed over the pixels
@@ -1142,12 +1180,18 @@ __global__ __launch_bounds__(RED_THREADS) void k_ls_batch(const double2 *__restr
                                                           const float *__restrict__ model, double minval, int n,
                                                           int nb_chi2, int nb_ent, double *part)
 {
-    const int grp = blockIdx.y, bx = blockIdx.x;
-    if (bx < nb_chi2)
-        ls_chi2_part(Vd, Ud, scal, s_ref, data, err, phasor, tri, npow, nbis, grp, bx, nb_chi2, part);
-    else
-        ls_entropy_part(img, d, model, s_ref, minval, n, grp, bx - nb_chi2, nb_ent, part + LS_HALF);
+    const int b = (int)blockIdx.x;
+    if (b < nb_chi2) {
+        ls_chi2_part(Vd, Ud, scal, s_ref, data, err, phasor, tri, npow, nbis, b, nb_chi2, part);
+    } else {
+        const int e = b - nb_chi2, grp = e / nb_ent;
+        ls_entropy_part(img, d, model, s_ref, minval, n, grp, e - grp * nb_ent, nb_ent, part + LS_HALF);
+    }
 }
+
+// blocks of a line-search batch launch: nb_chi2 for the chi2 change, then LS_K / LS_G
+// rows of nb_ent for the entropy change
+__host__ __device__ inline int ls_grid(int nb_chi2, int nb_ent) { return nb_chi2 + (LS_K / LS_G) * nb_ent; }
 
 // block columns of each half of k_ls_batch: one per partition unless a smaller count is given
 __host__ inline int ls_nparts_chi2(int npow, int nbis) { return red_nparts(npow + nbis, LS_SPAN); }
