@@ -486,7 +486,10 @@ def main(argv=None):
     import argparse
     import json
     ap = argparse.ArgumentParser(prog="python -m gpair_amd.learned")
-    ap.add_argument("mode", choices=["all", "eval"])
+    ap.add_argument("mode", choices=["all", "more", "eval"],
+                    help="all: generate, train from scratch, evaluate; more: go on training the saved "
+                         "network on --n new problems from --train-seed0; eval: evaluate the saved network")
+    ap.add_argument("--train-seed0", type=int, default=TRAIN_SEED0, help="first training seed (mode more)")
     ap.add_argument("--n", type=int, default=4000, help="training problems")
     ap.add_argument("--n-val", type=int, default=128, help="validation problems")
     ap.add_argument("--n-test", type=int, default=512, help="held-out problems")
@@ -505,6 +508,43 @@ def main(argv=None):
         print(m, flush=True)
 
     report = {}
+    if a.mode == "more":
+        # further training on fresh problems of the same family: the generated data
+        # cannot outlive one job, so a larger training set is seen in rounds; the
+        # record keeps every round's seeds and kept reconstructions
+        prev = np.load(a.out, allow_pickle=False)
+        t0 = time.perf_counter()
+        data = generate(range(a.train_seed0, a.train_seed0 + a.n), log=log)
+        val = generate(range(VAL_SEED0, VAL_SEED0 + a.n_val), log=log)
+        t_gen = time.perf_counter() - t0
+        log(f"data: {len(data)} training and {len(val)} validation reconstructions in {t_gen:.0f} s")
+        net = UNet.load(a.out, max_batch=a.batch)
+        t0 = time.perf_counter()
+        curve = train(net, data, val, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.train_seed0, log=log)
+        t_train = time.perf_counter() - t0
+        net.params = from_bf16_bits(to_bf16_bits(net.params))
+        trials = {}
+        for b in (float(v) for v in a.blends.split(",")):
+            probs = [family_problem(s) for s in range(VAL_SEED0, VAL_SEED0 + a.n_blend)]
+            trials[b] = summarize(evaluate(net, probs, blend=b, log=lambda m: None))
+            log(f"blend {b}: {trials[b]}")
+        blend = min(trials, key=lambda b: (trials[b]["median_learned_s"], -trials[b]["accepted"]))
+        rnd = int(prev["rounds"]) + 1 if "rounds" in prev.files else 2
+        info = np.concatenate([prev["train_info"], data.info])
+        # learning curve rows: round, epoch, steps, training loss, validation loss
+        old_curve = prev["curve"] if prev["curve"].shape[1] == 5 else np.concatenate(
+            [np.ones((len(prev["curve"]), 1)), prev["curve"]], axis=1)
+        new_curve = np.concatenate([np.full((len(curve), 1), float(rnd)), np.array(curve, np.float64)], axis=1)
+        np.savez(a.out, params_bf16=to_bf16_bits(net.params), **{k: np.int64(v) for k, v in net.arch.items()},
+                 blend=np.float64(blend), n_train=np.int64(len(info)), n_val=np.int64(len(val)),
+                 epochs=np.int64(int(prev["epochs"]) + a.epochs), batch=np.int64(a.batch), lr=np.float64(a.lr),
+                 curve=np.concatenate([old_curve, new_curve]), train_info=info,
+                 gen_s=np.float64(float(prev["gen_s"]) + t_gen), train_s=np.float64(float(prev["train_s"]) + t_train),
+                 rounds=np.int64(rnd))
+        log(f"saved {a.out} (round {rnd}: {len(info)} reconstructions in all, blend {blend})")
+        report.update(n_train=len(info), round=rnd, gen_s=t_gen, train_s=t_train, curve=curve,
+                      blend_trials={str(k): v for k, v in trials.items()}, blend=blend, params=net.nparam)
+        net.close()
     if a.mode == "all":
         t0 = time.perf_counter()
         data = generate(range(TRAIN_SEED0, TRAIN_SEED0 + a.n), log=log)
