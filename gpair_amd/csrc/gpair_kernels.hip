@@ -1839,7 +1839,9 @@ struct B3Stage {
 // the fine factors and multiplies by the coarse factor once, a row by Y once.  The
 // waves' sums are added in wave order: the result depends on the list only.  With
 // more than cap pixels the kernel writes nothing but *dense = 1, and the full DFT of
-// the change launched behind it runs (it exits at once when *dense = 0).
+// the change runs instead: in a batch of problems the one launched behind it (it
+// exits at once when *dense = 0), for a single problem the one the host launches when
+// the read-back shows the count.
 #define CS_PTS 64
 // soff[b] = sum of ccount[0 .. b-1] for b <= ncb (<= 128), by one wave: an inclusive
 // scan of two counts per lane (a serial loop would wait on 128 dependent loads)
