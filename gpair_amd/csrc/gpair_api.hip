@@ -363,6 +363,8 @@ extern "C" int gpu_copy_data(gpair_ctx *c, const float *data, const float *data_
         return fail(c, GPAIR_E_ARG,
                     "gpu_copy_data: inconsistent sizes (data %d, nuv %d, npow %d, nbis %d, phasor %d, image %d)",
                     data_size, nuv, npow, nbis, phasor_size, image_size);
+    if (npow + nbis == 0)   // chi2 / ndof and the step scales need at least one datum
+        return fail(c, GPAIR_E_ARG, "gpu_copy_data: no V2 and no T3 data");
     // pack T3 references: int4 (ab, bc, ca, conj bits) and the uv -> leg CSR
     std::vector<int4> tri(nbis > 0 ? nbis : 1);
     std::vector<int> cnt(nuv + 1, 0);

@@ -56,7 +56,8 @@ int gpu_init(gpair_ctx **ctx, int device);
  * init_data does (gpair1/trunk/gpair.c:997-1066); phasor = (re, im) float pairs
  * [nbis]; uvpnt = 4 x int64 per T3 (cl_long4, gpair1/trunk/gpair.c:585-592);
  * sign = 4 x int16 per T3 (cl_short4); default_model [image_width^2].
- * data_size = npow + 2*bsref_size, data_size_uv = nuv, phasor_size = bsref_size = nbis. */
+ * data_size = npow + 2*bsref_size, data_size_uv = nuv, phasor_size = bsref_size = nbis.
+ * Either of npow and nbis may be 0 (V2 only, T3 only), not both (GPAIR_E_ARG). */
 int gpu_copy_data(gpair_ctx *ctx, const float *data, const float *data_err, int data_size, int data_size_uv,
                   const float *phasor, int phasor_size, int pow_size, const int64_t *uvpnt, const int16_t *sign,
                   int bsref_size, const float *default_model, int image_size, int image_width);
