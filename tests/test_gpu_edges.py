@@ -4,7 +4,7 @@ every uv point reached through the triangles), sliced from the 64^2 workload.  T
 objective and gradient in FP32 and BF16x3 against the FP64 oracle with the bounds of
 test_gpu_parity.py; the accelerated loop against orc_pcg (FP32, the same rules:
 equal DFT and batch counts); the grouped batch over a mix of them bit-identical to
-the single runs; a data set with neither is refused."""
+the single runs; a data set with neither, and an unsupported width, are refused."""
 import dataclasses
 
 import numpy as np
@@ -110,5 +110,18 @@ def test_no_data_is_refused():
         with pytest.raises(GpairError, match="no V2 and no T3"):
             ctx.gpu_copy_data(p.data[:0], p.err[:0], p.nuv, p.phasor[:0], 0, p.uvpnt4()[:0], p.sign4()[:0],
                               p.model, p.W)
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("W", [32, 100, 320, 1024])
+def test_unsupported_width_is_refused(W):
+    """widths outside 64, 128, 192, 256, 512 fail with an error, before any launch."""
+    p = synth.make_problem("cfg1", seed=1)
+    ctx = GpairContext(0)
+    try:
+        with pytest.raises(GpairError, match="width"):
+            ctx.gpu_copy_data(p.data, p.err, p.nuv, p.phasor, p.npow, p.uvpnt4(), p.sign4(),
+                              np.ones(W * W, np.float32), W)
     finally:
         ctx.close()
