@@ -1031,8 +1031,10 @@ __device__ __forceinline__ void ls_chi2_part(const double2 *__restrict__ Vd, con
                                              int npow, int nbis, int bx, int nbx, double *part)
 {
     static_assert(RED_THREADS / 64 * LS_G == LS_K, "one candidate group per wave");
-    // per item (c1x, c1y, c2x, c2y), (c3x, c3y, 2 r0x, 2 r0y); a V2 point (a1, a2, 2 r0, 0)
-    __shared__ double4 cf[LS_SPAN][2];
+    // per item (c1x, c1y, c2x, c2y) in cf0, (c3x, c3y, 2 r0x, 2 r0y) in cf1; a V2 point
+    // (a1, a2, 2 r0, 0) in cf0.  Two arrays, not one of pairs: 32-byte lane strides keep
+    // the 16-byte LDS accesses of eight lanes on distinct banks
+    __shared__ double4 cf0[LS_SPAN], cf1[LS_SPAN];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const double FI = scal[SC_FLUX], Fd = scal[SC_FD];
@@ -1093,13 +1095,13 @@ __device__ __forceinline__ void ls_chi2_part(const double2 *__restrict__ Vd, con
                     x1 = make_double4(c3.x * er, c3.y * ei, 2.0 * (c0.x - (double)data[kk]) * er,
                                       2.0 * (c0.y - (double)data[kk + 1]) * ei);
                 }
-                cf[i - beg][0] = x0;
-                cf[i - beg][1] = x1;
+                cf0[i - beg] = x0;
+                cf1[i - beg] = x1;
             }
             __syncthreads();
             const int cnt = end - beg, nv = npow - beg;   // items [0, nv) of the chunk are V2 points
             for (int j = lane; j < cnt; j += 64) {
-                const double4 x0 = cf[j][0];
+                const double4 x0 = cf0[j];
                 if (j < nv) {
 #pragma unroll
                     for (int k = 0; k < LS_G; k++) {
@@ -1107,7 +1109,7 @@ __device__ __forceinline__ void ls_chi2_part(const double2 *__restrict__ Vd, con
                         v[k] = fma(dl, dl + x0.z, v[k]);
                     }
                 } else {
-                    const double4 x1 = cf[j][1];
+                    const double4 x1 = cf1[j];
 #pragma unroll
                     for (int k = 0; k < LS_G; k++) {
                         const double s = sg[k];
@@ -1118,7 +1120,7 @@ __device__ __forceinline__ void ls_chi2_part(const double2 *__restrict__ Vd, con
                     }
                 }
             }
-            __syncthreads();   // the next chunk overwrites cf
+            __syncthreads();   // the next chunk overwrites cf0, cf1
         }
 #pragma unroll
         for (int k = 0; k < LS_G; k++) {
